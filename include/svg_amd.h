@@ -1,0 +1,162 @@
+/*
+ * svg_amd.h -- C ABI of the MI355X Saddle Vertex Graph engine.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles); every entry point
+ * returns 0 on success or a negative SVG_E* code, with a message available
+ * from svg_last_error() (per thread).  Nothing calls exit().
+ *
+ * Each entry point replaces the reference interface cited beside it
+ * (files of Narusaki/SVG).  Handles own all device memory; output arrays are
+ * owned by the caller.
+ */
+#ifndef SVG_AMD_H
+#define SVG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVG_OK 0
+#define SVG_E_ARG -1        /* bad argument */
+#define SVG_E_HIP -2        /* HIP runtime error */
+#define SVG_E_MEM -3        /* out of device or host memory */
+#define SVG_E_IO -4         /* file error */
+#define SVG_E_STATE -5      /* handle in the wrong state */
+#define SVG_E_BUSY -6       /* serving: admission refused (memory / queue full) */
+
+#define SVG_MODE_PLAIN 0    /* one source per thread, reference layout (SVG.cu:9-96) */
+#define SVG_MODE_BATCHED 1  /* persistent work-stealing lanes + wave extraction */
+
+typedef struct svg_mesh_s *svg_mesh_t;
+typedef struct svg_graph_s *svg_graph_t;
+typedef struct svg_server_s *svg_server_t;
+
+/* Library / device.  Replaces cudaSetDevice(0) of kernel.cu:25. */
+int svg_init(int device);
+const char *svg_last_error(void);
+int svg_device_name(char *buf, int cap);
+
+/* ---------------------------------------------------------------- mesh */
+/* Mesh::LoadFromFile (Mesh.cu:24-126) from arrays: xyz[3*nv] (f64),
+ * tri[3*nf] (0-based, counter-clockwise).  Builds half-edges, twins,
+ * angle sums and firstEdge with the reference rules; open meshes use
+ * decisions D1/D2/D8/D9 (boundary twin -1). */
+int svg_mesh_from_arrays(const double *xyz, int nv, const int32_t *tri, int nf, svg_mesh_t *out);
+/* Mesh::LoadFromFile (Mesh.cu:24-126): 'v x y z' and 'f a b c' lines. */
+int svg_mesh_load_obj(const char *path, svg_mesh_t *out);
+/* Mesh::copyToGPU (Mesh.cu:128-142).  Called implicitly by svg_build. */
+int svg_mesh_upload(svg_mesh_t m);
+int svg_mesh_info(svg_mesh_t m, int *nv, int *nf, int *ne);
+/* Host half-edge arrays: twin[ne], len[ne], angle[nv], firstEdge[nv],
+ * boundary[nv] (any may be NULL). */
+int svg_mesh_export(svg_mesh_t m, int32_t *twin, double *len, double *angle, int32_t *firstEdge,
+                    uint8_t *boundary);
+/* Mesh::clear + Mesh::clearGPU (Mesh.cu:144-159). */
+int svg_mesh_free(svg_mesh_t m);
+
+/* ---------------------------------------------------------------- build */
+typedef struct {
+    int K;                  /* SVG::SetParameters K (SVG.cu:114-117) */
+    uint32_t split_coef;    /* splitInfoCoef, reference value 10 (kernel.cu:422) */
+    uint32_t vert_coef;     /* vertInfoCoef, reference value 2 */
+    int mode;               /* SVG_MODE_PLAIN or SVG_MODE_BATCHED */
+    int plain_blocks;       /* plain launch shape; 0 -> 64 x 128 (SVG.cuh:10-11) */
+    int plain_threads;
+    int lanes;              /* batched: propagation lanes in flight (0 = auto) */
+    int prop_block;         /* batched: threads per propagation block (0 = auto) */
+    int chunk;              /* batched: sources with resident tables (0 = auto) */
+    int vert_cap;           /* batched: vertex-table slots per source (0 = auto) */
+    int split_cap;          /* batched: split-table slots per source (0 = auto) */
+    int autotune;           /* batched: search lanes / block on a source sample */
+} svg_build_params;
+
+typedef struct {
+    double seconds;         /* device time of the build kernels (HIP events) */
+    double prop_seconds;
+    double extract_seconds;
+    double redo_seconds;
+    double alloc_seconds;
+    long long arcs;
+    int redone;             /* sources recomputed on the exact plain path */
+    int flagged_drop;       /* sources whose heap dropped a push (as the reference does) */
+    int flagged_overflow;   /* sources whose reference tables would overflow */
+    int flagged_error;
+    int lanes, prop_block, chunk, vert_cap, split_cap;
+    long long counters[16]; /* event counters (win push/pop, pseudo push/pop, propagate,
+                               children, split/vert writes, fan half-edges, pseudo
+                               neighbours, -, -, backtrace hops, reached, -, -) */
+} svg_build_stats;
+
+void svg_build_params_default(svg_build_params *p, int K);
+/* SVG::SetParameters + Allocation + ConstructSVG + Free (SVG.cu:109-182). */
+int svg_build(svg_mesh_t m, const svg_build_params *p, svg_graph_t *out);
+int svg_build_stats_get(svg_graph_t g, svg_build_stats *st);
+/* Per-vertex status bits of the build (1 drop, 2 pseudo drop, 4 reference
+ * table overflow, 8 batched capacity, 16 error, 32 recomputed exactly). */
+int svg_build_status(svg_graph_t g, uint32_t *status);
+
+/* ---------------------------------------------------------------- graph */
+int svg_graph_info(svg_graph_t g, int *V, int *K, long long *arcs);
+/* SVG::CopySVGToHost (SVG.cu:184-206): tails[V]; rows in K-stride layout
+ * (any array may be NULL): adj[V*K], dist[V*K], src_edge[V*K],
+ * dst_edge[V*K], src_x[V*K], dst_x[V*K] -- the SVGNode fields (SVG.cuh:21-27). */
+int svg_graph_to_host(svg_graph_t g, int32_t *tails, int32_t *adj, double *dist, int32_t *src_edge,
+                      int32_t *dst_edge, double *src_x, double *dst_x);
+/* SVG::SaveSVGToFile / LoadSVGFromFile (SVG.cu:208-239): per vertex an int32
+ * tail then tail 40-byte SVGNode records. */
+int svg_graph_save(svg_graph_t g, const char *path);
+int svg_graph_load(svg_mesh_t m, int K, const char *path, svg_graph_t *out);
+/* SVG::FreeSVGStructure (SVG.cu:152-158). */
+int svg_graph_free(svg_graph_t g);
+
+/* ---------------------------------------------------------------- queries */
+/* SVG::SolveSSSD(s, t) (SVG.cu:241-251): A* distance s -> t on the graph;
+ * path (optional, cap entries) receives t, parent(t), ..., s. */
+int svg_sssd(svg_graph_t g, int s, int t, double *dist_t, int32_t *path, int path_cap, int *path_len);
+/* SVG::SolveMSAD (SVG.cu:354-367): multi-source Dijkstra with initial
+ * weights; dist[V], src_id[V], parent[V] (src_id / parent may be NULL). */
+int svg_msad(svg_graph_t g, const int32_t *src, const double *w, int ns, double *dist,
+             int32_t *src_id, int32_t *parent);
+/* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */
+int svg_sssd_batch(svg_graph_t g, const int32_t *st, int nq, double *out);
+
+/* ---------------------------------------------------------------- serving */
+typedef struct {
+    int max_inflight;       /* request slots resident on the device */
+    long long mem_budget;   /* bytes of device memory for request state (0 = auto) */
+    int max_batch;          /* requests merged into one frontier launch */
+} svg_server_cfg;
+
+typedef struct {
+    int kind;               /* 0 = SSSD (A*), 1 = MSAD */
+    int s, t;               /* SSSD endpoints */
+    const int32_t *src;     /* MSAD sources (copied at submit) */
+    const double *w;        /* MSAD weights (copied at submit) */
+    int ns;
+    double *dist_out;       /* SSSD: 1 value; MSAD: V values (caller-owned) */
+    int32_t *srcid_out;     /* MSAD: V values or NULL */
+} svg_request;
+
+typedef struct {
+    long long completed, refused;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double qps;
+    int max_batch_seen;
+    long long launches;
+} svg_server_stats;
+
+int svg_server_start(svg_graph_t g, const svg_server_cfg *cfg, svg_server_t *out);
+/* Enqueue a request; returns a ticket >= 0 or SVG_E_BUSY. */
+long long svg_server_submit(svg_server_t s, const svg_request *req);
+/* Block until ticket completes; latency in milliseconds from submit. */
+int svg_server_wait(svg_server_t s, long long ticket, double *latency_ms);
+int svg_server_stats_get(svg_server_t s, svg_server_stats *st);
+int svg_server_stop(svg_server_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVG_AMD_H */

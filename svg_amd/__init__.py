@@ -1,0 +1,214 @@
+"""MI355X Saddle Vertex Graph engine: host-side mirror of the reference API.
+
+``Mesh`` and ``SVG`` keep the reference's method names (Mesh.cuh:32-52,
+SVG.cuh:49-82) so callers of the construct-then-solve path switch over
+unchanged; every method runs the HIP library through the C ABI
+(include/svg_amd.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (SVG_MODE_BATCHED, SVG_MODE_PLAIN, BuildParams, BuildStats, COUNTER_NAMES,
+                   SvgError, _ptr, check, lib)
+
+__all__ = ["Mesh", "SVG", "SvgError", "SVG_MODE_PLAIN", "SVG_MODE_BATCHED", "device_name"]
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().svg_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class Mesh:
+    """Half-edge mesh (Mesh.cuh:32-52)."""
+
+    def __init__(self):
+        self.h = None
+
+    @classmethod
+    def from_arrays(cls, xyz, tri) -> "Mesh":
+        m = cls()
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tri = np.ascontiguousarray(tri, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().svg_mesh_from_arrays(xyz.ravel(), xyz.shape[0], tri.ravel(), tri.shape[0],
+                                         C.byref(h)))
+        m.h = h
+        return m
+
+    def LoadFromFile(self, fileName: str) -> bool:            # Mesh.cu:24-126
+        self.clear()
+        h = C.c_void_p()
+        rc = lib().svg_mesh_load_obj(fileName.encode(), C.byref(h))
+        if rc < 0:
+            return False
+        self.h = h
+        return True
+
+    def copyToGPU(self) -> bool:                             # Mesh.cu:128-142
+        check(lib().svg_mesh_upload(self.h))
+        return True
+
+    def _info(self):
+        nv, nf, ne = C.c_int(), C.c_int(), C.c_int()
+        check(lib().svg_mesh_info(self.h, C.byref(nv), C.byref(nf), C.byref(ne)))
+        return nv.value, nf.value, ne.value
+
+    @property
+    def vertNum(self) -> int:
+        return self._info()[0]
+
+    @property
+    def faceNum(self) -> int:
+        return self._info()[1]
+
+    @property
+    def edgeNum(self) -> int:
+        return self._info()[2]
+
+    def export(self):
+        nv, nf, ne = self._info()
+        twin = np.empty(ne, np.int32)
+        ln = np.empty(ne, np.float64)
+        ang = np.empty(nv, np.float64)
+        first = np.empty(nv, np.int32)
+        bnd = np.empty(nv, np.uint8)
+        check(lib().svg_mesh_export(self.h, twin, ln, ang, first, bnd))
+        return dict(twin=twin, len=ln, angle=ang, firstEdge=first, boundary=bnd)
+
+    def clear(self):                                        # Mesh.cu:144-159
+        if self.h:
+            lib().svg_mesh_free(self.h)
+            self.h = None
+
+    clearGPU = clear
+
+    def __del__(self):
+        try:
+            self.clear()
+        except Exception:
+            pass
+
+
+class SVG:
+    """Saddle Vertex Graph (SVG.cuh:49-82)."""
+
+    def __init__(self):
+        self.mesh = None
+        self.g = None
+        self.K = 0
+        self.splitInfoCoef = 10
+        self.vertInfoCoef = 2
+
+    def AssignMesh(self, mesh: Mesh, d_mesh: Mesh | None = None):     # SVG.cu:109-112
+        self.mesh = mesh
+
+    def SetParameters(self, K: int, splitInfoCoef: int = 10, vertInfoCoef: int = 2):
+        self.K, self.splitInfoCoef, self.vertInfoCoef = int(K), int(splitInfoCoef), int(vertInfoCoef)
+
+    def Allocation(self) -> bool:
+        """Scratch is sized and allocated inside ConstructSVG (SVG.cu:119-140)."""
+        return True
+
+    def ConstructSVG(self, mode: int = SVG_MODE_BATCHED, **knobs) -> "SVG":   # SVG.cu:160-182
+        if self.mesh is None or self.K <= 0:
+            raise SvgError("AssignMesh and SetParameters must precede ConstructSVG")
+        self.FreeSVGStructure()
+        p = BuildParams()
+        lib().svg_build_params_default(C.byref(p), self.K)
+        p.split_coef = self.splitInfoCoef
+        p.vert_coef = self.vertInfoCoef
+        p.mode = mode
+        for k, v in knobs.items():
+            setattr(p, k, int(v))
+        g = C.c_void_p()
+        check(lib().svg_build(self.mesh.h, C.byref(p), C.byref(g)))
+        self.g = g
+        return self
+
+    def Free(self):
+        """Build scratch is released by ConstructSVG itself (SVG.cu:142-150)."""
+
+    def FreeSVGStructure(self):                              # SVG.cu:152-158
+        if self.g:
+            lib().svg_graph_free(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.FreeSVGStructure()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ info
+    def info(self):
+        V, K, arcs = C.c_int(), C.c_int(), C.c_longlong()
+        check(lib().svg_graph_info(self.g, C.byref(V), C.byref(K), C.byref(arcs)))
+        return V.value, K.value, arcs.value
+
+    def stats(self) -> dict:
+        st = BuildStats()
+        check(lib().svg_build_stats_get(self.g, C.byref(st)))
+        d = {f: getattr(st, f) for f, _ in BuildStats._fields_ if f != "counters"}
+        d["counters"] = dict(zip(COUNTER_NAMES, list(st.counters)))
+        return d
+
+    def status(self) -> np.ndarray:
+        V, _, _ = self.info()
+        out = np.empty(V, np.uint32)
+        check(lib().svg_build_status(self.g, out))
+        return out
+
+    def CopySVGToHost(self) -> dict:                          # SVG.cu:184-206
+        V, K, _ = self.info()
+        out = dict(tails=np.empty(V, np.int32), adj=np.empty(V * K, np.int32),
+                   dist=np.empty(V * K, np.float64), src_edge=np.empty(V * K, np.int32),
+                   dst_edge=np.empty(V * K, np.int32), src_x=np.empty(V * K, np.float64),
+                   dst_x=np.empty(V * K, np.float64))
+        check(lib().svg_graph_to_host(self.g, *(_ptr(out[k]) for k in
+                                                ("tails", "adj", "dist", "src_edge", "dst_edge",
+                                                 "src_x", "dst_x"))))
+        for k in ("adj", "dist", "src_edge", "dst_edge", "src_x", "dst_x"):
+            out[k] = out[k].reshape(V, K)
+        return out
+
+    def SaveSVGToFile(self, fileName: str):                   # SVG.cu:208-217
+        check(lib().svg_graph_save(self.g, fileName.encode()))
+
+    def LoadSVGFromFile(self, fileName: str):                 # SVG.cu:219-239
+        self.FreeSVGStructure()
+        g = C.c_void_p()
+        check(lib().svg_graph_load(self.mesh.h, self.K, fileName.encode(), C.byref(g)))
+        self.g = g
+
+    # --------------------------------------------------------------- queries
+    def SolveSSSD(self, s: int, t: int, path: bool = False):  # SVG.cu:241-251
+        d = C.c_double()
+        if path:
+            V, _, _ = self.info()
+            buf = np.empty(V, np.int32)
+            n = C.c_int()
+            check(lib().svg_sssd(self.g, int(s), int(t), C.byref(d), _ptr(buf), V, C.byref(n)))
+            return d.value, buf[: n.value].copy()
+        check(lib().svg_sssd(self.g, int(s), int(t), C.byref(d), None, 0, None))
+        return d.value
+
+    def SolveMSAD(self, sources, weights, want_parent: bool = True):   # SVG.cu:354-367
+        V, _, _ = self.info()
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        dist = np.empty(V, np.float64)
+        sid = np.empty(V, np.int32) if want_parent else None
+        par = np.empty(V, np.int32) if want_parent else None
+        check(lib().svg_msad(self.g, src, w, len(src), dist, _ptr(sid), _ptr(par)))
+        return dist, sid, par
+
+    def SolveSSSDBatch(self, pairs):
+        st = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1))
+        out = np.empty(len(st) // 2, np.float64)
+        check(lib().svg_sssd_batch(self.g, st, len(out), out))
+        return out

@@ -1,0 +1,1004 @@
+// SVG construction on gfx950: the plain one-source-per-thread form and the
+// batched persistent form.  Both run Ich<> (ich_engine.h); they differ only in
+// how the per-source state is stored and scheduled.
+//
+// plain   (SVG.cu:9-96 structure): reference records (80-B heap items, the
+//         MyHashTable with its hash and overflow slot), per-thread scratch,
+//         Clear() of every table slot per source, grid-stride over sources,
+//         O(V) neighbour scan.  Exact even where the reference's tables
+//         overflow; used to recompute sources the batched form flags.
+// batched persistent lanes take sources from an atomic counter; a source's
+//         vertex / split tables are compact power-of-two open-addressing
+//         tables that stay resident until extraction; the window heap keeps
+//         16-B (key, record) entries with 64-B records in a slot pool and the
+//         root in registers.  Extraction runs one wave per source: the
+//         reached list is sorted in LDS, 64 backtraces run at once, and a
+//         ballot + popcount keeps the first K qualifying vertices in id order
+//         (SVG.cu:44-76 semantics without the O(V) scan).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "svg_acos.h"
+#include "ich_engine.h"
+#include "svg_build.h"
+
+namespace svg {
+
+#define HIP_TRY(expr)                                                          \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess) {                                                \
+            char _b[256];                                                      \
+            snprintf(_b, sizeof(_b), "%s failed: %s (%s:%d)", #expr,          \
+                     hipGetErrorString(_e), __FILE__, __LINE__);               \
+            return std::string(_b);                                            \
+        }                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// mesh upload
+// ---------------------------------------------------------------------------
+std::string upload_mesh(const HostMesh& h, DevMesh& d, bool aos)
+{
+    d.nv = h.nv; d.nf = h.nf; d.ne = h.ne;
+    if (!d.face) {
+        std::vector<FaceRec> fr((size_t)h.nf);
+        for (int f = 0; f < h.nf; ++f) {
+            FaceRec& r = fr[f];
+            for (int j = 0; j < 3; ++j) {
+                r.l[j] = h.len[3 * f + j];
+                r.v[j] = h.tri[3 * f + j];
+                r.tw[j] = h.twin[3 * f + j];
+            }
+            r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
+        }
+        std::vector<double> pos(4 * (size_t)h.nv);
+        for (int v = 0; v < h.nv; ++v) {
+            pos[4 * v] = h.xyz[3 * v]; pos[4 * v + 1] = h.xyz[3 * v + 1];
+            pos[4 * v + 2] = h.xyz[3 * v + 2]; pos[4 * v + 3] = 0.0;
+        }
+        HIP_TRY(hipMalloc(&d.face, sizeof(FaceRec) * (size_t)h.nf));
+        HIP_TRY(hipMalloc(&d.cornerA, sizeof(double) * (size_t)h.ne));
+        HIP_TRY(hipMalloc(&d.cornerB, sizeof(double) * (size_t)h.ne));
+        HIP_TRY(hipMalloc(&d.firstEdge, sizeof(int32_t) * (size_t)h.nv));
+        HIP_TRY(hipMalloc(&d.flags, (size_t)h.nv));
+        HIP_TRY(hipMalloc(&d.pos, sizeof(double) * 4 * (size_t)h.nv));
+        HIP_TRY(hipMemcpy(d.face, fr.data(), sizeof(FaceRec) * (size_t)h.nf, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.cornerA, h.cornerA.data(), sizeof(double) * (size_t)h.ne, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.cornerB, h.cornerB.data(), sizeof(double) * (size_t)h.ne, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.firstEdge, h.firstEdge.data(), sizeof(int32_t) * (size_t)h.nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.flags, h.flags.data(), (size_t)h.nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.pos, pos.data(), sizeof(double) * 4 * (size_t)h.nv, hipMemcpyHostToDevice));
+    }
+    if (aos && !d.edges) {
+        std::vector<EdgeAoS> ea((size_t)h.ne);
+        for (int e = 0; e < h.ne; ++e) {
+            EdgeAoS& x = ea[e];
+            int f = e / 3, j = e % 3;
+            x.v0 = h.tri[e];
+            x.v1 = h.tri[3 * f + (j + 1) % 3];
+            x.twin = h.twin[e];
+            x.next = 3 * f + (j + 1) % 3;
+            x.prev = 3 * f + (j + 2) % 3;
+            x.face = f;
+            x.len = h.len[e];
+        }
+        std::vector<VertAoS> va((size_t)h.nv);
+        for (int v = 0; v < h.nv; ++v) {
+            va[v].x = h.xyz[3 * v]; va[v].y = h.xyz[3 * v + 1]; va[v].z = h.xyz[3 * v + 2];
+            va[v].angle = h.angle[v];
+            va[v].firstEdge = h.firstEdge[v];
+            va[v].flags = h.flags[v];
+        }
+        HIP_TRY(hipMalloc(&d.edges, sizeof(EdgeAoS) * (size_t)h.ne));
+        HIP_TRY(hipMalloc(&d.verts, sizeof(VertAoS) * (size_t)h.nv));
+        HIP_TRY(hipMemcpy(d.edges, ea.data(), sizeof(EdgeAoS) * (size_t)h.ne, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.verts, va.data(), sizeof(VertAoS) * (size_t)h.nv, hipMemcpyHostToDevice));
+    }
+    return std::string();
+}
+
+void free_dev_mesh(DevMesh& d)
+{
+    hipFree(d.face); hipFree(d.cornerA); hipFree(d.cornerB); hipFree(d.firstEdge);
+    hipFree(d.flags); hipFree(d.pos); hipFree(d.edges); hipFree(d.verts);
+    d = DevMesh();
+}
+
+std::string alloc_graph(int V, int K, DevGraph& g)
+{
+    g.V = V; g.K = K;
+    const size_t n = (size_t)V * (size_t)K;
+    HIP_TRY(hipMalloc(&g.tail, sizeof(int32_t) * (size_t)V));
+    HIP_TRY(hipMalloc(&g.adj, sizeof(int32_t) * n));
+    HIP_TRY(hipMalloc(&g.w, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&g.srcEdge, sizeof(int32_t) * n));
+    HIP_TRY(hipMalloc(&g.dstEdge, sizeof(int32_t) * n));
+    HIP_TRY(hipMalloc(&g.srcX, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&g.dstX, sizeof(double) * n));
+    HIP_TRY(hipMemset(g.tail, 0, sizeof(int32_t) * (size_t)V));
+    return std::string();
+}
+
+void free_graph(DevGraph& g)
+{
+    hipFree(g.tail); hipFree(g.adj); hipFree(g.w); hipFree(g.srcEdge); hipFree(g.dstEdge);
+    hipFree(g.srcX); hipFree(g.dstX);
+    g = DevGraph();
+}
+
+// ---------------------------------------------------------------------------
+// plain form: reference records
+// ---------------------------------------------------------------------------
+struct PWin {                   // ICHDevice::Window (ICHDevice.cuh:16-24), 72 B
+    unsigned edgeID;
+    double b0, b1, d0, d1, ps, minDist;
+    unsigned srcID, pseudoSrcId;
+    int birth;
+    int level;
+};
+struct PWinItem { double key; PWin item; };          // PQItem<Window>, 80 B
+struct PPs {                    // ICHDevice::PseudoWindow, 32 B
+    unsigned vertID;
+    double dist;
+    unsigned srcId, pseudoSrcId;
+    unsigned birth;
+    unsigned level;
+};
+struct PPsItem { double key; PPs item; };            // PQItem<PseudoWindow>, 40 B
+struct PSplitItem { double dist, x; unsigned ps; int index; };
+struct PVertItem {
+    int8_t birth;
+    double dist;
+    uint8_t isSource;
+    int enter;
+    unsigned ps, srcId;
+    int index;
+};
+
+struct PlainPolicy {
+    const EdgeAoS* __restrict__ E;
+    const VertAoS* __restrict__ Vt;
+    const double* __restrict__ cA;
+    const double* __restrict__ cB;
+    int nv_, nf_;
+    PSplitItem* sp; unsigned ssz;
+    PVertItem* vt; unsigned vsz;
+    PWinItem* wq; int wtail;
+    PPsItem* pq; int ptail;
+    uint32_t flags;
+    int srcVert;
+
+    __device__ int twin(int e) const { return E[e].twin; }
+    __device__ int next(int e) const { return E[e].next; }
+    __device__ int prev(int e) const { return E[e].prev; }
+    __device__ int v0(int e) const { return E[e].v0; }
+    __device__ int v1(int e) const { return E[e].v1; }
+    __device__ double len(int e) const { return E[e].len; }
+    __device__ int first_edge(int v) const { return Vt[v].firstEdge; }
+    __device__ uint8_t vflag(int v) const { return (uint8_t)Vt[v].flags; }
+    __device__ double cornerA(int e) const { return cA[e]; }
+    __device__ double cornerB(int e) const { return cB[e]; }
+    __device__ int nv() const { return nv_; }
+    __device__ int nf() const { return nf_; }
+    __device__ bool stop() const { return false; }
+    __device__ void face3(int e0, Tri& t) const
+    {
+        t.e0 = e0; t.e1 = E[e0].next; t.e2 = E[t.e1].next;
+        t.a = E[e0].v0; t.b = E[e0].v1; t.op = E[t.e1].v1;
+        t.l0 = E[e0].len; t.l1 = E[t.e1].len; t.l2 = E[t.e2].len;
+        t.tw1 = E[t.e1].twin; t.tw2 = E[t.e2].twin;
+    }
+    __device__ void count(int, long long) {}
+
+    // MyHashTable (MyHashTable.cuh:98-117)
+    __device__ static unsigned hfun(unsigned index, unsigned size)
+    {
+        double p = index * 0.618033989;
+        return (unsigned)(int)((p - (int)p) * size);
+    }
+    __device__ unsigned vkey(unsigned idx)
+    {
+        unsigned k = hfun(idx, vsz), i = 0;
+        for (; i < vsz; ++i) {
+            int ix = vt[k].index;
+            if ((unsigned)ix == idx || ix == -1) break;
+            k = (k + 1) % vsz;
+        }
+        if (i == vsz) { k = vsz; flags |= SF_REF_OVERFLOW; }
+        return k;
+    }
+    __device__ unsigned skey(unsigned idx)
+    {
+        unsigned k = hfun(idx, ssz), i = 0;
+        for (; i < ssz; ++i) {
+            int ix = sp[k].index;
+            if ((unsigned)ix == idx || ix == -1) break;
+            k = (k + 1) % ssz;
+        }
+        if (i == ssz) { k = ssz; flags |= SF_REF_OVERFLOW; }
+        return k;
+    }
+    __device__ VInfo vget(int v)
+    {
+        const PVertItem& it = vt[vkey((unsigned)v)];
+        VInfo r; r.dist = it.dist; r.enter = it.enter; r.birth = it.birth; r.isSource = it.isSource;
+        return r;
+    }
+    __device__ void vput(int v, const VInfo& x)
+    {
+        unsigned k = vkey((unsigned)v);
+        vt[k].index = v;
+        vt[k].birth = x.birth; vt[k].dist = x.dist; vt[k].enter = x.enter; vt[k].isSource = x.isSource;
+        vt[k].srcId = (unsigned)srcVert;
+    }
+    __device__ void vtouch(int v) { unsigned k = vkey((unsigned)v); vt[k].index = v; }
+    __device__ SInfo sget(int e)
+    {
+        const PSplitItem& it = sp[skey((unsigned)e)];
+        SInfo r; r.dist = it.dist; r.x = it.x; r.ps = (int)it.ps;
+        return r;
+    }
+    __device__ void sput(int e, const SInfo& x)
+    {
+        unsigned k = skey((unsigned)e);
+        sp[k].index = e;
+        sp[k].dist = x.dist; sp[k].x = x.x; sp[k].ps = (unsigned)x.ps;
+    }
+
+    // PriorityQueues (PriorityQueue.cuh:51-105)
+    __device__ bool wempty() const { return wtail == 1; }
+    __device__ void wtop(Win& w) const
+    {
+        const PWin& it = wq[1].item;
+        w.edge = (int)it.edgeID; w.b0 = it.b0; w.b1 = it.b1; w.d0 = it.d0; w.d1 = it.d1; w.ps = it.ps;
+        win_finish(w);
+        w.pseudoSrc = (int)it.pseudoSrcId; w.birth = it.birth; w.level = it.level;
+    }
+    __device__ void wpop()
+    {
+        --wtail;
+        const PWinItem top = wq[wtail];
+        int cur = 1, l = 2, r = 3, mc = -1;
+        if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+        else if (l < wtail) mc = l;
+        while (mc != -1 && wq[mc].key < top.key) {
+            wq[cur] = wq[mc];
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+            else if (l < wtail) mc = l;
+            else mc = -1;
+        }
+        wq[cur] = top;
+    }
+    __device__ void wpush(const Win& w)
+    {
+        if (wtail - 1 >= kHeapCap) { flags |= SF_WIN_DROP; return; }
+        ++wtail;
+        int cur = wtail - 1, p = cur / 2;
+        const double key = w.minDist;
+        while (p > 0 && wq[p].key > key) { wq[cur] = wq[p]; cur = p; p = cur / 2; }
+        PWin it;
+        it.edgeID = (unsigned)w.edge; it.b0 = w.b0; it.b1 = w.b1; it.d0 = w.d0; it.d1 = w.d1;
+        it.ps = w.ps; it.minDist = w.minDist; it.srcID = (unsigned)srcVert;
+        it.pseudoSrcId = (unsigned)w.pseudoSrc; it.birth = w.birth; it.level = w.level;
+        wq[cur].item = it;
+        wq[cur].key = key;
+    }
+    __device__ bool pempty() const { return ptail == 1; }
+    __device__ void ptop(Pseudo& q) const
+    {
+        const PPs& it = pq[1].item;
+        q.vert = (int)it.vertID; q.dist = it.dist; q.pseudoSrc = (int)it.pseudoSrcId;
+        q.birth = (int)it.birth; q.level = (int)it.level;
+    }
+    __device__ void ppop()
+    {
+        --ptail;
+        const PPsItem top = pq[ptail];
+        int cur = 1, l = 2, r = 3, mc = -1;
+        if (r < ptail) mc = pq[l].key < pq[r].key ? l : r;
+        else if (l < ptail) mc = l;
+        while (mc != -1 && pq[mc].key < top.key) {
+            pq[cur] = pq[mc];
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < ptail) mc = pq[l].key < pq[r].key ? l : r;
+            else if (l < ptail) mc = l;
+            else mc = -1;
+        }
+        pq[cur] = top;
+    }
+    __device__ void ppush(const Pseudo& q)
+    {
+        if (ptail - 1 >= kHeapCap) { flags |= SF_PSEUDO_DROP; return; }
+        ++ptail;
+        int cur = ptail - 1, p = cur / 2;
+        const double key = q.dist;
+        while (p > 0 && pq[p].key > key) { pq[cur] = pq[p]; cur = p; p = cur / 2; }
+        PPs it;
+        it.vertID = (unsigned)q.vert; it.dist = q.dist; it.srcId = (unsigned)srcVert;
+        it.pseudoSrcId = (unsigned)q.pseudoSrc; it.birth = (unsigned)q.birth; it.level = (unsigned)q.level;
+        pq[cur].item = it;
+        pq[cur].key = key;
+    }
+
+    // ICHDevice::Clear (ICHDevice.cu:487-523): every slot back to defaults.
+    __device__ void clear()
+    {
+        wtail = 1; ptail = 1; flags = 0;
+        for (unsigned i = 0; i < ssz + 1; ++i) {
+            sp[i].dist = DBL_MAX; sp[i].x = DBL_MAX; sp[i].ps = 0xFFFFFFFFu; sp[i].index = -1;
+        }
+        for (unsigned i = 0; i < vsz + 1; ++i) {
+            vt[i].birth = -1; vt[i].dist = DBL_MAX; vt[i].enter = -1; vt[i].isSource = 0;
+            vt[i].ps = 0xFFFFFFFFu; vt[i].srcId = 0xFFFFFFFFu; vt[i].index = -1;
+        }
+    }
+};
+
+struct PlainArgs {
+    DevMesh m;
+    DevGraph g;
+    int K;
+    unsigned ssz, vsz;
+    PWinItem* wq;
+    PPsItem* pq;
+    PSplitItem* sp;
+    PVertItem* vt;
+    const int32_t* srcList;     // null: all vertices
+    int nsrc;
+    uint32_t* status;           // per vertex
+    int hopCap;
+};
+
+__global__ void __launch_bounds__(256) plain_kernel(PlainArgs a)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = gridDim.x * blockDim.x;
+    PlainPolicy p;
+    p.E = a.m.edges; p.Vt = a.m.verts; p.cA = a.m.cornerA; p.cB = a.m.cornerB;
+    p.nv_ = a.m.nv; p.nf_ = a.m.nf;
+    p.ssz = a.ssz; p.vsz = a.vsz;
+    p.sp = a.sp + (size_t)idx * (a.ssz + 1);
+    p.vt = a.vt + (size_t)idx * (a.vsz + 1);
+    p.wq = a.wq + (size_t)idx * (kHeapCap + 1);
+    p.pq = a.pq + (size_t)idx * (kHeapCap + 1);
+    const int K = a.K;
+    for (int s = idx; s < a.nsrc; s += total) {
+        const int i = a.srcList ? a.srcList[s] : s;
+        p.srcVert = i;
+        p.clear();
+        Ich<PlainPolicy> ich(p, i, K);
+        ich.execute();
+        int tail = 0;
+        const size_t row = (size_t)i * (size_t)K;
+        for (int j = 0; j < a.m.nv; ++j) {
+            const VInfo vj = p.vget(j);
+            if (vj.dist == DBL_MAX) continue;
+            if (j == i) continue;
+            int se, de;
+            double sx, dx;
+            const bool pass = ich.path_to(j, se, sx, de, dx, a.hopCap);
+            if (pass) continue;
+            if (se == -1) { se = de; sx = dx; }
+            a.g.adj[row + tail] = j;
+            a.g.w[row + tail] = vj.dist;
+            a.g.srcEdge[row + tail] = se; a.g.srcX[row + tail] = sx;
+            a.g.dstEdge[row + tail] = de; a.g.dstX[row + tail] = dx;
+            ++tail;
+            if (tail == K) break;
+        }
+        a.g.tail[i] = tail;
+        if (a.status) a.status[i] = p.flags;
+    }
+}
+
+static int hop_cap(const HostMesh& h) { return 4 * (h.nf + h.nv) + 16; }
+
+static double elapsed_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return (double)ms;
+}
+
+std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& c, DevGraph& g,
+                        BuildStats& st, const int32_t* srcList, int nList)
+{
+    if (!d.edges) return "plain build needs the reference-layout mesh (upload with aos)";
+    const int K = c.K;
+    const int blocks = c.plainBlocks > 0 ? c.plainBlocks : 64;
+    const int threads = c.plainThreads > 0 ? c.plainThreads : 128;
+    const long long nthr = (long long)blocks * threads;
+    const unsigned ssz = (unsigned)K * c.splitCoef, vsz = (unsigned)K * c.vertCoef;
+    auto t0 = std::chrono::steady_clock::now();
+    PlainArgs a;
+    a.m = d; a.g = g; a.K = K; a.ssz = ssz; a.vsz = vsz;
+    a.wq = nullptr; a.pq = nullptr; a.sp = nullptr; a.vt = nullptr;
+    a.srcList = nullptr; a.nsrc = nList > 0 ? nList : h.nv; a.status = nullptr;
+    a.hopCap = hop_cap(h);
+    int32_t* dList = nullptr;
+    std::string err;
+    do {
+        if (hipMalloc(&a.wq, sizeof(PWinItem) * (size_t)(kHeapCap + 1) * nthr) != hipSuccess ||
+            hipMalloc(&a.pq, sizeof(PPsItem) * (size_t)(kHeapCap + 1) * nthr) != hipSuccess ||
+            hipMalloc(&a.sp, sizeof(PSplitItem) * (size_t)(ssz + 1) * nthr) != hipSuccess ||
+            hipMalloc(&a.vt, sizeof(PVertItem) * (size_t)(vsz + 1) * nthr) != hipSuccess ||
+            hipMalloc(&a.status, sizeof(uint32_t) * (size_t)h.nv) != hipSuccess) {
+            err = "plain build: out of device memory for per-thread scratch";
+            break;
+        }
+        hipMemset(a.status, 0, sizeof(uint32_t) * (size_t)h.nv);
+        if (nList > 0) {
+            if (hipMalloc(&dList, sizeof(int32_t) * (size_t)nList) != hipSuccess) { err = "oom"; break; }
+            hipMemcpy(dList, srcList, sizeof(int32_t) * (size_t)nList, hipMemcpyHostToDevice);
+            a.srcList = dList;
+        }
+        st.allocSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0); hipEventCreate(&e1);
+        hipEventRecord(e0, 0);
+        hipLaunchKernelGGL(plain_kernel, dim3(blocks), dim3(threads), 0, 0, a);
+        hipEventRecord(e1, 0);
+        hipError_t ke = hipEventSynchronize(e1);
+        if (ke == hipSuccess) ke = hipGetLastError();
+        if (ke != hipSuccess) { err = std::string("plain kernel: ") + hipGetErrorString(ke); }
+        const double ms = elapsed_ms(e0, e1);
+        if (nList > 0) st.redoSeconds += ms * 1e-3; else st.propSeconds += ms * 1e-3;
+        st.seconds += ms * 1e-3;
+        hipEventDestroy(e0); hipEventDestroy(e1);
+        if (err.empty()) {
+            std::vector<uint32_t> stv((size_t)h.nv);
+            hipMemcpy(stv.data(), a.status, sizeof(uint32_t) * (size_t)h.nv, hipMemcpyDeviceToHost);
+            if (st.status.size() != (size_t)h.nv) st.status.assign((size_t)h.nv, 0);
+            for (int k = 0; k < a.nsrc; ++k) {
+                const int v = nList > 0 ? srcList[k] : k;
+                st.status[v] = stv[v] | (nList > 0 ? SF_REDONE : 0);
+            }
+        }
+    } while (0);
+    hipFree(a.wq); hipFree(a.pq); hipFree(a.sp); hipFree(a.vt); hipFree(a.status); hipFree(dList);
+    st.lanes = (int)nthr;
+    return err;
+}
+
+// ---------------------------------------------------------------------------
+// batched form
+// ---------------------------------------------------------------------------
+struct BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 B
+struct BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B
+struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, pad; };  // 56 B
+struct alignas(8) BPsEnt { double key; int32_t vert, pseudoSrc, lvlb, pad; };   // 24 B
+struct BHeapEnt { double key; int32_t idx; int32_t pad; };     // 16 B
+
+constexpr uint32_t kEnterNone = 0xFFFFFFu;
+
+__device__ __forceinline__ unsigned hmix(unsigned k) { return k * 2654435761u; }
+
+struct BatchedPolicy {
+    const FaceRec* __restrict__ F;
+    const double* __restrict__ cA;
+    const double* __restrict__ cB;
+    const int32_t* __restrict__ firstE;
+    const uint8_t* __restrict__ vflags;
+    int nv_, nf_;
+    int src;
+    // tables of the current source
+    BSlotV* vt; int32_t* rl; unsigned vmask; int vcount; int vrefcap; int vlimit;
+    BSlotS* st; unsigned smask; int scount; int srefcap; int slimit;
+    // window heap
+    BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
+    int wtail, wftop, wfresh, wcache;
+    double rootKey; int rootIdx;
+    // pseudo heap
+    BPsEnt* ph; int ptail;
+    uint32_t flags;
+    bool dead;
+    int c_push, c_pop, c_ppush, c_ppop, c_prop, c_child, c_sw, c_vw, c_fan, c_nb, c_hops, c_err;
+
+    __device__ int twin(int e) const { return F[e / 3].tw[e % 3]; }
+    __device__ static int next(int e) { return (e % 3 == 2) ? e - 2 : e + 1; }
+    __device__ static int prev(int e) { return (e % 3 == 0) ? e + 2 : e - 1; }
+    __device__ int v0(int e) const { return F[e / 3].v[e % 3]; }
+    __device__ int v1(int e) const { return F[e / 3].v[(e % 3 == 2) ? 0 : e % 3 + 1]; }
+    __device__ double len(int e) const { return F[e / 3].l[e % 3]; }
+    __device__ int first_edge(int v) const { return firstE[v]; }
+    __device__ uint8_t vflag(int v) const { return vflags[v]; }
+    __device__ double cornerA(int e) const { return cA[e]; }
+    __device__ double cornerB(int e) const { return cB[e]; }
+    __device__ int nv() const { return nv_; }
+    __device__ int nf() const { return nf_; }
+    __device__ bool stop() const { return dead; }
+    __device__ void face3(int e0, Tri& t) const
+    {
+        const int f = e0 / 3, j = e0 - 3 * f;
+        const FaceRec r = F[f];
+        const int j1 = (j == 2) ? 0 : j + 1, j2 = (j == 0) ? 2 : j - 1;
+        t.e0 = e0; t.e1 = 3 * f + j1; t.e2 = 3 * f + j2;
+        t.l0 = r.l[j]; t.l1 = r.l[j1]; t.l2 = r.l[j2];
+        t.a = r.v[j]; t.b = r.v[j1]; t.op = r.v[j2];
+        t.tw1 = r.tw[j1]; t.tw2 = r.tw[j2];
+    }
+    __device__ void count(int which, long long n)
+    {
+        const int k = (int)n;
+        switch (which) {
+            case CNT_WIN_PUSH: c_push += k; break;
+            case CNT_WIN_POP: c_pop += k; break;
+            case CNT_PSEUDO_PUSH: c_ppush += k; break;
+            case CNT_PSEUDO_POP: c_ppop += k; break;
+            case CNT_PROPAGATE: c_prop += k; break;
+            case CNT_CHILD_BUILT: c_child += k; break;
+            case CNT_SPLIT_WRITE: c_sw += k; break;
+            case CNT_VERT_WRITE: c_vw += k; break;
+            case CNT_FAN_HALFEDGE: c_fan += k; break;
+            case CNT_PSEUDO_NEIGH: c_nb += k; break;
+            case CNT_BT_HOPS: c_hops += k; break;
+            case CNT_ERRORS: c_err += k; flags |= SF_ERROR; break;
+            default: break;
+        }
+    }
+
+    // --- vertex table
+    __device__ VInfo vget(int v) const
+    {
+        unsigned h = hmix((unsigned)v) & vmask;
+        for (;;) {
+            const BSlotV s = vt[h];
+            if (s.key == v) {
+                VInfo r;
+                const uint32_t e24 = s.eb & kEnterNone;
+                r.enter = e24 == kEnterNone ? -1 : (int)e24;
+                r.birth = (int8_t)(s.eb >> 24);
+                r.dist = s.dist;
+                r.isSource = (uint8_t)(v == src);
+                return r;
+            }
+            if (s.key == -1) return vinfo_default();
+            h = (h + 1) & vmask;
+        }
+    }
+    __device__ void vput(int v, const VInfo& x)
+    {
+        if (dead) return;
+        unsigned h = hmix((unsigned)v) & vmask;
+        for (;;) {
+            const int32_t k = vt[h].key;
+            if (k == v) break;
+            if (k == -1) {
+                if (vcount >= vlimit) { flags |= SF_CAPACITY; dead = true; return; }
+                vt[h].key = v;
+                rl[vcount++] = v;
+                if (vcount > vrefcap) flags |= SF_REF_OVERFLOW;
+                break;
+            }
+            h = (h + 1) & vmask;
+        }
+        vt[h].eb = ((uint32_t)x.enter & kEnterNone) | ((uint32_t)(uint8_t)x.birth << 24);
+        vt[h].dist = x.dist;
+    }
+    __device__ void vtouch(int) {}
+    // --- split table
+    __device__ SInfo sget(int e) const
+    {
+        unsigned h = hmix((unsigned)e) & smask;
+        for (;;) {
+            const int32_t k = st[h].key;
+            if (k == e) {
+                SInfo r; r.dist = st[h].dist; r.x = st[h].x; r.ps = st[h].ps;
+                return r;
+            }
+            if (k == -1) return sinfo_default();
+            h = (h + 1) & smask;
+        }
+    }
+    __device__ void sput(int e, const SInfo& x)
+    {
+        if (dead) return;
+        unsigned h = hmix((unsigned)e) & smask;
+        for (;;) {
+            const int32_t k = st[h].key;
+            if (k == e) break;
+            if (k == -1) {
+                if (scount >= slimit) { flags |= SF_CAPACITY; dead = true; return; }
+                st[h].key = e;
+                ++scount;
+                if (scount > srefcap) flags |= SF_REF_OVERFLOW;
+                break;
+            }
+            h = (h + 1) & smask;
+        }
+        st[h].ps = x.ps; st[h].dist = x.dist; st[h].x = x.x;
+    }
+
+    // --- window heap: PriorityQueue.cuh:51-105 semantics on (key, record) entries
+    __device__ int walloc()
+    {
+        int idx;
+        if (wcache >= 0) {
+            idx = wcache;
+            wcache = wftop > 0 ? (int)wfree[--wftop] : -1;
+        } else {
+            idx = wfresh++;
+        }
+        return idx;
+    }
+    __device__ void wrelease(int idx)
+    {
+        if (wcache >= 0) wfree[wftop++] = (uint16_t)wcache;
+        wcache = idx;
+    }
+    __device__ bool wempty() const { return wtail == 1; }
+    __device__ void wtop(Win& w) const
+    {
+        const BWinRec r = wr[rootIdx];
+        w.b0 = r.b0; w.b1 = r.b1; w.ps = r.ps; w.sx = r.sx; w.sy = r.sy;
+        w.d0 = 0.0; w.d1 = 0.0;
+        w.minDist = rootKey;
+        w.edge = r.edge; w.pseudoSrc = r.pseudoSrc;
+        w.birth = (int)(int8_t)(r.lvlb & 0xFF);
+        w.level = r.lvlb >> 8;
+    }
+    __device__ void wpop()
+    {
+        wrelease(rootIdx);
+        --wtail;
+        const BHeapEnt top = wh[wtail];
+        int cur = 1, l = 2, r = 3, mc = -1;
+        BHeapEnt cl, cr, cm;
+        if (r < wtail) { cl = wh[l]; cr = wh[r]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
+        else if (l < wtail) { cm = wh[l]; mc = l; }
+        bool movedRoot = false;
+        while (mc != -1 && cm.key < top.key) {
+            wh[cur] = cm;
+            if (cur == 1) { rootKey = cm.key; rootIdx = cm.idx; movedRoot = true; }
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < wtail) { cl = wh[l]; cr = wh[r]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
+            else if (l < wtail) { cm = wh[l]; mc = l; }
+            else mc = -1;
+        }
+        wh[cur] = top;
+        if (!movedRoot) { rootKey = top.key; rootIdx = top.idx; }
+    }
+    __device__ void wpush(const Win& w)
+    {
+        if (wtail - 1 >= kHeapCap) { flags |= SF_WIN_DROP; return; }
+        const int idx = walloc();
+        BWinRec rec;
+        rec.b0 = w.b0; rec.b1 = w.b1; rec.ps = w.ps; rec.sx = w.sx; rec.sy = w.sy;
+        rec.edge = w.edge; rec.pseudoSrc = w.pseudoSrc;
+        rec.lvlb = (w.level << 8) | (w.birth & 0xFF); rec.pad = 0;
+        wr[idx] = rec;
+        ++wtail;
+        int cur = wtail - 1, p = cur / 2;
+        const double key = w.minDist;
+        while (p > 0) {
+            const BHeapEnt pe = (p == 1) ? BHeapEnt{rootKey, rootIdx, 0} : wh[p];
+            if (!(pe.key > key)) break;
+            wh[cur] = pe;
+            cur = p; p = cur / 2;
+        }
+        BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
+        wh[cur] = ne;
+        if (cur == 1) { rootKey = key; rootIdx = idx; }
+    }
+    // --- pseudo heap (inline 24-B entries)
+    __device__ bool pempty() const { return ptail == 1; }
+    __device__ void ptop(Pseudo& q) const
+    {
+        const BPsEnt e = ph[1];
+        q.dist = e.key; q.vert = e.vert; q.pseudoSrc = e.pseudoSrc;
+        q.birth = (int)(int8_t)(e.lvlb & 0xFF); q.level = e.lvlb >> 8;
+    }
+    __device__ void ppop()
+    {
+        --ptail;
+        const BPsEnt top = ph[ptail];
+        int cur = 1, l = 2, r = 3, mc = -1;
+        if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
+        else if (l < ptail) mc = l;
+        while (mc != -1 && ph[mc].key < top.key) {
+            ph[cur] = ph[mc];
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
+            else if (l < ptail) mc = l;
+            else mc = -1;
+        }
+        ph[cur] = top;
+    }
+    __device__ void ppush(const Pseudo& q)
+    {
+        if (ptail - 1 >= kHeapCap) { flags |= SF_PSEUDO_DROP; return; }
+        ++ptail;
+        int cur = ptail - 1, p = cur / 2;
+        const double key = q.dist;
+        while (p > 0 && ph[p].key > key) { ph[cur] = ph[p]; cur = p; p = cur / 2; }
+        BPsEnt e; e.key = key; e.vert = q.vert; e.pseudoSrc = q.pseudoSrc;
+        e.lvlb = (q.level << 8) | (q.birth & 0xFF); e.pad = 0;
+        ph[cur] = e;
+    }
+};
+
+struct BArgs {
+    DevMesh m;
+    DevGraph g;
+    int K;
+    int lanes;
+    int nsrc;                   // sources in this chunk
+    int srcBase;                // first source of the chunk
+    unsigned vcap, scap;        // table slots (powers of two)
+    int vrefcap, srefcap;       // reference table sizes (vertCoef*K, splitCoef*K)
+    BSlotV* vtab;               // chunk * vcap
+    int32_t* rlist;             // chunk * vcap
+    BSlotS* stab;               // chunk * scap
+    BHeapEnt* wheap;            // lanes * 1024
+    BWinRec* wrec;              // lanes * 1024
+    uint16_t* wfree;            // lanes * 1024
+    BPsEnt* pheap;              // lanes * 1024
+    int* work;                  // work counter
+    uint32_t* status;           // chunk
+    int32_t* vcount;            // chunk
+    int32_t* cnt;               // chunk * 12 event counters
+    int hopCap;
+};
+
+constexpr int kNumBatchedCounters = 12;
+
+__device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
+{
+    p.F = a.m.face; p.cA = a.m.cornerA; p.cB = a.m.cornerB; p.firstE = a.m.firstEdge;
+    p.vflags = a.m.flags; p.nv_ = a.m.nv; p.nf_ = a.m.nf;
+}
+
+__device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, int slot, int src)
+{
+    p.src = src;
+    p.vt = a.vtab + (size_t)slot * a.vcap;
+    p.rl = a.rlist + (size_t)slot * a.vcap;
+    p.vmask = a.vcap - 1;
+    p.st = a.stab + (size_t)slot * a.scap;
+    p.smask = a.scap - 1;
+    p.vrefcap = a.vrefcap; p.srefcap = a.srefcap;
+    p.vlimit = (int)(a.vcap - (a.vcap >> 2));      // keep the load factor <= 3/4
+    p.slimit = (int)(a.scap - (a.scap >> 2));
+    p.vcount = 0; p.scount = 0;
+    p.flags = 0; p.dead = false;
+    p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
+    p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;
+}
+
+__global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
+{
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.lanes) return;
+    BatchedPolicy p;
+    policy_mesh(p, a);
+    p.wh = a.wheap + (size_t)lane * (kHeapCap + 1);
+    p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
+    p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
+    p.ph = a.pheap + (size_t)lane * (kHeapCap + 1);
+    for (;;) {
+        const int slot = atomicAdd(a.work, 1);
+        if (slot >= a.nsrc) break;
+        const int src = a.srcBase + slot;
+        policy_tables(p, a, slot, src);
+        p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
+        p.ptail = 1;
+        Ich<BatchedPolicy> ich(p, src, a.K);
+        ich.execute();
+        a.status[slot] = p.flags;
+        a.vcount[slot] = p.vcount;
+        int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
+        c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
+        c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
+        c[10] = 0; c[11] = p.vcount;
+    }
+}
+
+// One wave per source: sort the reached vertices by id in LDS, backtrace 64
+// candidates at a time, keep the first K whose path passes no vertex.
+template <int SORTN>
+__global__ void __launch_bounds__(64) batched_extract_kernel(BArgs a)
+{
+    __shared__ int32_t buf[SORTN];
+    const int lane = threadIdx.x;
+    BatchedPolicy p;
+    policy_mesh(p, a);
+    for (int slot = blockIdx.x; slot < a.nsrc; slot += gridDim.x) {
+        const int src = a.srcBase + slot;
+        const uint32_t stt = a.status[slot];
+        const int K = a.K;
+        if (stt & (SF_CAPACITY | SF_REF_OVERFLOW)) {
+            if (lane == 0) a.g.tail[src] = 0;
+            continue;
+        }
+        const int n = a.vcount[slot];
+        policy_tables(p, a, slot, src);
+        int N = 64;
+        while (N < n) N <<= 1;
+        for (int i = lane; i < N; i += 64) buf[i] = i < n ? p.rl[i] : 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= N; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < N; i += 64) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const int x = buf[i], y = buf[ixj];
+                        const bool asc = (i & k) == 0;
+                        if ((x > y) == asc) { buf[i] = y; buf[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        Ich<BatchedPolicy> ich(p, src, K);
+        int tail = 0;
+        const size_t row = (size_t)src * (size_t)K;
+        for (int base = 0; base < n && tail < K; base += 64) {
+            const int c = base + lane;
+            const int j = c < n ? buf[c] : -1;
+            bool q = false;
+            int se = -1, de = -1;
+            double sx = 0.0, dx = 0.0, dist = 0.0;
+            if (j >= 0 && j != src) {
+                dist = p.vget(j).dist;
+                const bool pass = ich.path_to(j, se, sx, de, dx, a.hopCap);
+                q = !pass;
+                if (se == -1) { se = de; sx = dx; }
+            }
+            const unsigned long long m = __ballot(q);
+            const int pos = tail + __popcll(m & ((1ull << lane) - 1ull));
+            if (q && pos < K) {
+                a.g.adj[row + pos] = j;
+                a.g.w[row + pos] = dist;
+                a.g.srcEdge[row + pos] = se; a.g.srcX[row + pos] = sx;
+                a.g.dstEdge[row + pos] = de; a.g.dstX[row + pos] = dx;
+            }
+            tail += __popcll(m);
+        }
+        const int hops = p.c_hops;
+        // wave reduction of the backtrace hop count
+        int hsum = hops;
+        for (int off = 32; off > 0; off >>= 1) hsum += __shfl_xor(hsum, off);
+        if (lane == 0) {
+            a.g.tail[src] = tail < K ? tail : K;
+            a.cnt[(size_t)slot * kNumBatchedCounters + 10] = hsum;
+        }
+        if (p.c_err) atomicOr(&a.status[slot], SF_ERROR);
+        __syncthreads();
+    }
+}
+
+static unsigned next_pow2(unsigned x)
+{
+    unsigned p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig& c, DevGraph& g,
+                          BuildStats& st)
+{
+    const int K = c.K;
+    const int V = h.nv;
+    if (h.ne >= (int)kEnterNone) return "batched build supports fewer than 2^24-1 half-edges";
+    const int vrefcap = (int)((unsigned)K * c.vertCoef);
+    const int srefcap = (int)((unsigned)K * c.splitCoef);
+    unsigned vcap = c.vertCap > 0 ? next_pow2((unsigned)c.vertCap) : next_pow2(2u * (unsigned)vrefcap + 2u);
+    unsigned scap = c.splitCap > 0 ? next_pow2((unsigned)c.splitCap) : next_pow2(4u * (unsigned)K);
+    if (vcap < 64) vcap = 64;
+    if (scap < 64) scap = 64;
+    if (vcap > 8192) return "vertex table larger than the extraction sort buffer (8192)";
+    const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
+    const int block = c.propBlock > 0 ? c.propBlock : 64;
+    size_t freeB = 0, totB = 0;
+    hipMemGetInfo(&freeB, &totB);
+    const size_t perLane = (sizeof(BHeapEnt) + sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
+    const size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
+    int chunk = c.chunk > 0 ? c.chunk : V;
+    {
+        const size_t budget = freeB > ((size_t)lanes * perLane + (4ull << 30)) ? freeB - (size_t)lanes * perLane - (4ull << 30) : 0;
+        const size_t maxChunk = budget / perSrc;
+        if ((size_t)chunk > maxChunk) chunk = (int)std::max<size_t>(maxChunk, 1);
+    }
+    if (chunk > V) chunk = V;
+    st.lanes = lanes; st.propBlock = block; st.chunk = chunk; st.vertCap = (int)vcap; st.splitCap = (int)scap;
+
+    auto t0 = std::chrono::steady_clock::now();
+    BArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
+    a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
+    std::string err;
+    if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
+        hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
+        hipMalloc(&a.stab, sizeof(BSlotS) * scap * (size_t)chunk) != hipSuccess ||
+        hipMalloc(&a.wheap, sizeof(BHeapEnt) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wrec, sizeof(BWinRec) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wfree, sizeof(uint16_t) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.pheap, sizeof(BPsEnt) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.work, sizeof(int)) != hipSuccess ||
+        hipMalloc(&a.status, sizeof(uint32_t) * (size_t)chunk) != hipSuccess ||
+        hipMalloc(&a.vcount, sizeof(int32_t) * (size_t)chunk) != hipSuccess ||
+        hipMalloc(&a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)chunk) != hipSuccess) {
+        err = "batched build: out of device memory";
+    }
+    st.allocSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<int32_t> redo;
+    st.status.assign((size_t)V, 0);
+    st.counters.assign(CNT_NUM, 0);
+    std::vector<uint32_t> hs;
+    std::vector<int32_t> hc;
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2);
+    for (int base = 0; err.empty() && base < V; base += chunk) {
+        const int n = std::min(chunk, V - base);
+        a.nsrc = n; a.srcBase = base;
+        hipMemsetAsync(a.vtab, 0xFF, sizeof(BSlotV) * vcap * (size_t)n, 0);
+        hipMemsetAsync(a.stab, 0xFF, sizeof(BSlotS) * scap * (size_t)n, 0);
+        hipMemsetAsync(a.work, 0, sizeof(int), 0);
+        hipEventRecord(e0, 0);
+        const int pl = std::min(lanes, n);
+        a.lanes = pl;
+        hipLaunchKernelGGL(batched_prop_kernel, dim3((pl + block - 1) / block), dim3(block), 0, 0, a);
+        hipEventRecord(e1, 0);
+        const int eblocks = std::min(n, 16384);
+        if (vcap <= 1024)
+            hipLaunchKernelGGL(batched_extract_kernel<1024>, dim3(eblocks), dim3(64), 0, 0, a);
+        else if (vcap <= 2048)
+            hipLaunchKernelGGL(batched_extract_kernel<2048>, dim3(eblocks), dim3(64), 0, 0, a);
+        else if (vcap <= 4096)
+            hipLaunchKernelGGL(batched_extract_kernel<4096>, dim3(eblocks), dim3(64), 0, 0, a);
+        else
+            hipLaunchKernelGGL(batched_extract_kernel<8192>, dim3(eblocks), dim3(64), 0, 0, a);
+        hipEventRecord(e2, 0);
+        hipError_t ke = hipEventSynchronize(e2);
+        if (ke == hipSuccess) ke = hipGetLastError();
+        if (ke != hipSuccess) { err = std::string("batched kernels: ") + hipGetErrorString(ke); break; }
+        const double pms = elapsed_ms(e0, e1), ems = elapsed_ms(e1, e2);
+        st.propSeconds += pms * 1e-3; st.extractSeconds += ems * 1e-3; st.seconds += (pms + ems) * 1e-3;
+        hs.resize((size_t)n); hc.resize((size_t)n * kNumBatchedCounters);
+        hipMemcpy(hs.data(), a.status, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost);
+        hipMemcpy(hc.data(), a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)n, hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; ++i) {
+            st.status[base + i] = hs[i];
+            if (hs[i] & (SF_CAPACITY | SF_REF_OVERFLOW)) redo.push_back(base + i);
+            if (hs[i] & (SF_WIN_DROP | SF_PSEUDO_DROP)) st.flaggedDrop++;
+            if (hs[i] & SF_REF_OVERFLOW) st.flaggedOverflow++;
+            if (hs[i] & SF_ERROR) st.flaggedError++;
+            const int32_t* cc = &hc[(size_t)i * kNumBatchedCounters];
+            st.counters[CNT_WIN_PUSH] += cc[0]; st.counters[CNT_WIN_POP] += cc[1];
+            st.counters[CNT_PSEUDO_PUSH] += cc[2]; st.counters[CNT_PSEUDO_POP] += cc[3];
+            st.counters[CNT_PROPAGATE] += cc[4]; st.counters[CNT_CHILD_BUILT] += cc[5];
+            st.counters[CNT_SPLIT_WRITE] += cc[6]; st.counters[CNT_VERT_WRITE] += cc[7];
+            st.counters[CNT_FAN_HALFEDGE] += cc[8]; st.counters[CNT_PSEUDO_NEIGH] += cc[9];
+            st.counters[CNT_BT_HOPS] += cc[10]; st.counters[CNT_REACHED] += cc[11];
+        }
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+    hipFree(a.vtab); hipFree(a.rlist); hipFree(a.stab); hipFree(a.wheap); hipFree(a.wrec);
+    hipFree(a.wfree); hipFree(a.pheap); hipFree(a.work); hipFree(a.status); hipFree(a.vcount); hipFree(a.cnt);
+    if (!err.empty()) return err;
+    if (!redo.empty()) {
+        // Exact recomputation on the reference-layout path (its own tables
+        // replicate the reference, including overflow aliasing).
+        BuildConfig pc = c;
+        pc.plainBlocks = std::max(1, std::min(512, (int)((redo.size() + 127) / 128)));
+        pc.plainThreads = 128;
+        BuildStats ps;
+        err = build_plain(h, d, pc, g, ps, redo.data(), (int)redo.size());
+        if (!err.empty()) return "redo of flagged sources: " + err;
+        for (int v : redo) st.status[v] = ps.status[v] | SF_REDONE | (st.status[v] & (SF_CAPACITY | SF_REF_OVERFLOW));
+        st.redoSeconds += ps.redoSeconds;
+        st.seconds += ps.redoSeconds;
+        st.redone = (int)redo.size();
+    }
+    return std::string();
+}
+
+}  // namespace svg

@@ -1,0 +1,590 @@
+// Single-source bounded ICH window propagation and geodesic backtrace,
+// written once over a storage policy P so that the plain (reference-layout)
+// and batched (compact SoA) kernels run the same arithmetic in the same
+// order.  Every floating-point expression keeps the operand order of the
+// reference so that, compiled with -ffp-contract=off, the results are the
+// correctly rounded values the CPU restatement produces.
+//
+// Reference: ICHDevice.cuh:16-93 (records), ICHDevice.cu:67-132 (Execute),
+// :336-480 (BuildGeodesicPathTo), :525-581 (Initialize, vertex source),
+// :618-726 (PropagateWindow), :728-904 (pseudo-source windows),
+// :906-979 (IsValidWindow, BuildWindow, Intersect); Geometry.cu:306-311
+// (toLeft).  Open-mesh decisions D8/D9 are documented in the oracle header.
+//
+// The policy P provides
+//   mesh:   int twin(e), next(e), prev(e), v0(e), v1(e); double len(e);
+//           int first_edge(v); uint8 vflag(v); double cornerA(e), cornerB(e);
+//           int nv(), nf();
+//           void face3(e0, Tri&) -- the triangle of half-edge e0 as seen from e0
+//   tables: VInfo vget(v); void vput(v, VInfo); void vtouch(v);
+//           SInfo sget(e); void sput(e, SInfo)
+//   heaps:  bool wempty(); void wtop(Win&); void wpop(); void wpush(const Win&);
+//           bool pempty(); void ptop(Pseudo&); void ppop(); void ppush(const Pseudo&)
+//   counters: void count(int which, long long n)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+#include "svg_acos.h"
+
+namespace svg {
+
+#define SVG_HD __host__ __device__ __forceinline__
+
+constexpr double kPI = 3.141592653589793238;     // Geometry.cuh:18
+constexpr double kRelErr = 1e-8;                 // ICHDevice.cuh:10
+constexpr int kHeapCap = 1023;                   // WINPQ_SIZE - 1 (SVG.cuh:13; SVG.cu:19-22)
+
+// Event counters kept per source (units of the algorithmic-bytes model).
+enum Counter {
+    CNT_WIN_PUSH = 0, CNT_WIN_POP, CNT_PSEUDO_PUSH, CNT_PSEUDO_POP, CNT_PROPAGATE,
+    CNT_CHILD_BUILT, CNT_SPLIT_WRITE, CNT_VERT_WRITE, CNT_FAN_HALFEDGE, CNT_PSEUDO_NEIGH,
+    CNT_WIN_DROP, CNT_PSEUDO_DROP, CNT_BT_HOPS, CNT_REACHED, CNT_ARCS, CNT_ERRORS,
+    CNT_NUM
+};
+
+// Window (ICHDevice.cuh:16-50) in registers.  sx, sy cache FlatenedSrc().
+struct Win {
+    double b0, b1, d0, d1, ps, minDist, sx, sy;
+    int edge;
+    int pseudoSrc;
+    int birth;
+    int level;
+};
+
+struct Pseudo {       // ICHDevice.cuh:52-63 (srcId is always the source here)
+    double dist;
+    int vert;
+    int pseudoSrc;
+    int birth;
+    int level;
+};
+
+struct VInfo {        // ICHDevice.cuh:74-86 (srcId / pseudoSrcId are never read)
+    double dist;
+    int enter;
+    int8_t birth;
+    uint8_t isSource;
+};
+
+struct SInfo {        // ICHDevice.cuh:65-72
+    double dist, x;
+    int ps;
+};
+
+SVG_HD VInfo vinfo_default() { VInfo v; v.dist = DBL_MAX; v.enter = -1; v.birth = -1; v.isSource = 0; return v; }
+SVG_HD SInfo sinfo_default() { SInfo s; s.dist = DBL_MAX; s.x = DBL_MAX; s.ps = -1; return s; }
+
+// The triangle on the other side of a crossing: e0 and its two successors.
+struct Tri {
+    int e0, e1, e2;
+    int a, b, op;            // v0(e0), v1(e0), v1(e1) = opposite vertex
+    double l0, l1, l2;
+    int tw1, tw2;            // twins of e1, e2
+};
+
+SVG_HD double len2(double x, double y) { return sqrt(x * x + y * y); }
+
+// Window::calcMinDist + Window::FlatenedSrc (ICHDevice.cuh:30-49)
+SVG_HD void win_finish(Win& w)
+{
+    double wLen = w.b1 - w.b0;
+    double xProj = (w.d0 * w.d0 + wLen * wLen - w.d1 * w.d1) / (2.0 * wLen);
+    double y = sqrt(fabs(w.d0 * w.d0 - xProj * xProj));
+    if (xProj < 0.0) w.minDist = w.d0 + w.ps;
+    else if (xProj > wLen) w.minDist = w.d1 + w.ps;
+    else w.minDist = y + w.ps;
+    w.sx = xProj + w.b0;
+    w.sy = y;
+}
+
+// ICHDevice::Intersect (ICHDevice.cu:975-979)
+SVG_HD double intersect(double v0x, double v0y, double v1x, double v1y,
+                        double p0x, double p0y, double p1x, double p1y)
+{
+    double a00 = p0x - p1x, a01 = v1x - v0x, b0 = v1x - p1x;
+    double a10 = p0y - p1y, a11 = v1y - v0y, b1 = v1y - p1y;
+    return (b0 * a11 - b1 * a01) / (a00 * a11 - a10 * a01);
+}
+
+SVG_HD bool beaten(double a, double b, double tol) { return a > b && a / b - 1.0 > tol; }
+
+// One face crossing of the backtrace unfolding (ICHDevice.cu:420-462).
+struct Unfold { int gid; double gpos, lpx, lpy, cpx, cpy; };
+
+template <class P>
+struct Ich {
+    P& p;
+    int source;
+    int total;              // totalCalcVertNum
+    int K;
+    bool pathPassVert;
+
+    SVG_HD Ich(P& pol, int src, int K_) : p(pol), source(src), total(0), K(K_), pathPassVert(false) {}
+
+    SVG_HD void push_win(const Win& w) { p.count(CNT_WIN_PUSH, 1); p.wpush(w); }
+    SVG_HD void push_pseudo(const Pseudo& q) { p.count(CNT_PSEUDO_PUSH, 1); p.ppush(q); }
+
+    // ICHDevice::Initialize, vertex source (ICHDevice.cu:527-581) + D9
+    SVG_HD void initialize()
+    {
+        const int src = source;
+        const int start = p.first_edge(src);
+        int cur = start;
+        do {
+            const int opEdge = p.next(cur);
+            Win w;
+            w.edge = opEdge;
+            w.b0 = 0.0; w.b1 = p.len(opEdge);
+            w.d0 = p.len(cur);
+            w.d1 = p.len(p.prev(cur));
+            w.ps = 0.0;
+            win_finish(w);
+            w.pseudoSrc = src; w.birth = 0; w.level = 0;
+            push_win(w);
+            const int opVert = p.v1(cur);
+            p.vtouch(opVert);
+            if (p.len(cur) > p.vget(opVert).dist) {
+                cur = p.twin(cur);
+                if (cur != -1) cur = p.next(cur);
+                continue;
+            }
+            VInfo vi = p.vget(opVert);
+            vi.birth = 0;
+            vi.dist = p.len(cur);
+            vi.enter = p.twin(cur) != -1 ? p.twin(cur) : cur;     // D9
+            p.vput(opVert, vi);
+            if (p.vflag(opVert) & 1) {                            // angle < 2*PI
+                cur = p.twin(cur);
+                if (cur != -1) cur = p.next(cur);
+                continue;
+            }
+            Pseudo q;
+            q.vert = opVert; q.dist = p.len(cur);
+            q.pseudoSrc = src;
+            q.birth = p.vget(opVert).birth;
+            q.level = 0;
+            push_pseudo(q);
+            cur = p.twin(cur);
+            if (cur != -1) cur = p.next(cur);
+        } while (cur != start && cur != -1);
+        if (cur == -1) {                                          // D9: boundary source
+            const int inE = p.prev(start);
+            const int w = p.v0(inE);
+            p.vtouch(w);
+            if (!(p.len(inE) > p.vget(w).dist)) {
+                VInfo vi = p.vget(w);
+                vi.birth = 0; vi.dist = p.len(inE); vi.enter = inE;
+                p.vput(w, vi);
+            }
+        }
+        VInfo sv = p.vget(src);
+        sv.birth = 0; sv.dist = 0.0; sv.enter = -1; sv.isSource = 1;
+        p.vput(src, sv);
+    }
+
+    // ICHDevice::BuildWindow (ICHDevice.cu:957-973); (sx, sy) is the father's source.
+    SVG_HD void build_window(const Win& fa, int edge, double elen, double t0, double t1,
+                             double v0x, double v0y, double v1x, double v1y, Win& w)
+    {
+        w.edge = edge;
+        w.b0 = (1 - t0) * elen;
+        w.b1 = (1 - t1) * elen;
+        {
+            double mx = t0 * v0x + (1 - t0) * v1x, my = t0 * v0y + (1 - t0) * v1y;
+            w.d0 = len2(fa.sx - mx, fa.sy - my);
+        }
+        {
+            double mx = t1 * v0x + (1 - t1) * v1x, my = t1 * v0y + (1 - t1) * v1y;
+            w.d1 = len2(fa.sx - mx, fa.sy - my);
+        }
+        w.ps = fa.ps;
+        win_finish(w);
+        w.pseudoSrc = fa.pseudoSrc;
+        w.birth = fa.birth;
+        w.level = fa.level + 1;
+    }
+
+    // ICHDevice::IsValidWindow (ICHDevice.cu:906-955).  The child lies on edge
+    // `edge` of triangle t: its verts[0], verts[1] and opposite vertex and the
+    // lengths l0 = len(edge), l1 = len(next), l2 = len(next next).
+    SVG_HD bool is_valid(const Win& w, bool isLeft, int v1, int v2, int v3,
+                         double l0, double l1, double l2)
+    {
+        if (w.b1 <= w.b0) return false;
+        double p3x = (l2 * l2 + l0 * l0 - l1 * l1) / (2.0 * l0);
+        double p3y = sqrt(fabs(l2 * l2 - p3x * p3x));
+        double ax = w.b0, bx = w.b1;
+        double dv1 = p.vget(v1).dist;
+        double dv2 = p.vget(v2).dist;
+        double dv3 = p.vget(v3).dist;
+        double srcB = w.ps + len2(w.sx - bx, w.sy - 0.0);
+        double srcA = w.ps + len2(w.sx - ax, w.sy - 0.0);
+        if (beaten(srcB, dv1 + w.b1, 0.0)) return false;
+        if (beaten(srcA, dv2 + l0 - w.b0, 0.0)) return false;
+        if (isLeft) {
+            if (beaten(srcA, dv3 + len2(p3x - ax, p3y - 0.0), 0.0)) return false;
+        } else {
+            if (beaten(srcB, dv3 + len2(p3x - bx, p3y - 0.0), kRelErr)) return false;
+        }
+        return true;
+    }
+
+    // ICHDevice::PropagateWindow (ICHDevice.cu:618-726)
+    SVG_HD void propagate(const Win& win)
+    {
+        const int e0 = p.twin(win.edge);
+        if (e0 == -1) return;
+        p.count(CNT_PROPAGATE, 1);
+        Tri t;
+        p.face3(e0, t);
+        const double l0 = t.l0, l1 = t.l1, l2 = t.l2;
+        const double sx = win.sx, sy = win.sy;
+        const double leftx = win.b0, rightx = win.b1;
+        const double v0x = 0.0, v0y = 0.0, v1x = l0, v1y = 0.0;
+        const double v2x = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0);
+        const double v2y = -sqrt(fabs(l1 * l1 - v2x * v2x));
+        const double interX = v2x - v2y * (v2x - sx) / (v2y - sy);
+        Win lw, rw;
+        bool hasL = true, hasR = true;
+        // child on e1 (left): edge e1 runs b -> op; its next is e2 (op -> a)
+        // child on e2 (right): edge e2 runs op -> a; its next is e0 (a -> b)
+        if (interX <= leftx) {
+            hasL = false;
+            double t0 = intersect(sx, sy, leftx, 0.0, v2x, v2y, v1x, v1y);
+            double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
+            build_window(win, t.e2, l2, t0, t1, v2x, v2y, v1x, v1y, rw);
+            p.count(CNT_CHILD_BUILT, 1);
+            if (!is_valid(rw, false, t.op, t.a, t.b, l2, l0, l1)) hasR = false;
+        } else if (interX >= rightx) {
+            hasR = false;
+            double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
+            double t1 = intersect(sx, sy, rightx, 0.0, v0x, v0y, v2x, v2y);
+            build_window(win, t.e1, l1, t0, t1, v0x, v0y, v2x, v2y, lw);
+            p.count(CNT_CHILD_BUILT, 1);
+            if (!is_valid(lw, true, t.b, t.op, t.a, l1, l2, l0)) hasL = false;
+        } else {
+            const double directDist = len2(v2x - sx, v2y - sy);
+            const double cand = directDist + win.ps;
+            const SInfo sp = p.sget(e0);
+            if (cand > sp.dist && cand / sp.dist - 1.0 > kRelErr) {
+                hasL = sp.x < interX;
+                hasR = !hasL;
+            } else {
+                if (cand < sp.dist) {
+                    SInfo ns;
+                    ns.dist = cand;
+                    ns.ps = win.pseudoSrc;
+                    ns.x = l0 - interX;
+                    p.sput(e0, ns);
+                    p.count(CNT_SPLIT_WRITE, 1);
+                }
+                VInfo ov = p.vget(t.op);
+                if (cand < ov.dist) {
+                    if (ov.dist == DBL_MAX) ++total;
+                    ov.birth = (int8_t)(ov.birth + 1);
+                    ov.dist = cand;
+                    ov.enter = e0;
+                    p.vput(t.op, ov);
+                    p.count(CNT_VERT_WRITE, 1);
+                    if (p.vflag(t.op) & 2) {                        // angle > 2*PI
+                        Pseudo q;
+                        q.vert = t.op; q.dist = ov.dist;
+                        q.pseudoSrc = win.pseudoSrc;
+                        q.birth = ov.birth;
+                        q.level = win.level + 1;
+                        push_pseudo(q);
+                    }
+                }
+            }
+            if (hasL) {
+                double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
+                build_window(win, t.e1, l1, t0, 0.0, v0x, v0y, v2x, v2y, lw);
+                p.count(CNT_CHILD_BUILT, 1);
+                if (!is_valid(lw, true, t.b, t.op, t.a, l1, l2, l0)) hasL = false;
+            }
+            if (hasR) {
+                double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
+                build_window(win, t.e2, l2, 1.0, t1, v2x, v2y, v1x, v1y, rw);
+                p.count(CNT_CHILD_BUILT, 1);
+                if (!is_valid(rw, false, t.op, t.a, t.b, l2, l0, l1)) hasR = false;
+            }
+        }
+        if (hasL) push_win(lw);
+        if (hasR) push_win(rw);
+    }
+
+    // ICHDevice::GenSubWinsForPseudoSrcFromWindow (ICHDevice.cu:795-858)
+    SVG_HD void sweep_from_window(const Pseudo& pw, int e0, int& startEdge, int& endEdge)
+    {
+        (void)pw;
+        const int e1 = p.next(e0);
+        const int e2 = p.next(e1);
+        const double l0 = p.len(e0), l1 = p.len(e1), l2 = p.len(e2);
+        const double epx = l0 - p.sget(e0).x, epy = 0.0;
+        const double v0x = 0.0, v0y = 0.0, v1x = l0, v1y = 0.0;
+        const double v2x = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0);
+        const double v2y = -sqrt(fabs(l1 * l1 - v2x * v2x));
+        const double dx = epx - v2x, dy = epy - v2y;
+        const double dlen = len2(dx, dy);
+        double a0 = (dx * (v0x - v2x) + dy * (v0y - v2y)) / dlen / l1;
+        double a1 = (dx * (v1x - v2x) + dy * (v1y - v2y)) / dlen / l2;
+        if (a0 > 1.0) a0 = 1.0; else if (a0 < -1.0) a0 = -1.0;
+        if (a1 > 1.0) a1 = 1.0; else if (a1 < -1.0) a1 = -1.0;
+        double angle0 = svg_acos(a0), angle1 = svg_acos(a1);
+        startEdge = -1; endEdge = -1;
+        int cur = p.twin(e1);
+        while (angle0 < kPI && cur != -1) {
+            angle0 += p.cornerA(cur);
+            p.count(CNT_FAN_HALFEDGE, 1);
+            cur = p.twin(p.prev(cur));
+        }
+        if (cur != -1) startEdge = p.next(p.twin(cur));
+        cur = p.twin(e2);
+        while (angle1 < kPI && cur != -1) {
+            angle1 += p.cornerB(cur);
+            p.count(CNT_FAN_HALFEDGE, 1);
+            cur = p.twin(p.next(cur));
+        }
+        if (cur != -1) endEdge = p.twin(p.prev(p.twin(cur)));
+    }
+
+    // ICHDevice::GenSubWinsForPseudoSrcFromPseudoSrc (ICHDevice.cu:860-904)
+    SVG_HD void sweep_from_pseudo(int enter, int& startEdge, int& endEdge)
+    {
+        double angle0 = 0.0, angle1 = 0.0;
+        startEdge = -1; endEdge = -1;
+        int cur = enter;
+        while (angle0 < kPI && cur != -1) {
+            angle0 += p.cornerA(cur);
+            p.count(CNT_FAN_HALFEDGE, 1);
+            cur = p.twin(p.prev(cur));
+        }
+        if (cur != -1) startEdge = p.next(p.twin(cur));
+        cur = p.twin(enter);
+        while (angle1 < kPI && cur != -1) {
+            angle1 += p.cornerB(cur);
+            p.count(CNT_FAN_HALFEDGE, 1);
+            cur = p.twin(p.next(cur));
+        }
+        if (cur != -1) endEdge = p.twin(p.prev(p.twin(cur)));
+    }
+
+    // ICHDevice::GenSubWinsForPseudoSrc (ICHDevice.cu:728-793)
+    SVG_HD void gen_sub_wins(const Pseudo& pw)
+    {
+        int startEdge = -1, endEdge = -1;
+        const VInfo vi = p.vget(pw.vert);
+        if (vi.enter == -1 && vi.birth != -1) {
+            startEdge = p.first_edge(pw.vert);
+            endEdge = startEdge;
+        } else if (vi.enter >= 0 && p.v0(vi.enter) == pw.vert) {
+            sweep_from_pseudo(vi.enter, startEdge, endEdge);
+        } else if (vi.enter >= 0 && p.v1(p.next(vi.enter)) == pw.vert) {
+            sweep_from_window(pw, vi.enter, startEdge, endEdge);
+        } else {
+            p.count(CNT_ERRORS, 1);      // unreachable: only interior saddles are pseudo-sources
+            return;
+        }
+        if (startEdge == -1 || endEdge == -1) { p.count(CNT_ERRORS, 1); return; }
+        int guard = 0;
+        do {
+            Win w;
+            w.edge = p.next(startEdge);
+            w.b0 = 0.0; w.b1 = p.len(w.edge);
+            w.d0 = p.len(startEdge);
+            w.d1 = p.len(p.next(w.edge));
+            w.ps = pw.dist;
+            win_finish(w);
+            w.pseudoSrc = pw.vert;
+            w.birth = pw.birth;
+            w.level = pw.level + 1;
+            push_win(w);
+            startEdge = p.twin(p.prev(startEdge));
+            if (startEdge == -1 || ++guard > 4096) { p.count(CNT_ERRORS, 1); break; }
+        } while (startEdge != endEdge);
+
+        const int start = p.first_edge(pw.vert);
+        int cur = start;
+        do {
+            const int opVert = p.v1(cur);
+            p.count(CNT_PSEUDO_NEIGH, 1);
+            VInfo ov = p.vget(opVert);
+            if ((p.vflag(opVert) & 1) || ov.dist < pw.dist + p.len(cur)) {
+                cur = p.twin(cur);
+                if (cur != -1) cur = p.next(cur);
+                continue;
+            }
+            if (ov.dist == DBL_MAX) ++total;
+            ov.dist = pw.dist + p.len(cur);
+            ov.birth = (int8_t)(ov.birth + 1);
+            ov.enter = p.twin(cur);
+            p.vput(opVert, ov);
+            p.count(CNT_VERT_WRITE, 1);
+            Pseudo c;
+            c.vert = opVert; c.dist = ov.dist;
+            c.pseudoSrc = pw.vert;
+            c.birth = ov.birth;
+            c.level = pw.level;
+            push_pseudo(c);
+            cur = p.twin(cur);
+            if (cur != -1) cur = p.next(cur);
+        } while (cur != start && cur != -1);
+    }
+
+    // ICHDevice::Execute (ICHDevice.cu:67-132) for a vertex source.
+    SVG_HD void execute()
+    {
+        initialize();
+        const int nf = p.nf();
+        while (!p.wempty() || !p.pempty()) {
+            Win top;
+            bool haveTop = false;
+            while (!p.wempty()) {
+                p.wtop(top);
+                if (top.birth == (int)p.vget(top.pseudoSrc).birth) { haveTop = true; break; }
+                p.wpop();
+                p.count(CNT_WIN_POP, 1);
+            }
+            Pseudo ptop;
+            bool havePtop = false;
+            while (!p.pempty()) {
+                p.ptop(ptop);
+                if (ptop.birth == (int)p.vget(ptop.vert).birth) { havePtop = true; break; }
+                p.ppop();
+                p.count(CNT_PSEUDO_POP, 1);
+            }
+            if (haveTop && (!havePtop || top.minDist < ptop.dist)) {
+                p.wpop();
+                p.count(CNT_WIN_POP, 1);
+                if (top.level > nf) continue;
+                propagate(top);
+            } else if (havePtop && (!haveTop || top.minDist >= ptop.dist)) {
+                p.ppop();
+                p.count(CNT_PSEUDO_POP, 1);
+                if (ptop.level >= nf) continue;
+                gen_sub_wins(ptop);
+            }
+            if (K != -1 && total >= K) break;
+            if (p.stop()) break;
+        }
+    }
+
+    // One unfolding step (ICHDevice.cu:420-462).
+    SVG_HD bool unfold(Unfold& u)
+    {
+        const int e0 = p.twin(u.gid);
+        if (e0 == -1) return false;
+        Tri t;
+        p.face3(e0, t);
+        const double l0 = t.l0, l1 = t.l1, l2 = t.l2;
+        const double ox = (l0 * l0 + l2 * l2 - l1 * l1) / (2.0 * l0);
+        const double oy = sqrt(fabs(l2 * l2 - ox * ox));
+        // toLeft(opVert2D, lastPoint, curPoint) (Geometry.cu:306-311)
+        double ax = u.cpx - u.lpx, ay = u.cpy - u.lpy;
+        double bx = ox - u.lpx, by = oy - u.lpy;
+        const double la = len2(ax, ay);
+        if (!(fabs(la) < 1e-12)) { ax /= la; ay /= la; }
+        const double lb = len2(bx, by);
+        if (!(fabs(lb) < 1e-12)) { bx /= lb; by /= lb; }
+        const bool left = (ax * by - ay * bx) > 0.0;
+        const double f = u.gpos / l0;
+        if (left) {
+            const double p0x = (l2 * l2 + l1 * l1 - l0 * l0) / (2.0 * l1);
+            const double p0y = -sqrt(fabs(l2 * l2 - p0x * p0x));
+            const double p1x = l1, p1y = 0.0;
+            const double nlx = f * p0x + (1.0 - f) * p1x, nly = f * p0y + (1.0 - f) * p1y;
+            const double tt = intersect(u.lpx, u.lpy, u.cpx, u.cpy, l0, 0.0, ox, oy);
+            u.gpos = (1.0 - tt) * l1;
+            u.gid = t.e1;
+            u.cpx = l1 - u.gpos; u.cpy = 0.0;
+            u.lpx = nlx; u.lpy = nly;
+        } else {
+            const double p0x = 0.0, p0y = 0.0;
+            const double p1x = (l2 * l2 + l0 * l0 - l1 * l1) / (2.0 * l2);
+            const double p1y = -sqrt(fabs(l0 * l0 - p1x * p1x));
+            const double nlx = f * p0x + (1.0 - f) * p1x, nly = f * p0y + (1.0 - f) * p1y;
+            const double tt = intersect(u.lpx, u.lpy, u.cpx, u.cpy, ox, oy, 0.0, 0.0);
+            u.gpos = (1.0 - tt) * l2;
+            u.gid = t.e2;
+            u.cpx = l2 - u.gpos; u.cpy = 0.0;
+            u.lpx = nlx; u.lpy = nly;
+        }
+        return true;
+    }
+
+    // ICHDevice::BuildGeodesicPathTo(vertId, ...) (ICHDevice.cu:336-480) + D9.
+    // Returns pathPassVert; fills the two key points (edge -1 -> x = 0).
+    SVG_HD bool path_to(int vertId, int& srcEdge, double& srcX, int& dstEdge, double& dstX, int hopCap)
+    {
+        bool pass = false;
+        srcEdge = -1; dstEdge = -1; srcX = 0.0; dstX = 0.0;
+        int curVert = vertId;
+        int hops = 0;
+        VInfo cv = p.vget(curVert);
+        while (cv.dist != 0.0) {
+            if (++hops > hopCap) { p.count(CNT_ERRORS, 1); return true; }
+            p.count(CNT_BT_HOPS, 1);
+            const int enter = cv.enter;
+            if (enter == -1) {
+                if (dstEdge == -1) { dstEdge = p.first_edge(curVert); dstX = 0.0; }
+                else { srcEdge = p.first_edge(curVert); srcX = 0.0; }
+                return true;
+            } else if (p.v0(enter) == curVert) {
+                const int nextVert = p.v1(enter);
+                if (!p.vget(nextVert).isSource) {
+                    if (dstEdge == -1) { dstEdge = p.next(enter); dstX = 0.0; }
+                    else { srcEdge = p.next(enter); srcX = 0.0; }
+                    pass = true;
+                }
+                curVert = nextVert;
+            } else if (p.v1(enter) == curVert) {                       // D9
+                const int nextVert = p.v0(enter);
+                if (!p.vget(nextVert).isSource) {
+                    if (dstEdge == -1) { dstEdge = enter; dstX = 0.0; }
+                    else { srcEdge = enter; srcX = 0.0; }
+                    pass = true;
+                }
+                curVert = nextVert;
+            } else {
+                const SInfo sp = p.sget(enter);
+                Unfold u;
+                u.gid = enter;
+                u.gpos = sp.x;
+                if (dstEdge == -1) { dstEdge = u.gid; dstX = u.gpos; }
+                else { srcEdge = u.gid; srcX = u.gpos; }
+                const int tw = p.twin(u.gid);
+                if (tw == -1) { p.count(CNT_ERRORS, 1); return true; }
+                int opVert = p.v1(p.next(tw));
+                const double l0 = p.len(u.gid);
+                const double l1 = p.len(p.next(u.gid));
+                const double l2 = p.len(p.prev(u.gid));
+                u.lpx = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0);
+                u.lpy = -sqrt(fabs(l1 * l1 - u.lpx * u.lpx));
+                u.cpx = l0 - u.gpos; u.cpy = 0.0;
+                const int ps = sp.ps;
+                if (ps < 0 || ps >= p.nv()) { p.count(CNT_ERRORS, 1); return true; }
+                while (opVert != ps) {
+                    if (++hops > hopCap) { p.count(CNT_ERRORS, 1); return true; }
+                    p.count(CNT_BT_HOPS, 1);
+                    if (!unfold(u)) { p.count(CNT_ERRORS, 1); return true; }
+                    srcEdge = u.gid; srcX = u.gpos;
+                    const int tw2 = p.twin(u.gid);
+                    if (tw2 == -1) { p.count(CNT_ERRORS, 1); return true; }
+                    opVert = p.v1(p.next(tw2));
+                }
+                if (!p.vget(opVert).isSource) {
+                    srcEdge = p.first_edge(opVert); srcX = 0.0;
+                    pass = true;
+                }
+                curVert = opVert;
+            }
+            cv = p.vget(curVert);
+        }
+        return pass;
+    }
+};
+
+}  // namespace svg

@@ -1,0 +1,128 @@
+// Host half-edge builder (Mesh.cu:24-126 semantics, decisions D1/D2/D8).
+// Compiled with -ffp-contract=off so lengths and angle sums are the same
+// correctly rounded values on every build.
+#include "svg_mesh.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <unordered_map>
+
+namespace svg {
+
+static const double kPi = 3.141592653589793238;   // Geometry.cuh:18
+
+static double clamp_acos(double c)
+{
+    if (c > 1.0) c = 1.0; else if (c < -1.0) c = -1.0;
+    return std::acos(c);
+}
+
+std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int nf, HostMesh& m)
+{
+    if (nv <= 0 || nf <= 0) return "empty mesh";
+    if ((long long)nf * 3 > 0x7fffffffLL) return "too many faces";
+    for (long long i = 0; i < 3LL * nf; ++i)
+        if (tri[i] < 0 || tri[i] >= nv) return "triangle index out of range";
+    m.nv = nv; m.nf = nf; m.ne = 3 * nf;
+    m.xyz.assign(xyz, xyz + 3 * (size_t)nv);
+    m.tri.assign(tri, tri + 3 * (size_t)nf);
+    m.twin.assign(m.ne, -1);                                   // D1
+    m.len.resize(m.ne);
+    for (int f = 0; f < nf; ++f) {
+        for (int j = 0; j < 3; ++j) {
+            int a = tri[3 * f + j], b = tri[3 * f + (j + 1) % 3];
+            double x = xyz[3 * a] - xyz[3 * b];
+            double y = xyz[3 * a + 1] - xyz[3 * b + 1];
+            double z = xyz[3 * a + 2] - xyz[3 * b + 2];
+            m.len[3 * f + j] = std::sqrt(x * x + y * y + z * z);
+        }
+    }
+    // Twins: the first half-edge of each unordered vertex pair is kept and
+    // every later one is paired with it (Mesh.cu:79-89).
+    {
+        std::unordered_map<uint64_t, int32_t> first;
+        first.reserve((size_t)m.ne);
+        for (int e = 0; e < m.ne; ++e) {
+            uint64_t a = (uint32_t)tri[e], b = (uint32_t)tri[3 * (e / 3) + (e % 3 + 1) % 3];
+            if (a > b) std::swap(a, b);
+            uint64_t key = (a << 32) | b;
+            auto it = first.find(key);
+            if (it == first.end()) {
+                first.emplace(key, e);
+            } else {
+                m.twin[e] = it->second;
+                m.twin[it->second] = e;
+            }
+        }
+    }
+    auto next = [](int e) { return 3 * (e / 3) + (e % 3 + 1) % 3; };
+    auto prev = [](int e) { return 3 * (e / 3) + (e % 3 + 2) % 3; };
+    auto v0 = [&](int e) { return tri[e]; };
+    auto v1 = [&](int e) { return tri[next(e)]; };
+    m.angle.assign(nv, 0.0);
+    m.firstEdge.assign(nv, -1);
+    for (int e = 0; e < m.ne; ++e) {
+        double l0 = m.len[e], l1 = m.len[next(e)], l2 = m.len[prev(e)];
+        m.angle[v0(e)] += clamp_acos((l0 * l0 + l2 * l2 - l1 * l1) / (2.0 * l0 * l2));
+        m.firstEdge[v0(e)] = e;
+    }
+    std::vector<uint8_t> bnd(nv, 0);
+    for (int e = 0; e < m.ne; ++e)
+        if (m.twin[e] == -1) { bnd[v0(e)] = 1; bnd[v1(e)] = 1; }
+    for (int e = 0; e < m.ne; ++e)                             // D2
+        if (bnd[v0(e)] && m.twin[prev(e)] == -1) m.firstEdge[v0(e)] = e;
+    m.flags.assign(nv, 0);
+    for (int v = 0; v < nv; ++v) {
+        uint8_t f = 0;
+        if (bnd[v] || m.angle[v] < 2.0 * kPi) f |= VF_NOT_SADDLE;     // D8
+        if (!bnd[v] && m.angle[v] > 2.0 * kPi) f |= VF_SADDLE;
+        if (bnd[v]) f |= VF_BOUNDARY;
+        m.flags[v] = f;
+    }
+    // Corner angles used by the pseudo-source fan sweeps, with the operand
+    // order of ICHDevice.cu:830 / 847 (and 876 / 893):
+    //   A(e): L0 = len(e), L1 = len(prev e), L2 = len(next e)   (left sweep)
+    //   B(e): L0 = len(e), L1 = len(next e), L2 = len(prev e)   (right sweep)
+    m.cornerA.resize(m.ne);
+    m.cornerB.resize(m.ne);
+    for (int e = 0; e < m.ne; ++e) {
+        double L0 = m.len[e];
+        double La1 = m.len[prev(e)], La2 = m.len[next(e)];
+        m.cornerA[e] = clamp_acos((L0 * L0 + La1 * La1 - La2 * La2) / (2.0 * L0 * La1));
+        double Lb1 = m.len[next(e)], Lb2 = m.len[prev(e)];
+        m.cornerB[e] = clamp_acos((L0 * L0 + Lb1 * Lb1 - Lb2 * Lb2) / (2.0 * L0 * Lb1));
+    }
+    return std::string();
+}
+
+// OBJ reader with the reference's rules (Mesh.cu:41-60): "v x y z" lines and
+// "f a b c" lines with plain 1-based indices; other lines are skipped.
+std::string load_obj(const char* path, std::vector<double>& xyz, std::vector<int32_t>& tri)
+{
+    std::ifstream in(path);
+    if (!in) return std::string("cannot open file ") + path;
+    std::string line;
+    xyz.clear(); tri.clear();
+    while (std::getline(in, line)) {
+        if (line.size() < 2) continue;
+        std::istringstream ss(line);
+        char t;
+        if (line[0] == 'v' && (line[1] == ' ' || line[1] == '\t')) {
+            double x, y, z;
+            ss >> t >> x >> y >> z;
+            if (!ss) return "bad vertex line: " + line;
+            xyz.push_back(x); xyz.push_back(y); xyz.push_back(z);
+        } else if (line[0] == 'f') {
+            long a, b, c;
+            ss >> t >> a >> b >> c;
+            if (!ss) return "bad face line (only 'f a b c' is accepted): " + line;
+            tri.push_back((int32_t)(a - 1)); tri.push_back((int32_t)(b - 1)); tri.push_back((int32_t)(c - 1));
+        }
+    }
+    return std::string();
+}
+
+}  // namespace svg

@@ -1,0 +1,91 @@
+"""GPU parity of the SVG build (plain and batched) with the oracle.
+
+Bit-exact: neighbour ids, arc lengths, key-point edges and positions.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import svg_amd as S
+from svg_amd import meshgen as mg
+from tests.helpers import compare_rows, gpu_rows, oracle_build
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(xyz, tri, K, mode, **knobs):
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(K, 10, 2)
+    g.ConstructSVG(mode=mode, **knobs)
+    return m, g
+
+
+SMALL = {
+    "disk": (lambda: mg.planar_disk(), 40),
+    "cone": (lambda: mg.saddle_cone()[:2], 40),
+    "sphere642": (lambda: mg.noisy_sphere(seed=4, splits4=3, centroid_split=False), 60),
+    "torus40x24": (lambda: mg.bumpy_torus(nu=40, nv=24), 80),
+}
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+@pytest.mark.parametrize("mode", [S.SVG_MODE_PLAIN, S.SVG_MODE_BATCHED])
+def test_build_matches_oracle_small(name, mode):
+    (xyz, tri), K = SMALL[name][0](), SMALL[name][1]
+    print(f"case {name} mode {mode} V={len(xyz)} K={K}", flush=True)
+    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K)
+    m, g = _build(xyz, tri, K, mode)
+    bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+    st = g.status()
+    assert not np.any(st & 16), "error flags"
+
+
+def test_build_heightfield64_full_both_modes():
+    """The small plain-method workload: 64x64 heightfield, K=50, every source."""
+    xyz, tri = mg.heightfield64()
+    K = 50
+    print("case heightfield64 full", flush=True)
+    _, srcs, orows, otails, cnt = oracle_build(xyz, tri, K)
+    for mode in (S.SVG_MODE_PLAIN, S.SVG_MODE_BATCHED):
+        m, g = _build(xyz, tri, K, mode)
+        bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
+        assert not bad, (mode, bad[:10])
+
+
+def test_build_sphere30k_sample():
+    """noisy_sphere (V=30,722, K=100), batched, oracle on 200 spread sources."""
+    xyz, tri = mg.noisy_sphere()
+    K = 100
+    srcs = np.linspace(0, len(xyz) - 1, 200).astype(int)
+    print("case sphere30k", flush=True)
+    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
+    m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
+    bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+
+
+def test_build_torus72k_sample_and_properties():
+    """bumpy_torus (V=72,000, K=500): oracle parity on 64 sources, and
+    size-independent properties on every row."""
+    xyz, tri = mg.bumpy_torus()
+    K = 500
+    srcs = np.linspace(0, len(xyz) - 1, 64).astype(int)
+    print("case torus72k", flush=True)
+    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
+    m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
+    tails, h = gpu_rows(g)
+    bad = compare_rows((tails, h), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+    V = len(xyz)
+    assert np.all(tails <= K) and np.all(tails > 0)
+    mask = np.arange(K)[None, :] < tails[:, None]
+    adj = np.where(mask, h["adj"], np.iinfo(np.int32).max)
+    assert np.all(np.diff(adj, axis=1)[mask[:, 1:]] > 0), "rows are sorted by vertex id"
+    assert not np.any((h["adj"] == np.arange(V)[:, None]) & mask), "no self arcs"
+    eu = np.linalg.norm(xyz[np.where(mask, h["adj"], 0)] - xyz[:, None, :], axis=2)
+    assert np.all(h["dist"][mask] >= eu[mask] * (1 - 1e-12)), "arc >= chord"
+    st = g.status()
+    assert not np.any(st & (1 | 2 | 16)), "drops or errors"

@@ -1,0 +1,81 @@
+"""GPU SSSD (A*) / MSAD (Dijkstra) parity with the oracle's heap solvers."""
+import numpy as np
+import pytest
+
+import oracle as O
+import svg_amd as S
+from svg_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_graph():
+    xyz, tri = mg.noisy_sphere(seed=9, splits4=3, centroid_split=False)
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(40)
+    g.ConstructSVG()
+    h = g.CopySVGToHost()
+    rows = np.zeros(h["adj"].shape, dtype=O.ROW_DTYPE)
+    rows["adj"] = h["adj"]
+    rows["geodDist"] = h["dist"]
+    og = O.Graph(xyz, rows, h["tails"])
+    return m, g, og, h
+
+
+def test_sssd_matches_oracle_astar(small_graph):
+    m, g, og, _ = small_graph
+    rng = np.random.default_rng(0)
+    for _ in range(40):
+        s, t = (int(x) for x in rng.choice(og.V, 2, replace=False))
+        print(f"sssd {s}->{t}", flush=True)
+        d_gpu = g.SolveSSSD(s, t)
+        d_ref, *_ = og.sssd(s, t)
+        assert abs(d_gpu - d_ref) <= 1e-9 * d_ref, (s, t, d_gpu, d_ref)
+
+
+def test_sssd_path_reproduces_distance(small_graph):
+    m, g, og, h = small_graph
+    d, path = g.SolveSSSD(5, 400, path=True)
+    assert path[0] == 400 and path[-1] == 5
+    tot = 0.0
+    for a, b in zip(path[::-1][:-1], path[::-1][1:]):
+        r = h["adj"][a, : h["tails"][a]]
+        tot = tot + h["dist"][a, : h["tails"][a]][r == b][0]
+    assert tot == d
+
+
+@pytest.mark.parametrize("ns", [1, 3, 17])
+def test_msad_matches_oracle_dijkstra(small_graph, ns):
+    m, g, og, h = small_graph
+    rng = np.random.default_rng(ns)
+    src = rng.choice(og.V, ns, replace=False).astype(np.int32)
+    w = rng.uniform(0, 0.05, ns) if ns > 1 else np.zeros(1)
+    print(f"msad ns={ns}", flush=True)
+    dist, sid, par = g.SolveMSAD(src, w)
+    rdist, rpar, rsid, _ = og.msad(src, w)
+    assert np.array_equal(dist, rdist)
+    # parents reproduce every distance bit-for-bit and carry the source id
+    for v in range(og.V):
+        p = par[v]
+        if p == -1:
+            assert v in src
+            continue
+        r = h["adj"][p, : h["tails"][p]]
+        wv = h["dist"][p, : h["tails"][p]][r == v][0]
+        assert dist[p] + wv == dist[v]
+        assert sid[v] == sid[p]
+    agree = np.mean(sid == rsid)
+    assert agree > 0.99
+
+
+def test_sssd_batch_equals_single(small_graph):
+    m, g, og, _ = small_graph
+    rng = np.random.default_rng(7)
+    pairs = rng.integers(0, og.V, size=(50, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    out = g.SolveSSSDBatch(pairs)
+    for (s, t), d in zip(pairs, out):
+        assert d == g.SolveSSSD(int(s), int(t))
