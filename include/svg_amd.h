@@ -154,6 +154,11 @@ long long svg_server_submit(svg_server_t s, const svg_request *req);
 int svg_server_wait(svg_server_t s, long long ticket, double *latency_ms);
 int svg_server_stats_get(svg_server_t s, svg_server_stats *st);
 int svg_server_stop(svg_server_t s);
+/* Load driver: submits reqs[i] at arrival_s[i] seconds after the call
+ * starts and waits for all; latency_ms[i] is completion minus the scheduled
+ * arrival (-1 when the request was refused). */
+int svg_server_replay(svg_server_t s, const svg_request *reqs, const double *arrival_s, int n,
+                      double *latency_ms);
 
 #ifdef __cplusplus
 }

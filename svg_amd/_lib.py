@@ -65,7 +65,7 @@ EXPORTS = [
     "svg_build_params_default", "svg_build", "svg_build_stats_get", "svg_build_status",
     "svg_graph_info", "svg_graph_to_host", "svg_graph_save", "svg_graph_load", "svg_graph_free",
     "svg_sssd", "svg_msad", "svg_sssd_batch", "svg_server_start", "svg_server_submit",
-    "svg_server_wait", "svg_server_stats_get", "svg_server_stop",
+    "svg_server_wait", "svg_server_stats_get", "svg_server_stop", "svg_server_replay",
 ]
 
 _lib = None
@@ -115,6 +115,7 @@ def lib():
         L.svg_server_wait.argtypes = [_vp, C.c_longlong, C.POINTER(C.c_double)]
         L.svg_server_stats_get.argtypes = [_vp, C.POINTER(ServerStats)]
         L.svg_server_stop.argtypes = [_vp]
+        L.svg_server_replay.argtypes = [_vp, C.POINTER(Request), _dp, C.c_int, _dp]
     _lib = L
     return L
 

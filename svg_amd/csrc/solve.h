@@ -49,9 +49,9 @@ struct SolverState {
     int* fcount = nullptr;                // [2]
     SlotParam* sp = nullptr;              // S (device)
     SlotParam* spHost = nullptr;          // S (pinned)
-    int* seedsDev = nullptr;              // staging for seeds
-    double* seedWDev = nullptr;
-    int seedCap = 0;
+    void* stage = nullptr;                // admission staging (device)
+    void* stageHost = nullptr;            // admission staging (pinned)
+    size_t stageCap = 0;
     int cur = 0;
     uint32_t round = 1;
     size_t frontCap = 0;
@@ -61,6 +61,14 @@ struct SolverState {
 
 std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double delta, SolverState& s);
 void solver_destroy(SolverState& s);
+struct Admission {
+    int slot = 0, kind = Q_SSSD, target = -1;
+    bool wantParent = false;
+    std::vector<int> src;
+    std::vector<double> w;
+};
+// Seeds several slots with one init launch and one seed launch (stream 0).
+std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vector<Admission>& adm);
 // Seeds a slot (device work queued on stream 0).
 std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, const int* src,
                          const double* w, int ns, int target, bool wantParent);

@@ -63,30 +63,9 @@ __global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t*
     }
 }
 
-// Seeds of one slot, in order (SVG.cu:246-249, 359-365): a repeated source
-// keeps its last weight, as the reference's array writes do.
 __device__ __forceinline__ uint64_t pack_entry(int gen, int slot, int v)
 {
     return ((uint64_t)(uint16_t)gen << 48) | ((uint64_t)(uint16_t)slot << 32) | (uint64_t)(uint32_t)v;
-}
-
-__global__ void k_seed(SolveArgs a, int slot, const int* seeds, const double* wts, int ns, int gen)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const size_t base = (size_t)slot * a.V;
-    uint64_t* front = (uint64_t*)a.frontCur;
-    for (int i = 0; i < ns; ++i) {
-        const int v = seeds[i];
-        a.dist[base + v] = (unsigned long long)__double_as_longlong(wts[i]);
-        a.parent[base + v] = -1;
-        a.srcid[base + v] = v;
-        if (a.inq[base + v] == 0) {
-            a.inq[base + v] = 1;
-            const int pos = atomicAdd(&a.fcount[a.curIdx], 1);
-            if (pos < a.frontCap) front[pos] = pack_entry(gen, slot, v);
-            else atomicOr(a.overflow, 1);
-        }
-    }
 }
 
 // One round: each wave classifies 64 frontier entries (defer / drop /
@@ -363,9 +342,6 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
     for (auto& p : init) { std::memset(&p, 0, sizeof(p)); p.kind = -1; p.done = 1; p.minDeferred = ~0ull; }
     HTRY(hipMemcpy(s.sp, init.data(), sizeof(SlotParam) * (size_t)slots, hipMemcpyHostToDevice));
     std::memcpy(s.spHost, init.data(), sizeof(SlotParam) * (size_t)slots);
-    s.seedCap = 4096;
-    HTRY(hipMalloc(&s.seedsDev, sizeof(int) * (size_t)s.seedCap));
-    HTRY(hipMalloc(&s.seedWDev, sizeof(double) * (size_t)s.seedCap));
     s.cur = 0;
     s.round = 1;
     s.gens.assign((size_t)slots, 0);
@@ -376,59 +352,120 @@ void solver_destroy(SolverState& s)
 {
     hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid);
     for (int i = 0; i < 2; ++i) { hipFree(s.front[i]); hipFree(s.used[i]); }
-    hipFree(s.fcount); hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.seedsDev); hipFree(s.seedWDev);
+    hipFree(s.fcount); hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
     s = SolverState();
+}
+
+// Seeds of the admitted slots, one block each, in order (SVG.cu:246-249,
+// 359-365): a repeated source keeps its last weight, as the reference's
+// array writes do.  The first bucket bound is the smallest seed key + delta.
+__global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* __restrict__ kinds,
+                        const int* __restrict__ targets, const int* __restrict__ wantP,
+                        const int* __restrict__ gens, const int* __restrict__ off,
+                        const int* __restrict__ seeds, const double* __restrict__ wts, double delta)
+{
+    if (threadIdx.x != 0) return;
+    const int k = blockIdx.x;
+    const int slot = slots[k], kind = kinds[k], target = targets[k], gen = gens[k];
+    const size_t base = (size_t)slot * a.V;
+    uint64_t* front = (uint64_t*)a.frontCur;
+    double mk = DBL_MAX;
+    for (int i = off[k]; i < off[k + 1]; ++i) {
+        const int v = seeds[i];
+        const double w = wts[i];
+        a.dist[base + v] = (unsigned long long)__double_as_longlong(w);
+        a.parent[base + v] = -1;
+        a.srcid[base + v] = v;
+        if (a.inq[base + v] == 0) {
+            a.inq[base + v] = 1;
+            const int pos = atomicAdd(&a.fcount[a.curIdx], 1);
+            if (pos < a.frontCap) front[pos] = pack_entry(gen, slot, v);
+            else atomicOr(a.overflow, 1);
+        }
+        const double key = kind == Q_SSSD ? w + heur(a.pos, v, target) : w;
+        mk = key < mk ? key : mk;
+    }
+    SlotParam p;
+    p.kind = kind; p.target = target; p.wantParent = wantP[k]; p.done = 0;
+    p.thr = mk + delta; p.delta = delta; p.minDeferred = ~0ull;
+    p.expanded = 0; p.rounds = 0; p.gen = gen; p.pad = 0; p.relaxed = 0; p.settled = 0;
+    a.sp[slot] = p;
+}
+
+std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vector<Admission>& adm)
+{
+    const int n = (int)adm.size();
+    if (n == 0) return std::string();
+    HTRY(hipStreamSynchronize(0));      // the pinned staging buffer is free again
+    std::vector<int> ints;          // slots, kinds, targets, wantP, gens, off[n+1], seeds
+    ints.reserve((size_t)6 * n + 64);
+    size_t nseeds = 0;
+    for (const Admission& a : adm) {
+        if (a.slot < 0 || a.slot >= s.S) return "slot out of range";
+        if (a.src.empty() || a.src.size() != a.w.size()) return "bad seeds";
+        for (int v : a.src) if (v < 0 || v >= s.V) return "source out of range";
+        if (a.kind == Q_SSSD && (a.target < 0 || a.target >= s.V)) return "target out of range";
+        nseeds += a.src.size();
+    }
+    for (const Admission& a : adm) ints.push_back(a.slot);
+    for (const Admission& a : adm) ints.push_back(a.kind);
+    for (const Admission& a : adm) ints.push_back(a.kind == Q_SSSD ? a.target : -1);
+    for (const Admission& a : adm) ints.push_back(a.wantParent ? 1 : 0);
+    for (const Admission& a : adm) {
+        s.gens[a.slot] = (s.gens[a.slot] + 1) & 0xFFFF;
+        ints.push_back(s.gens[a.slot]);
+    }
+    int o = 0;
+    for (const Admission& a : adm) { ints.push_back(o); o += (int)a.src.size(); }
+    ints.push_back(o);
+    for (const Admission& a : adm) ints.insert(ints.end(), a.src.begin(), a.src.end());
+    const size_t intBytes = ((ints.size() * sizeof(int) + 15) / 16) * 16;
+    const size_t bytes = intBytes + nseeds * sizeof(double);
+    if (bytes > s.stageCap) {
+        hipFree(s.stage);
+        s.stageCap = std::max(bytes, (size_t)1 << 20);
+        HTRY(hipMalloc(&s.stage, s.stageCap));
+        hipHostFree(s.stageHost);
+        HTRY(hipHostMalloc(&s.stageHost, s.stageCap, hipHostMallocDefault));
+    }
+    std::memcpy(s.stageHost, ints.data(), ints.size() * sizeof(int));
+    double* wh = (double*)((char*)s.stageHost + intBytes);
+    for (const Admission& a : adm) for (double w : a.w) *wh++ = w;
+    HTRY(hipMemcpyAsync(s.stage, s.stageHost, bytes, hipMemcpyHostToDevice, 0));
+    const int* d = (const int*)s.stage;
+    const int* dSlots = d;
+    const int* dKinds = d + n;
+    const int* dTargets = d + 2 * n;
+    const int* dWant = d + 3 * n;
+    const int* dGens = d + 4 * n;
+    const int* dOff = d + 5 * n;
+    const int* dSeeds = d + 6 * n + 1;
+    const double* dW = (const double*)((const char*)s.stage + intBytes);
+    const size_t total = (size_t)n * s.V;
+    hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, 0,
+                       s.dist, s.parent, s.stamp, s.inq, s.srcid, dSlots, n, s.V);
+    DevGraph dummy;
+    SolveArgs args = make_args(s, dummy, m);
+    hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
+                       dSeeds, dW, s.delta);
+    s.launches += 2;
+    for (const Admission& a : adm) {
+        SlotParam& p = s.spHost[a.slot];
+        p.kind = a.kind; p.target = a.target; p.done = 0; p.wantParent = a.wantParent ? 1 : 0;
+        p.gen = s.gens[a.slot];
+    }
+    return std::string();
 }
 
 std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, const int* src,
                          const double* w, int ns, int target, bool wantParent)
 {
-    if (slot < 0 || slot >= s.S) return "slot out of range";
     if (ns <= 0) return "no sources";
-    for (int i = 0; i < ns; ++i)
-        if (src[i] < 0 || src[i] >= s.V) return "source out of range";
-    if (kind == Q_SSSD && (target < 0 || target >= s.V)) return "target out of range";
-    if (ns > s.seedCap) {
-        hipFree(s.seedsDev); hipFree(s.seedWDev);
-        s.seedCap = ns;
-        HTRY(hipMalloc(&s.seedsDev, sizeof(int) * (size_t)ns));
-        HTRY(hipMalloc(&s.seedWDev, sizeof(double) * (size_t)ns));
-    }
-    int* slotDev = nullptr;
-    HTRY(hipMalloc(&slotDev, sizeof(int)));
-    HTRY(hipMemcpy(slotDev, &slot, sizeof(int), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_slot_init, dim3(std::min(1024, (s.V + 255) / 256)), dim3(256), 0, 0,
-                       s.dist, s.parent, s.stamp, s.inq, s.srcid, slotDev, 1, s.V);
-    HTRY(hipMemcpy(s.seedsDev, src, sizeof(int) * (size_t)ns, hipMemcpyHostToDevice));
-    HTRY(hipMemcpy(s.seedWDev, w, sizeof(double) * (size_t)ns, hipMemcpyHostToDevice));
-    SlotParam p;
-    std::memset(&p, 0, sizeof(p));
-    p.kind = kind; p.target = target; p.wantParent = wantParent ? 1 : 0; p.done = 0;
-    p.delta = s.delta; p.minDeferred = ~0ull;
-    s.gens[slot] = (s.gens[slot] + 1) & 0xFFFF;
-    p.gen = s.gens[slot];
-    // first bucket: the smallest seed key plus delta
-    double mk = DBL_MAX;
-    std::vector<double> pos4;
-    if (kind == Q_SSSD) {
-        double pt[4], ps[4];
-        HTRY(hipMemcpy(pt, m.pos + 4 * (size_t)target, sizeof(pt), hipMemcpyDeviceToHost));
-        HTRY(hipMemcpy(ps, m.pos + 4 * (size_t)src[0], sizeof(ps), hipMemcpyDeviceToHost));
-        const double x = ps[0] - pt[0], y = ps[1] - pt[1], z = ps[2] - pt[2];
-        mk = w[0] + std::sqrt(x * x + y * y + z * z);
-    } else {
-        for (int i = 0; i < ns; ++i) mk = std::min(mk, w[i]);
-    }
-    p.thr = mk + s.delta;
-    HTRY(hipMemcpy(s.sp + slot, &p, sizeof(p), hipMemcpyHostToDevice));
-    s.spHost[slot] = p;
-    DevGraph dummy;
-    SolveArgs a = make_args(s, dummy, m);
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, 0, a, slot, s.seedsDev, s.seedWDev, ns, p.gen);
-    hipError_t e = hipDeviceSynchronize();
-    hipFree(slotDev);
-    if (e != hipSuccess) return std::string("seed: ") + hipGetErrorString(e);
-    return std::string();
+    std::vector<Admission> adm(1);
+    adm[0].slot = slot; adm[0].kind = kind; adm[0].target = target; adm[0].wantParent = wantParent;
+    adm[0].src.assign(src, src + ns);
+    adm[0].w.assign(w, w + ns);
+    return solver_admit_many(s, m, adm);
 }
 
 std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running)
@@ -470,13 +507,11 @@ std::string solver_fetch(SolverState& s, int slot, double* distOut, double* dist
     return std::string();
 }
 
+// A finished slot stays "done" on the device (its entries are dropped) until
+// it is admitted again; releasing it is host bookkeeping only.
 void solver_release_slot(SolverState& s, int slot)
 {
-    SlotParam p;
-    std::memset(&p, 0, sizeof(p));
-    p.kind = -1; p.done = 1; p.minDeferred = ~0ull; p.gen = s.gens[slot];
-    hipMemcpy(s.sp + slot, &p, sizeof(p), hipMemcpyHostToDevice);
-    s.spHost[slot] = p;
+    s.spHost[slot].kind = -1;
 }
 
 }  // namespace svg
@@ -582,13 +617,18 @@ extern "C" int svg_sssd_batch(svg_graph_t g, const int32_t* stp, int nq, double*
     int nextQ = 0, done = 0;
     const double w0 = 0.0;
     while (done < nq) {
+        std::vector<Admission> adm;
         for (int q = 0; q < st->S && nextQ < nq; ++q) {
             if (slotQuery[q] != -1) continue;
-            const int s = stp[2 * nextQ], t = stp[2 * nextQ + 1];
-            std::string e = solver_admit(*st, g->mesh->d, q, Q_SSSD, &s, &w0, 1, t, false);
-            if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
+            Admission a;
+            a.slot = q; a.kind = Q_SSSD; a.target = stp[2 * nextQ + 1]; a.wantParent = false;
+            a.src.push_back(stp[2 * nextQ]);
+            a.w.push_back(w0);
+            adm.push_back(std::move(a));
             slotQuery[q] = nextQ++;
         }
+        std::string ea = solver_admit_many(*st, g->mesh->d, adm);
+        if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
         int running = 0;
         std::string e = solver_round(*st, g->d, g->mesh->d, running);
         if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
