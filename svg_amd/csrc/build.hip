@@ -238,6 +238,11 @@ struct PlainPolicy {
         vt[k].srcId = (unsigned)srcVert;
     }
     __device__ void vtouch(int v) { unsigned k = vkey((unsigned)v); vt[k].index = v; }
+    __device__ int birth_of(int v) { return (int)vget(v).birth; }
+    __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& s)
+    {
+        va = vget(t.a); vb = vget(t.b); vo = vget(t.op); s = sget(t.e0);
+    }
     __device__ SInfo sget(int e)
     {
         const PSplitItem& it = sp[skey((unsigned)e)];
@@ -258,6 +263,7 @@ struct PlainPolicy {
         const PWin& it = wq[1].item;
         w.edge = (int)it.edgeID; w.b0 = it.b0; w.b1 = it.b1; w.d0 = it.d0; w.d1 = it.d1; w.ps = it.ps;
         win_finish(w);
+        w.twin = E[w.edge].twin;
         w.pseudoSrc = (int)it.pseudoSrcId; w.birth = it.birth; w.level = it.level;
     }
     __device__ void wpop()
@@ -472,11 +478,30 @@ std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& 
 // ---------------------------------------------------------------------------
 struct BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 B
 struct BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B
-struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, pad; };  // 56 B
+struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, twin; };  // 56 B
 struct alignas(8) BPsEnt { double key; int32_t vert, pseudoSrc, lvlb, pad; };   // 24 B
 struct BHeapEnt { double key; int32_t idx; int32_t pad; };     // 16 B
 
 constexpr uint32_t kEnterNone = 0xFFFFFFu;
+
+// Window-heap layout: logical binary-heap index i (1-based) is stored so that
+// every three consecutive levels below a band root share one 128-B block of
+// eight 16-B entries; a sift-down through the ~10 levels of a full heap then
+// touches about four memory lines instead of ten.  The heap semantics are the
+// reference's (only addresses change).
+constexpr int kHeapBlocks = 1 + 8 + 64 + 512;        // bands of levels 0-2, 3-5, 6-8, 9
+constexpr int kHeapSlots = kHeapBlocks * 8;
+
+__device__ __forceinline__ int bslot(int i)
+{
+    const int d = 31 - __clz(i);
+    const int b = d / 3;
+    const int r = d - 3 * b;
+    const int root = i >> r;
+    const int base = ((1 << (3 * b)) - 1) / 7;
+    const int block = base + (root - (1 << (3 * b)));
+    return block * 8 + ((1 << r) | (i & ((1 << r) - 1)));
+}
 
 __device__ __forceinline__ unsigned hmix(unsigned k) { return k * 2654435761u; }
 
@@ -495,8 +520,10 @@ struct BatchedPolicy {
     BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
     int wtail, wftop, wfresh, wcache;
     double rootKey; int rootIdx;
+    BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
     // pseudo heap
     BPsEnt* ph; int ptail;
+    BPsEnt pRoot;
     uint32_t flags;
     bool dead;
     int c_push, c_pop, c_ppush, c_ppop, c_prop, c_child, c_sw, c_vw, c_fan, c_nb, c_hops, c_err;
@@ -514,15 +541,30 @@ struct BatchedPolicy {
     __device__ int nv() const { return nv_; }
     __device__ int nf() const { return nf_; }
     __device__ bool stop() const { return dead; }
+    // The record is read as three 16-B vectors and rotated with selects, so
+    // nothing is indexed dynamically in registers (no scratch).
     __device__ void face3(int e0, Tri& t) const
     {
         const int f = e0 / 3, j = e0 - 3 * f;
-        const FaceRec r = F[f];
-        const int j1 = (j == 2) ? 0 : j + 1, j2 = (j == 0) ? 2 : j - 1;
-        t.e0 = e0; t.e1 = 3 * f + j1; t.e2 = 3 * f + j2;
-        t.l0 = r.l[j]; t.l1 = r.l[j1]; t.l2 = r.l[j2];
-        t.a = r.v[j]; t.b = r.v[j1]; t.op = r.v[j2];
-        t.tw1 = r.tw[j1]; t.tw2 = r.tw[j2];
+        const uint4* p = reinterpret_cast<const uint4*>(F + f);
+        const uint4 a = p[0], b = p[1], c = p[2];
+        const double L0 = __hiloint2double((int)a.y, (int)a.x);
+        const double L1 = __hiloint2double((int)a.w, (int)a.z);
+        const double L2 = __hiloint2double((int)b.y, (int)b.x);
+        const int V0 = (int)b.z, V1 = (int)b.w, V2 = (int)c.x;
+        const int T0 = (int)c.y, T1 = (int)c.z, T2 = (int)c.w;
+        const bool j0 = j == 0, j1 = j == 1;
+        t.e0 = e0;
+        t.e1 = 3 * f + (j0 ? 1 : (j1 ? 2 : 0));
+        t.e2 = 3 * f + (j0 ? 2 : (j1 ? 0 : 1));
+        t.l0 = j0 ? L0 : (j1 ? L1 : L2);
+        t.l1 = j0 ? L1 : (j1 ? L2 : L0);
+        t.l2 = j0 ? L2 : (j1 ? L0 : L1);
+        t.a = j0 ? V0 : (j1 ? V1 : V2);
+        t.b = j0 ? V1 : (j1 ? V2 : V0);
+        t.op = j0 ? V2 : (j1 ? V0 : V1);
+        t.tw1 = j0 ? T1 : (j1 ? T2 : T0);
+        t.tw2 = j0 ? T2 : (j1 ? T0 : T1);
     }
     __device__ void count(int which, long long n)
     {
@@ -545,11 +587,9 @@ struct BatchedPolicy {
     }
 
     // --- vertex table
-    __device__ VInfo vget(int v) const
+    __device__ VInfo vresolve(BSlotV s, unsigned h, int v) const
     {
-        unsigned h = hmix((unsigned)v) & vmask;
         for (;;) {
-            const BSlotV s = vt[h];
             if (s.key == v) {
                 VInfo r;
                 const uint32_t e24 = s.eb & kEnterNone;
@@ -561,7 +601,39 @@ struct BatchedPolicy {
             }
             if (s.key == -1) return vinfo_default();
             h = (h + 1) & vmask;
+            s = vt[h];
         }
+    }
+    __device__ VInfo vget(int v) const
+    {
+        const unsigned h = hmix((unsigned)v) & vmask;
+        return vresolve(vt[h], h, v);
+    }
+    // The source keeps birth time 0 from Initialize on: no relaxation can
+    // improve a distance of 0.
+    __device__ int birth_of(int v) const { return v == src ? 0 : (int)vget(v).birth; }
+    __device__ SInfo sresolve(unsigned h, int e) const
+    {
+        for (;;) {
+            const int32_t k = st[h].key;
+            if (k == e) {
+                SInfo r; r.dist = st[h].dist; r.x = st[h].x; r.ps = st[h].ps;
+                return r;
+            }
+            if (k == -1) return sinfo_default();
+            h = (h + 1) & smask;
+        }
+    }
+    __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp) const
+    {
+        const unsigned ha = hmix((unsigned)t.a) & vmask;
+        const unsigned hb = hmix((unsigned)t.b) & vmask;
+        const unsigned ho = hmix((unsigned)t.op) & vmask;
+        const BSlotV sa = vt[ha], sb = vt[hb], so = vt[ho];
+        sp = sresolve(hmix((unsigned)t.e0) & smask, t.e0);
+        va = vresolve(sa, ha, t.a);
+        vb = vresolve(sb, hb, t.b);
+        vo = vresolve(so, ho, t.op);
     }
     __device__ void vput(int v, const VInfo& x)
     {
@@ -584,19 +656,7 @@ struct BatchedPolicy {
     }
     __device__ void vtouch(int) {}
     // --- split table
-    __device__ SInfo sget(int e) const
-    {
-        unsigned h = hmix((unsigned)e) & smask;
-        for (;;) {
-            const int32_t k = st[h].key;
-            if (k == e) {
-                SInfo r; r.dist = st[h].dist; r.x = st[h].x; r.ps = st[h].ps;
-                return r;
-            }
-            if (k == -1) return sinfo_default();
-            h = (h + 1) & smask;
-        }
-    }
+    __device__ SInfo sget(int e) const { return sresolve(hmix((unsigned)e) & smask, e); }
     __device__ void sput(int e, const SInfo& x)
     {
         if (dead) return;
@@ -636,11 +696,11 @@ struct BatchedPolicy {
     __device__ bool wempty() const { return wtail == 1; }
     __device__ void wtop(Win& w) const
     {
-        const BWinRec r = wr[rootIdx];
+        const BWinRec r = (nrecIdx == rootIdx) ? nrec : wr[rootIdx];
         w.b0 = r.b0; w.b1 = r.b1; w.ps = r.ps; w.sx = r.sx; w.sy = r.sy;
         w.d0 = 0.0; w.d1 = 0.0;
         w.minDist = rootKey;
-        w.edge = r.edge; w.pseudoSrc = r.pseudoSrc;
+        w.edge = r.edge; w.pseudoSrc = r.pseudoSrc; w.twin = r.twin;
         w.birth = (int)(int8_t)(r.lvlb & 0xFF);
         w.level = r.lvlb >> 8;
     }
@@ -648,22 +708,24 @@ struct BatchedPolicy {
     {
         wrelease(rootIdx);
         --wtail;
-        const BHeapEnt top = wh[wtail];
+        const BHeapEnt top = wh[bslot(wtail)];
         int cur = 1, l = 2, r = 3, mc = -1;
         BHeapEnt cl, cr, cm;
-        if (r < wtail) { cl = wh[l]; cr = wh[r]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
-        else if (l < wtail) { cm = wh[l]; mc = l; }
+        if (r < wtail) { cl = wh[bslot(l)]; cr = wh[bslot(r)]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
+        else if (l < wtail) { cm = wh[bslot(l)]; mc = l; }
         bool movedRoot = false;
         while (mc != -1 && cm.key < top.key) {
-            wh[cur] = cm;
+            wh[bslot(cur)] = cm;
             if (cur == 1) { rootKey = cm.key; rootIdx = cm.idx; movedRoot = true; }
             cur = mc; l = cur * 2; r = cur * 2 + 1;
-            if (r < wtail) { cl = wh[l]; cr = wh[r]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
-            else if (l < wtail) { cm = wh[l]; mc = l; }
+            if (r < wtail) { cl = wh[bslot(l)]; cr = wh[bslot(r)]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
+            else if (l < wtail) { cm = wh[bslot(l)]; mc = l; }
             else mc = -1;
         }
-        wh[cur] = top;
+        wh[bslot(cur)] = top;
         if (!movedRoot) { rootKey = top.key; rootIdx = top.idx; }
+        if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
+        else nrecIdx = -1;
     }
     __device__ void wpush(const Win& w)
     {
@@ -672,26 +734,26 @@ struct BatchedPolicy {
         BWinRec rec;
         rec.b0 = w.b0; rec.b1 = w.b1; rec.ps = w.ps; rec.sx = w.sx; rec.sy = w.sy;
         rec.edge = w.edge; rec.pseudoSrc = w.pseudoSrc;
-        rec.lvlb = (w.level << 8) | (w.birth & 0xFF); rec.pad = 0;
+        rec.lvlb = (w.level << 8) | (w.birth & 0xFF); rec.twin = w.twin;
         wr[idx] = rec;
         ++wtail;
         int cur = wtail - 1, p = cur / 2;
         const double key = w.minDist;
         while (p > 0) {
-            const BHeapEnt pe = (p == 1) ? BHeapEnt{rootKey, rootIdx, 0} : wh[p];
+            const BHeapEnt pe = (p == 1) ? BHeapEnt{rootKey, rootIdx, 0} : wh[bslot(p)];
             if (!(pe.key > key)) break;
-            wh[cur] = pe;
+            wh[bslot(cur)] = pe;
             cur = p; p = cur / 2;
         }
         BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
-        wh[cur] = ne;
-        if (cur == 1) { rootKey = key; rootIdx = idx; }
+        wh[bslot(cur)] = ne;
+        if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
-    // --- pseudo heap (inline 24-B entries)
+    // --- pseudo heap (inline 24-B entries, root cached in registers)
     __device__ bool pempty() const { return ptail == 1; }
     __device__ void ptop(Pseudo& q) const
     {
-        const BPsEnt e = ph[1];
+        const BPsEnt e = pRoot;
         q.dist = e.key; q.vert = e.vert; q.pseudoSrc = e.pseudoSrc;
         q.birth = (int)(int8_t)(e.lvlb & 0xFF); q.level = e.lvlb >> 8;
     }
@@ -702,14 +764,18 @@ struct BatchedPolicy {
         int cur = 1, l = 2, r = 3, mc = -1;
         if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
         else if (l < ptail) mc = l;
+        bool movedRoot = false;
         while (mc != -1 && ph[mc].key < top.key) {
-            ph[cur] = ph[mc];
+            const BPsEnt c = ph[mc];
+            ph[cur] = c;
+            if (cur == 1) { pRoot = c; movedRoot = true; }
             cur = mc; l = cur * 2; r = cur * 2 + 1;
             if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
             else if (l < ptail) mc = l;
             else mc = -1;
         }
         ph[cur] = top;
+        if (!movedRoot) pRoot = top;
     }
     __device__ void ppush(const Pseudo& q)
     {
@@ -717,10 +783,16 @@ struct BatchedPolicy {
         ++ptail;
         int cur = ptail - 1, p = cur / 2;
         const double key = q.dist;
-        while (p > 0 && ph[p].key > key) { ph[cur] = ph[p]; cur = p; p = cur / 2; }
+        while (p > 0) {
+            const BPsEnt pe = (p == 1) ? pRoot : ph[p];
+            if (!(pe.key > key)) break;
+            ph[cur] = pe;
+            cur = p; p = cur / 2;
+        }
         BPsEnt e; e.key = key; e.vert = q.vert; e.pseudoSrc = q.pseudoSrc;
         e.lvlb = (q.level << 8) | (q.birth & 0xFF); e.pad = 0;
         ph[cur] = e;
+        if (cur == 1) pRoot = e;
     }
 };
 
@@ -736,7 +808,7 @@ struct BArgs {
     BSlotV* vtab;               // chunk * vcap
     int32_t* rlist;             // chunk * vcap
     BSlotS* stab;               // chunk * scap
-    BHeapEnt* wheap;            // lanes * 1024
+    BHeapEnt* wheap;            // lanes * kHeapSlots (blocked layout)
     BWinRec* wrec;              // lanes * 1024
     uint16_t* wfree;            // lanes * 1024
     BPsEnt* pheap;              // lanes * 1024
@@ -778,7 +850,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     if (lane >= a.lanes) return;
     BatchedPolicy p;
     policy_mesh(p, a);
-    p.wh = a.wheap + (size_t)lane * (kHeapCap + 1);
+    p.wh = a.wheap + (size_t)lane * kHeapSlots;
     p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
     p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
     p.ph = a.pheap + (size_t)lane * (kHeapCap + 1);
@@ -788,6 +860,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
         const int src = a.srcBase + slot;
         policy_tables(p, a, slot, src);
         p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
+        p.nrecIdx = -1;
         p.ptail = 1;
         Ich<BatchedPolicy> ich(p, src, a.K);
         ich.execute();
@@ -898,7 +971,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     const int block = c.propBlock > 0 ? c.propBlock : 64;
     size_t freeB = 0, totB = 0;
     hipMemGetInfo(&freeB, &totB);
-    const size_t perLane = (sizeof(BHeapEnt) + sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
+    const size_t perLane = sizeof(BHeapEnt) * kHeapSlots +
+                           (sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
     const size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
     int chunk = c.chunk > 0 ? c.chunk : V;
     {
@@ -918,7 +992,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.stab, sizeof(BSlotS) * scap * (size_t)chunk) != hipSuccess ||
-        hipMalloc(&a.wheap, sizeof(BHeapEnt) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wheap, sizeof(BHeapEnt) * kHeapSlots * (size_t)lanes) != hipSuccess ||
         hipMalloc(&a.wrec, sizeof(BWinRec) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
         hipMalloc(&a.wfree, sizeof(uint16_t) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
         hipMalloc(&a.pheap, sizeof(BPsEnt) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||

@@ -16,8 +16,9 @@
 //           int first_edge(v); uint8 vflag(v); double cornerA(e), cornerB(e);
 //           int nv(), nf();
 //           void face3(e0, Tri&) -- the triangle of half-edge e0 as seen from e0
-//   tables: VInfo vget(v); void vput(v, VInfo); void vtouch(v);
-//           SInfo sget(e); void sput(e, SInfo)
+//   tables: VInfo vget(v); void vput(v, VInfo); void vtouch(v); int birth_of(v);
+//           SInfo sget(e); void sput(e, SInfo);
+//           void lookup_face(Tri, VInfo& a, VInfo& b, VInfo& op, SInfo& split(e0))
 //   heaps:  bool wempty(); void wtop(Win&); void wpop(); void wpush(const Win&);
 //           bool pempty(); void ptop(Pseudo&); void ppop(); void ppush(const Pseudo&)
 //   counters: void count(int which, long long n)
@@ -49,6 +50,7 @@ enum Counter {
 struct Win {
     double b0, b1, d0, d1, ps, minDist, sx, sy;
     int edge;
+    int twin;                // twin of edge: the half-edge the window crosses into
     int pseudoSrc;
     int birth;
     int level;
@@ -143,6 +145,7 @@ struct Ich {
             w.ps = 0.0;
             win_finish(w);
             w.pseudoSrc = src; w.birth = 0; w.level = 0;
+            w.twin = p.twin(opEdge);
             push_win(w);
             const int opVert = p.v1(cur);
             p.vtouch(opVert);
@@ -207,19 +210,16 @@ struct Ich {
         w.level = fa.level + 1;
     }
 
-    // ICHDevice::IsValidWindow (ICHDevice.cu:906-955).  The child lies on edge
-    // `edge` of triangle t: its verts[0], verts[1] and opposite vertex and the
-    // lengths l0 = len(edge), l1 = len(next), l2 = len(next next).
-    SVG_HD bool is_valid(const Win& w, bool isLeft, int v1, int v2, int v3,
+    // ICHDevice::IsValidWindow (ICHDevice.cu:906-955).  The child lies on an
+    // edge whose verts[0], verts[1] and opposite vertex have distances dv1,
+    // dv2, dv3; l0 = len(edge), l1 = len(next), l2 = len(next next).
+    SVG_HD bool is_valid(const Win& w, bool isLeft, double dv1, double dv2, double dv3,
                          double l0, double l1, double l2)
     {
         if (w.b1 <= w.b0) return false;
         double p3x = (l2 * l2 + l0 * l0 - l1 * l1) / (2.0 * l0);
         double p3y = sqrt(fabs(l2 * l2 - p3x * p3x));
         double ax = w.b0, bx = w.b1;
-        double dv1 = p.vget(v1).dist;
-        double dv2 = p.vget(v2).dist;
-        double dv3 = p.vget(v3).dist;
         double srcB = w.ps + len2(w.sx - bx, w.sy - 0.0);
         double srcA = w.ps + len2(w.sx - ax, w.sy - 0.0);
         if (beaten(srcB, dv1 + w.b1, 0.0)) return false;
@@ -232,14 +232,20 @@ struct Ich {
         return true;
     }
 
-    // ICHDevice::PropagateWindow (ICHDevice.cu:618-726)
+    // ICHDevice::PropagateWindow (ICHDevice.cu:618-726).  The three vertex
+    // records of the far triangle and its split record are read together up
+    // front; the only one the step can change before IsValidWindow reads it
+    // again is the opposite vertex, whose new distance is then `cand`.
     SVG_HD void propagate(const Win& win)
     {
-        const int e0 = p.twin(win.edge);
+        const int e0 = win.twin;
         if (e0 == -1) return;
         p.count(CNT_PROPAGATE, 1);
         Tri t;
         p.face3(e0, t);
+        VInfo va, vb, vo;
+        SInfo sp;
+        p.lookup_face(t, va, vb, vo, sp);
         const double l0 = t.l0, l1 = t.l1, l2 = t.l2;
         const double sx = win.sx, sy = win.sy;
         const double leftx = win.b0, rightx = win.b1;
@@ -249,6 +255,7 @@ struct Ich {
         const double interX = v2x - v2y * (v2x - sx) / (v2y - sy);
         Win lw, rw;
         bool hasL = true, hasR = true;
+        double dOp = vo.dist;
         // child on e1 (left): edge e1 runs b -> op; its next is e2 (op -> a)
         // child on e2 (right): edge e2 runs op -> a; its next is e0 (a -> b)
         if (interX <= leftx) {
@@ -257,18 +264,17 @@ struct Ich {
             double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
             build_window(win, t.e2, l2, t0, t1, v2x, v2y, v1x, v1y, rw);
             p.count(CNT_CHILD_BUILT, 1);
-            if (!is_valid(rw, false, t.op, t.a, t.b, l2, l0, l1)) hasR = false;
+            if (!is_valid(rw, false, dOp, va.dist, vb.dist, l2, l0, l1)) hasR = false;
         } else if (interX >= rightx) {
             hasR = false;
             double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
             double t1 = intersect(sx, sy, rightx, 0.0, v0x, v0y, v2x, v2y);
             build_window(win, t.e1, l1, t0, t1, v0x, v0y, v2x, v2y, lw);
             p.count(CNT_CHILD_BUILT, 1);
-            if (!is_valid(lw, true, t.b, t.op, t.a, l1, l2, l0)) hasL = false;
+            if (!is_valid(lw, true, vb.dist, dOp, va.dist, l1, l2, l0)) hasL = false;
         } else {
             const double directDist = len2(v2x - sx, v2y - sy);
             const double cand = directDist + win.ps;
-            const SInfo sp = p.sget(e0);
             if (cand > sp.dist && cand / sp.dist - 1.0 > kRelErr) {
                 hasL = sp.x < interX;
                 hasR = !hasL;
@@ -281,13 +287,14 @@ struct Ich {
                     p.sput(e0, ns);
                     p.count(CNT_SPLIT_WRITE, 1);
                 }
-                VInfo ov = p.vget(t.op);
+                VInfo ov = vo;
                 if (cand < ov.dist) {
                     if (ov.dist == DBL_MAX) ++total;
                     ov.birth = (int8_t)(ov.birth + 1);
                     ov.dist = cand;
                     ov.enter = e0;
                     p.vput(t.op, ov);
+                    dOp = cand;
                     p.count(CNT_VERT_WRITE, 1);
                     if (p.vflag(t.op) & 2) {                        // angle > 2*PI
                         Pseudo q;
@@ -303,17 +310,17 @@ struct Ich {
                 double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
                 build_window(win, t.e1, l1, t0, 0.0, v0x, v0y, v2x, v2y, lw);
                 p.count(CNT_CHILD_BUILT, 1);
-                if (!is_valid(lw, true, t.b, t.op, t.a, l1, l2, l0)) hasL = false;
+                if (!is_valid(lw, true, vb.dist, dOp, va.dist, l1, l2, l0)) hasL = false;
             }
             if (hasR) {
                 double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
                 build_window(win, t.e2, l2, 1.0, t1, v2x, v2y, v1x, v1y, rw);
                 p.count(CNT_CHILD_BUILT, 1);
-                if (!is_valid(rw, false, t.op, t.a, t.b, l2, l0, l1)) hasR = false;
+                if (!is_valid(rw, false, dOp, va.dist, vb.dist, l2, l0, l1)) hasR = false;
             }
         }
-        if (hasL) push_win(lw);
-        if (hasR) push_win(rw);
+        if (hasL) { lw.twin = t.tw1; push_win(lw); }
+        if (hasR) { rw.twin = t.tw2; push_win(rw); }
     }
 
     // ICHDevice::GenSubWinsForPseudoSrcFromWindow (ICHDevice.cu:795-858)
@@ -401,6 +408,7 @@ struct Ich {
             w.pseudoSrc = pw.vert;
             w.birth = pw.birth;
             w.level = pw.level + 1;
+            w.twin = p.twin(w.edge);
             push_win(w);
             startEdge = p.twin(p.prev(startEdge));
             if (startEdge == -1 || ++guard > 4096) { p.count(CNT_ERRORS, 1); break; }
@@ -444,7 +452,7 @@ struct Ich {
             bool haveTop = false;
             while (!p.wempty()) {
                 p.wtop(top);
-                if (top.birth == (int)p.vget(top.pseudoSrc).birth) { haveTop = true; break; }
+                if (top.birth == p.birth_of(top.pseudoSrc)) { haveTop = true; break; }
                 p.wpop();
                 p.count(CNT_WIN_POP, 1);
             }
@@ -452,7 +460,7 @@ struct Ich {
             bool havePtop = false;
             while (!p.pempty()) {
                 p.ptop(ptop);
-                if (ptop.birth == (int)p.vget(ptop.vert).birth) { havePtop = true; break; }
+                if (ptop.birth == p.birth_of(ptop.vert)) { havePtop = true; break; }
                 p.ppop();
                 p.count(CNT_PSEUDO_POP, 1);
             }
