@@ -94,24 +94,32 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                 a.inq[o] = 0;
                 a.usedCur[f] = 0ull;
             } else {
-                dv = __longlong_as_double((long long)a.dist[o]);
+                // Take the entry first, then read the distance: an improvement
+                // that lands after the flag is cleared re-enters the frontier
+                // itself, and one that landed before is visible in the read.
+                // (acquire pairs with the release exchange of the improver)
+                __hip_atomic_exchange(&a.inq[o], 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long db =
+                    __hip_atomic_load(&a.dist[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                dv = __longlong_as_double((long long)db);
                 key = dv;
                 bool prune = false;
                 if (sp.kind == Q_SSSD) {
-                    const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+                    const double ub = __longlong_as_double((long long)__hip_atomic_load(
+                        &a.dist[(size_t)q * a.V + sp.target], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                     key = dv + heur(a.pos, v, sp.target);
                     prune = key * (1.0 - 1e-12) > ub;
                 }
+                a.usedCur[f] = 0ull;
                 if (prune) {
-                    a.inq[o] = 0;
-                    a.usedCur[f] = 0ull;
+                    // dropped: no path through v can beat the answer any more
                 } else if (key >= sp.thr) {
-                    act = 1;
-                    a.usedCur[f] = 0ull;
+                    // deferred: re-arm the flag unless an improvement already
+                    // put v back into the next frontier
+                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u) act = 1;
                     atomicMin(&a.sp[q].minDeferred, (unsigned long long)__double_as_longlong(key));
                 } else {
                     act = 2;
-                    a.inq[o] = 0;
                     a.usedCur[f] = (unsigned long long)__double_as_longlong(dv) | 0x8000000000000000ull;
                 }
             }
@@ -161,7 +169,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                         if (nb < old) {
                             a.stamp[sb + x] = a.round;
                             if (wantParent) a.parent[sb + x] = INT_MAX;
-                            push = atomicExch(&a.inq[sb + x], 1u) == 0u;
+                            push = __hip_atomic_exchange(&a.inq[sb + x], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u;
                         }
                     }
                 }
