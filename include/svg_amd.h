@@ -127,6 +127,7 @@ typedef struct {
     int max_inflight;       /* request slots resident on the device */
     long long mem_budget;   /* bytes of device memory for request state (0 = auto) */
     int max_batch;          /* requests merged into one frontier launch */
+    double delta_scale;     /* bucket width = delta_scale x mean arc length (0 = 1) */
 } svg_server_cfg;
 
 typedef struct {
@@ -145,6 +146,9 @@ typedef struct {
     double qps;
     int max_batch_seen;
     long long launches;
+    /* work of the completed queries, per kind (0 = SSSD, 1 = MSAD): queries,
+     * shared rounds they took part in, frontier entries expanded, arcs scanned */
+    long long n_kind[2], rounds_kind[2], settled_kind[2], relaxed_kind[2];
 } svg_server_stats;
 
 int svg_server_start(svg_graph_t g, const svg_server_cfg *cfg, svg_server_t *out);

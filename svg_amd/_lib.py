@@ -39,7 +39,8 @@ class BuildStats(C.Structure):
 
 
 class ServerCfg(C.Structure):
-    _fields_ = [("max_inflight", C.c_int), ("mem_budget", C.c_longlong), ("max_batch", C.c_int)]
+    _fields_ = [("max_inflight", C.c_int), ("mem_budget", C.c_longlong), ("max_batch", C.c_int),
+                ("delta_scale", C.c_double)]
 
 
 class Request(C.Structure):
@@ -51,7 +52,9 @@ class Request(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_longlong), ("refused", C.c_longlong), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("qps", C.c_double), ("max_batch_seen", C.c_int), ("launches", C.c_longlong)]
+                ("qps", C.c_double), ("max_batch_seen", C.c_int), ("launches", C.c_longlong),
+                ("n_kind", C.c_longlong * 2), ("rounds_kind", C.c_longlong * 2),
+                ("settled_kind", C.c_longlong * 2), ("relaxed_kind", C.c_longlong * 2)]
 
 
 COUNTER_NAMES = ["win_push", "win_pop", "pseudo_push", "pseudo_pop", "propagate",

@@ -68,6 +68,7 @@ struct svg_server_s {
     int maxBatchSeen = 0;
     Clock::time_point firstSubmit, lastDone;
     bool anySubmit = false;
+    long long nKind[2] = {0, 0}, roundsKind[2] = {0, 0}, settledKind[2] = {0, 0}, relaxedKind[2] = {0, 0};
     // gather staging
     int* gSlots = nullptr;
     int* gTargets = nullptr;
@@ -159,6 +160,12 @@ static void worker_loop(svg_server_s* S)
             } else {
                 solver_fetch(S->st, q, T.distOut, nullptr, T.srcidOut, nullptr);
             }
+            const SlotParam& sp = S->st.spHost[q];
+            const int kd = T.kind == Q_MSAD ? 1 : 0;
+            S->nKind[kd] += 1;
+            S->roundsKind[kd] += sp.rounds;
+            S->settledKind[kd] += sp.settled;
+            S->relaxedKind[kd] += sp.relaxed;
             solver_release_slot(S->st, q);
             const Clock::time_point now = Clock::now();
             std::lock_guard<std::mutex> lk(S->mu);
@@ -195,7 +202,8 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     S->slots = slots;
     S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
     S->queueCap = (size_t)std::max(1024, 64 * slots);
-    const double delta = graph_mean_arc(g->d);
+    const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
+    const double delta = graph_mean_arc(g->d) * scale;
     std::string e = solver_create(g->d, g->mesh->d, slots, delta, S->st);
     if (!e.empty()) { solver_destroy(S->st); delete S; return svg_set_error(SVG_E_MEM, e); }
     if (hipMalloc(&S->gSlots, sizeof(int) * 2 * (size_t)slots) != hipSuccess ||
@@ -279,6 +287,12 @@ extern "C" int svg_server_stats_get(svg_server_t S, svg_server_stats* st)
     st->qps = span > 0 ? (double)S->lat.size() / span : 0.0;
     st->max_batch_seen = S->maxBatchSeen;
     st->launches = S->st.launches;
+    for (int k = 0; k < 2; ++k) {
+        st->n_kind[k] = S->nKind[k];
+        st->rounds_kind[k] = S->roundsKind[k];
+        st->settled_kind[k] = S->settledKind[k];
+        st->relaxed_kind[k] = S->relaxedKind[k];
+    }
     return SVG_OK;
 }
 

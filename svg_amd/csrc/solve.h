@@ -46,7 +46,9 @@ struct SolverState {
     int32_t* srcid = nullptr;             // S*V
     uint64_t* front[2] = {nullptr, nullptr};   // entries: gen << 48 | slot << 32 | vertex
     unsigned long long* used[2] = {nullptr, nullptr};   // per entry: dist used when expanded (0 = deferred)
-    int* fcount = nullptr;                // [2]
+    int* fcount = nullptr;                // [0,1] frontier sizes, [2] overflow, [3] expand count
+    int* expList = nullptr;               // entries expanded this round
+    bool anyParent = false;               // some active slot keeps parents
     SlotParam* sp = nullptr;              // S (device)
     SlotParam* spHost = nullptr;          // S (pinned)
     void* stage = nullptr;                // admission staging (device)

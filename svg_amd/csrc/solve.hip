@@ -29,8 +29,9 @@ struct SolveArgs {
     uint32_t* inq;
     int32_t* srcid;
     const uint64_t* frontCur;
-    unsigned long long* usedCur;
+    unsigned long long* usedCur;   // per entry: the distance it was expanded with
     uint64_t* frontNext;
+    int* expList;                  // entries to expand this round
     int* fcount;          // [cur, next] as selected by curIdx
     int curIdx;
     SlotParam* sp;
@@ -68,41 +69,39 @@ __device__ __forceinline__ uint64_t pack_entry(int gen, int slot, int v)
     return ((uint64_t)(uint16_t)gen << 48) | ((uint64_t)(uint16_t)slot << 32) | (uint64_t)(uint32_t)v;
 }
 
-// One round: each wave classifies 64 frontier entries (defer / drop /
-// expand) and then expands the kept ones one row at a time, 64 arcs per
-// step; improved targets enter the next frontier once (inq flag).
-__global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
+// One round, pass 1: one lane per frontier entry.  Deferred entries move to
+// the next frontier; entries to expand are listed (with the distance they
+// expand with) for pass 2.
+__global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
     const int n = min(a.fcount[a.curIdx], a.frontCap);
     int* nextCount = &a.fcount[a.curIdx ^ 1];
-    for (int f0 = wave * 64; f0 < n; f0 += nw * 64) {
+    int* expCount = &a.fcount[3];
+    const int stride = gridDim.x * blockDim.x;
+    for (int f0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); f0 < n; f0 += stride) {
         const int f = f0 + lane;
-        int q = 0, v = 0, act = 0;           // act: 0 none, 1 defer, 2 expand
-        double dv = 0.0, key = 0.0;
+        int q = 0, act = 0;           // act: 0 none, 1 defer, 2 expand
         if (f < n) {
             const uint64_t e = a.frontCur[f];
             q = (int)((e >> 32) & 0xFFFFu);
-            v = (int)(e & 0xFFFFFFFFu);
+            const int v = (int)(e & 0xFFFFFFFFu);
             const SlotParam& sp = a.sp[q];
             const size_t o = (size_t)q * a.V + v;
             if (sp.kind < 0 || (int)(e >> 48) != (sp.gen & 0xFFFF)) {
-                a.usedCur[f] = 0ull;                 // entry of an earlier query in this slot
+                // entry of an earlier query in this slot: dropped
             } else if (sp.done) {
                 a.inq[o] = 0;
-                a.usedCur[f] = 0ull;
             } else {
                 // Take the entry first, then read the distance: an improvement
                 // that lands after the flag is cleared re-enters the frontier
-                // itself, and one that landed before is visible in the read.
-                // (acquire pairs with the release exchange of the improver)
+                // itself, and one that landed before is visible in the read
+                // (the acquire pairs with the improver's release exchange).
                 __hip_atomic_exchange(&a.inq[o], 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
                 const unsigned long long db =
                     __hip_atomic_load(&a.dist[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                dv = __longlong_as_double((long long)db);
-                key = dv;
+                const double dv = __longlong_as_double((long long)db);
+                double key = dv;
                 bool prune = false;
                 if (sp.kind == Q_SSSD) {
                     const double ub = __longlong_as_double((long long)__hip_atomic_load(
@@ -110,21 +109,20 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                     key = dv + heur(a.pos, v, sp.target);
                     prune = key * (1.0 - 1e-12) > ub;
                 }
-                a.usedCur[f] = 0ull;
                 if (prune) {
-                    // dropped: no path through v can beat the answer any more
+                    // no path through v can beat the answer any more
                 } else if (key >= sp.thr) {
                     // deferred: re-arm the flag unless an improvement already
                     // put v back into the next frontier
-                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u) act = 1;
+                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+                        act = 1;
                     atomicMin(&a.sp[q].minDeferred, (unsigned long long)__double_as_longlong(key));
                 } else {
                     act = 2;
-                    a.usedCur[f] = (unsigned long long)__double_as_longlong(dv) | 0x8000000000000000ull;
+                    a.usedCur[f] = db;
                 }
             }
         }
-        // deferred entries move to the next frontier unchanged
         const unsigned long long dm = __ballot(act == 1);
         if (dm) {
             int basePos = 0;
@@ -136,59 +134,80 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                 else atomicOr(a.overflow, 1);
             }
         }
-        unsigned long long em = __ballot(act == 2);
-        if (act == 2) atomicAdd(&a.sp[q].expanded, 1);
-        while (em) {
-            const int src = __ffsll((long long)em) - 1;
-            em &= em - 1;
-            const int qq = __shfl(q, src);
-            const int vv = __shfl(v, src);
-            const double dvv = __shfl(dv, src);
-            const SlotParam& sp = a.sp[qq];
-            const int kind = sp.kind;
-            const int t = sp.target;
-            const int wantParent = sp.wantParent;
-            const int gen = sp.gen;
-            const size_t sb = (size_t)qq * a.V;
-            const int tl = a.tail[vv];
-            const int tlr = (tl + 63) & ~63;
-            const size_t row = (size_t)vv * a.K;
-            double ub = DBL_MAX;
-            if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
-            for (int i = lane; i < tlr; i += 64) {
-                bool push = false;
-                int x = -1;
-                if (i < tl) {
-                    x = a.adj[row + i];
-                    const double nd = dvv + a.w[row + i];
-                    bool ok = true;
-                    if (kind == Q_SSSD) ok = !((nd + heur(a.pos, x, t) * (1.0 - 1e-12)) > ub);
-                    if (ok) {
-                        const unsigned long long nb = (unsigned long long)__double_as_longlong(nd);
-                        const unsigned long long old = atomicMin(&a.dist[sb + x], nb);
-                        if (nb < old) {
-                            a.stamp[sb + x] = a.round;
-                            if (wantParent) a.parent[sb + x] = INT_MAX;
-                            push = __hip_atomic_exchange(&a.inq[sb + x], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u;
-                        }
-                    }
-                }
-                const unsigned long long pm = __ballot(push);
-                if (pm) {
-                    int basePos = 0;
-                    if (lane == 0) basePos = atomicAdd(nextCount, __popcll(pm));
-                    basePos = __shfl(basePos, 0);
-                    const int ppos = basePos + __popcll(pm & ((1ull << lane) - 1ull));
-                    if (push) {
-                        if (ppos < a.frontCap) a.frontNext[ppos] = pack_entry(gen, qq, x);
-                        else atomicOr(a.overflow, 1);
+        const unsigned long long em = __ballot(act == 2);
+        if (em) {
+            int basePos = 0;
+            if (lane == 0) basePos = atomicAdd(expCount, __popcll(em));
+            basePos = __shfl(basePos, 0);
+            if (act == 2) {
+                a.expList[basePos + __popcll(em & ((1ull << lane) - 1ull))] = f;
+                atomicAdd(&a.sp[q].expanded, 1);
+            }
+        }
+    }
+}
+
+// One round, pass 2: one wave per listed entry, 64 arcs of its row per step.
+// A plain read of the target's distance filters the relaxations first; it
+// can only be stale-high (distances only fall), so no improvement is missed.
+__global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int n = a.fcount[3];
+    int* nextCount = &a.fcount[a.curIdx ^ 1];
+    for (int k = wave; k < n; k += nw) {
+        const int f = a.expList[k];
+        const uint64_t e = a.frontCur[f];
+        const int qq = (int)((e >> 32) & 0xFFFFu);
+        const int vv = (int)(e & 0xFFFFFFFFu);
+        const double dvv = __longlong_as_double((long long)a.usedCur[f]);
+        const SlotParam& sp = a.sp[qq];
+        const int kind = sp.kind;
+        const int t = sp.target;
+        const int wantParent = sp.wantParent;
+        const int gen = sp.gen;
+        const size_t sb = (size_t)qq * a.V;
+        const int tl = a.tail[vv];
+        const int tlr = (tl + 63) & ~63;
+        const size_t row = (size_t)vv * a.K;
+        double ub = DBL_MAX;
+        if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
+        for (int i = lane; i < tlr; i += 64) {
+            bool push = false;
+            int x = -1;
+            if (i < tl) {
+                x = a.adj[row + i];
+                const double nd = dvv + a.w[row + i];
+                const unsigned long long nb = (unsigned long long)__double_as_longlong(nd);
+                bool ok = nb < a.dist[sb + x];
+                if (ok && kind == Q_SSSD) ok = !((nd + heur(a.pos, x, t) * (1.0 - 1e-12)) > ub);
+                if (ok) {
+                    const unsigned long long old = atomicMin(&a.dist[sb + x], nb);
+                    if (nb < old) {
+                        a.stamp[sb + x] = a.round;
+                        if (wantParent) a.parent[sb + x] = INT_MAX;
+                        push = __hip_atomic_exchange(&a.inq[sb + x], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT) == 0u;
                     }
                 }
             }
-            if (lane == 0) {
-                atomicAdd((unsigned long long*)&a.sp[qq].relaxed, (unsigned long long)tl);
-                atomicAdd((unsigned long long*)&a.sp[qq].settled, 1ull);
+            const unsigned long long pm = __ballot(push);
+            if (pm) {
+                int basePos = 0;
+                if (lane == 0) basePos = atomicAdd(nextCount, __popcll(pm));
+                basePos = __shfl(basePos, 0);
+                const int ppos = basePos + __popcll(pm & ((1ull << lane) - 1ull));
+                if (push) {
+                    if (ppos < a.frontCap) a.frontNext[ppos] = pack_entry(gen, qq, x);
+                    else atomicOr(a.overflow, 1);
+                }
             }
+        }
+        if (lane == 0) {
+            atomicAdd((unsigned long long*)&a.sp[qq].relaxed, (unsigned long long)tl);
+            atomicAdd((unsigned long long*)&a.sp[qq].settled, 1ull);
         }
     }
 }
@@ -200,36 +219,23 @@ __global__ void __launch_bounds__(kSolveBlock) k_parent_fix(SolveArgs a)
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    const int n = min(a.fcount[a.curIdx], a.frontCap);
-    for (int f0 = wave * 64; f0 < n; f0 += nw * 64) {
-        const int f = f0 + lane;
-        unsigned long long u = 0ull;
-        int q = 0, v = 0;
-        if (f < n) {
-            u = a.usedCur[f];
-            const uint64_t e = a.frontCur[f];
-            q = (int)((e >> 32) & 0xFFFFu);
-            v = (int)(e & 0xFFFFFFFFu);
-            if (u && !a.sp[q].wantParent) u = 0ull;
-        }
-        unsigned long long em = __ballot(u != 0ull);
-        while (em) {
-            const int src = __ffsll((long long)em) - 1;
-            em &= em - 1;
-            const int qq = __shfl(q, src);
-            const int vv = __shfl(v, src);
-            const unsigned long long ub = __shfl(u, src) & 0x7FFFFFFFFFFFFFFFull;
-            const double dvv = __longlong_as_double((long long)ub);
-            const size_t sb = (size_t)qq * a.V;
-            const int tl = a.tail[vv];
-            const size_t row = (size_t)vv * a.K;
-            for (int i = lane; i < tl; i += 64) {
-                const int x = a.adj[row + i];
-                if (a.stamp[sb + x] != a.round) continue;
-                const double nd = dvv + a.w[row + i];
-                if ((unsigned long long)__double_as_longlong(nd) == a.dist[sb + x])
-                    atomicMin(&a.parent[sb + x], vv);
-            }
+    const int n = a.fcount[3];
+    for (int k = wave; k < n; k += nw) {
+        const int f = a.expList[k];
+        const uint64_t e = a.frontCur[f];
+        const int qq = (int)((e >> 32) & 0xFFFFu);
+        if (!a.sp[qq].wantParent) continue;
+        const int vv = (int)(e & 0xFFFFFFFFu);
+        const double dvv = __longlong_as_double((long long)a.usedCur[f]);
+        const size_t sb = (size_t)qq * a.V;
+        const int tl = a.tail[vv];
+        const size_t row = (size_t)vv * a.K;
+        for (int i = lane; i < tl; i += 64) {
+            const int x = a.adj[row + i];
+            if (a.stamp[sb + x] != a.round) continue;
+            const double nd = dvv + a.w[row + i];
+            if ((unsigned long long)__double_as_longlong(nd) == a.dist[sb + x])
+                atomicMin(&a.parent[sb + x], vv);
         }
     }
 }
@@ -257,7 +263,7 @@ __global__ void k_bookkeep(SolveArgs a, int S)
         sp.expanded = 0;
         sp.minDeferred = 0xFFFFFFFFFFFFFFFFull;
     }
-    if (threadIdx.x == 0) a.fcount[a.curIdx] = 0;
+    if (threadIdx.x == 0) { a.fcount[a.curIdx] = 0; a.fcount[3] = 0; }
 }
 
 __global__ void k_resolve_src(const int32_t* parent, int32_t* srcid, const unsigned long long* dist, int V,
@@ -315,7 +321,7 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
     a.dist = s.dist; a.parent = s.parent; a.stamp = s.stamp; a.inq = s.inq; a.srcid = s.srcid;
     a.frontCur = s.front[s.cur]; a.usedCur = s.used[s.cur]; a.frontNext = s.front[s.cur ^ 1];
     a.fcount = s.fcount; a.curIdx = s.cur; a.sp = s.sp; a.round = s.round;
-    a.frontCap = (int)s.frontCap; a.overflow = s.fcount + 2;
+    a.frontCap = (int)s.frontCap; a.overflow = s.fcount + 2; a.expList = s.expList;
     return a;
 }
 
@@ -342,6 +348,7 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
         HTRY(hipMalloc(&s.front[i], sizeof(uint64_t) * s.frontCap));
         HTRY(hipMalloc(&s.used[i], sizeof(unsigned long long) * s.frontCap));
     }
+    HTRY(hipMalloc(&s.expList, sizeof(int) * s.frontCap));
     HTRY(hipMalloc(&s.fcount, sizeof(int) * 4));
     HTRY(hipMemset(s.fcount, 0, sizeof(int) * 4));
     HTRY(hipMalloc(&s.sp, sizeof(SlotParam) * (size_t)slots));
@@ -360,7 +367,7 @@ void solver_destroy(SolverState& s)
 {
     hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid);
     for (int i = 0; i < 2; ++i) { hipFree(s.front[i]); hipFree(s.used[i]); }
-    hipFree(s.fcount); hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
+    hipFree(s.fcount); hipFree(s.expList); hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
     s = SolverState();
 }
 
@@ -479,10 +486,14 @@ std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, c
 std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running)
 {
     SolveArgs a = make_args(s, g, m);
-    hipLaunchKernelGGL(k_expand, dim3(1024), dim3(kSolveBlock), 0, 0, a);
-    hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, 0, a);
+    s.anyParent = false;
+    for (int q = 0; q < s.S; ++q)
+        if (s.spHost[q].kind >= 0 && !s.spHost[q].done && s.spHost[q].wantParent) s.anyParent = true;
+    hipLaunchKernelGGL(k_classify, dim3(512), dim3(kSolveBlock), 0, 0, a);
+    hipLaunchKernelGGL(k_expand, dim3(2048), dim3(kSolveBlock), 0, 0, a);
+    if (s.anyParent) hipLaunchKernelGGL(k_parent_fix, dim3(2048), dim3(kSolveBlock), 0, 0, a);
     hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(256), 0, 0, a, s.S);
-    s.launches += 3;
+    s.launches += s.anyParent ? 4 : 3;
     hipError_t e = hipMemcpy(s.spHost, s.sp, sizeof(SlotParam) * (size_t)s.S, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return std::string("solver round: ") + hipGetErrorString(e);
     int ov = 0;
