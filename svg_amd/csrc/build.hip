@@ -478,32 +478,38 @@ std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& 
 // ---------------------------------------------------------------------------
 struct BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 B
 struct BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B
-struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, twin; };  // 56 B
+struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, twin; double pad; };  // 64 B
+static_assert(sizeof(BWinRec) == 64, "window records fill one 64-B half-line");
 struct alignas(8) BPsEnt { double key; int32_t vert, pseudoSrc, lvlb, pad; };   // 24 B
 struct BHeapEnt { double key; int32_t idx; int32_t pad; };     // 16 B
 
 constexpr uint32_t kEnterNone = 0xFFFFFFu;
 
-// Window-heap layout: logical binary-heap index i (1-based) is stored so that
-// every three consecutive levels below a band root share one 128-B block of
-// eight 16-B entries; a sift-down through the ~10 levels of a full heap then
-// touches about four memory lines instead of ten.  The heap semantics are the
-// reference's (only addresses change).
-constexpr int kHeapBlocks = 1 + 8 + 64 + 512;        // bands of levels 0-2, 3-5, 6-8, 9
+// Window-heap placement: logical binary-heap index i (1-based) keeps the
+// reference's semantics; only where entries live changes.  Levels 0-3
+// (i < 16) sit in LDS, lane-major so a wave's lanes hit distinct banks;
+// deeper levels sit in HBM with every three levels below a band root in one
+// 128-B block of eight 16-B entries (levels 4-6: 16 blocks, 7-9: 128 blocks),
+// so a sift-down through a full heap touches about two memory lines.
+constexpr int kHeapLds = 16;                          // logical 1..15 in LDS
+constexpr int kHeapBlocks = 16 + 128;
 constexpr int kHeapSlots = kHeapBlocks * 8;
 
-__device__ __forceinline__ int bslot(int i)
+__device__ __forceinline__ int bslot(int i)           // i >= 16
 {
-    const int d = 31 - __clz(i);
-    const int b = d / 3;
-    const int r = d - 3 * b;
+    const int d = 31 - __clz(i);                      // 4..9
+    const int b = d >= 7 ? 1 : 0;
+    const int r = d - 4 - 3 * b;
     const int root = i >> r;
-    const int base = ((1 << (3 * b)) - 1) / 7;
-    const int block = base + (root - (1 << (3 * b)));
+    const int block = b ? 16 + (root - 128) : (root - 16);
     return block * 8 + ((1 << r) | (i & ((1 << r) - 1)));
 }
 
 __device__ __forceinline__ unsigned hmix(unsigned k) { return k * 2654435761u; }
+// Vertex-table slot: XOR-folding the high bits into the low ones keeps every
+// aligned group of eight consecutive ids inside one 128-B line of slots, so
+// the vertices of one face and of neighbouring faces share lines.
+__device__ __forceinline__ unsigned vslot(unsigned v) { return v ^ (v >> 11) ^ (v >> 22); }
 
 struct BatchedPolicy {
     const FaceRec* __restrict__ F;
@@ -518,9 +524,24 @@ struct BatchedPolicy {
     BSlotS* st; unsigned smask; int scount; int srefcap; int slimit;
     // window heap
     BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
+    double* lk; int* li; int lstride;   // LDS part of the heap (logical 1..15), lane-major
     int wtail, wftop, wfresh, wcache;
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
+
+    __device__ BHeapEnt hget(int i) const
+    {
+        if (i < kHeapLds) {
+            BHeapEnt e; e.key = lk[(i - 1) * lstride]; e.idx = li[(i - 1) * lstride]; e.pad = 0;
+            return e;
+        }
+        return wh[bslot(i)];
+    }
+    __device__ void hset(int i, const BHeapEnt& e)
+    {
+        if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
+        else wh[bslot(i)] = e;
+    }
     // pseudo heap
     BPsEnt* ph; int ptail;
     BPsEnt pRoot;
@@ -606,7 +627,7 @@ struct BatchedPolicy {
     }
     __device__ VInfo vget(int v) const
     {
-        const unsigned h = hmix((unsigned)v) & vmask;
+        const unsigned h = vslot((unsigned)v) & vmask;
         return vresolve(vt[h], h, v);
     }
     // The source keeps birth time 0 from Initialize on: no relaxation can
@@ -626,9 +647,9 @@ struct BatchedPolicy {
     }
     __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp) const
     {
-        const unsigned ha = hmix((unsigned)t.a) & vmask;
-        const unsigned hb = hmix((unsigned)t.b) & vmask;
-        const unsigned ho = hmix((unsigned)t.op) & vmask;
+        const unsigned ha = vslot((unsigned)t.a) & vmask;
+        const unsigned hb = vslot((unsigned)t.b) & vmask;
+        const unsigned ho = vslot((unsigned)t.op) & vmask;
         const BSlotV sa = vt[ha], sb = vt[hb], so = vt[ho];
         sp = sresolve(hmix((unsigned)t.e0) & smask, t.e0);
         va = vresolve(sa, ha, t.a);
@@ -638,7 +659,7 @@ struct BatchedPolicy {
     __device__ void vput(int v, const VInfo& x)
     {
         if (dead) return;
-        unsigned h = hmix((unsigned)v) & vmask;
+        unsigned h = vslot((unsigned)v) & vmask;
         for (;;) {
             const int32_t k = vt[h].key;
             if (k == v) break;
@@ -696,7 +717,9 @@ struct BatchedPolicy {
     __device__ bool wempty() const { return wtail == 1; }
     __device__ void wtop(Win& w) const
     {
-        const BWinRec r = (nrecIdx == rootIdx) ? nrec : wr[rootIdx];
+        // Every change of the root refreshes nrec (wpop loads it, wpush
+        // copies the new record), so the root's record is always at hand.
+        const BWinRec& r = nrec;
         w.b0 = r.b0; w.b1 = r.b1; w.ps = r.ps; w.sx = r.sx; w.sy = r.sy;
         w.d0 = 0.0; w.d1 = 0.0;
         w.minDist = rootKey;
@@ -708,21 +731,34 @@ struct BatchedPolicy {
     {
         wrelease(rootIdx);
         --wtail;
-        const BHeapEnt top = wh[bslot(wtail)];
-        int cur = 1, l = 2, r = 3, mc = -1;
-        BHeapEnt cl, cr, cm;
-        if (r < wtail) { cl = wh[bslot(l)]; cr = wh[bslot(r)]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
-        else if (l < wtail) { cm = wh[bslot(l)]; mc = l; }
+        const BHeapEnt top = hget(wtail);
+        // Children are compared field by field (no select between two
+        // aggregates, which would put them on the stack).
+        int cur = 1, mc = -1;
+        double mk = 0.0; int mi = 0;
+        auto pick = [&](int c) {
+            const int l = 2 * c, r = l + 1;
+            if (r < wtail) {
+                const BHeapEnt cl = hget(l), cr = hget(r);
+                const bool left = cl.key < cr.key;
+                mc = left ? l : r; mk = left ? cl.key : cr.key; mi = left ? cl.idx : cr.idx;
+            } else if (l < wtail) {
+                const BHeapEnt cl = hget(l);
+                mc = l; mk = cl.key; mi = cl.idx;
+            } else {
+                mc = -1;
+            }
+        };
+        pick(1);
         bool movedRoot = false;
-        while (mc != -1 && cm.key < top.key) {
-            wh[bslot(cur)] = cm;
-            if (cur == 1) { rootKey = cm.key; rootIdx = cm.idx; movedRoot = true; }
-            cur = mc; l = cur * 2; r = cur * 2 + 1;
-            if (r < wtail) { cl = wh[bslot(l)]; cr = wh[bslot(r)]; mc = cl.key < cr.key ? l : r; cm = cl.key < cr.key ? cl : cr; }
-            else if (l < wtail) { cm = wh[bslot(l)]; mc = l; }
-            else mc = -1;
+        while (mc != -1 && mk < top.key) {
+            BHeapEnt m; m.key = mk; m.idx = mi; m.pad = 0;
+            hset(cur, m);
+            if (cur == 1) { rootKey = mk; rootIdx = mi; movedRoot = true; }
+            cur = mc;
+            pick(cur);
         }
-        wh[bslot(cur)] = top;
+        hset(cur, top);
         if (!movedRoot) { rootKey = top.key; rootIdx = top.idx; }
         if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
         else nrecIdx = -1;
@@ -740,13 +776,13 @@ struct BatchedPolicy {
         int cur = wtail - 1, p = cur / 2;
         const double key = w.minDist;
         while (p > 0) {
-            const BHeapEnt pe = (p == 1) ? BHeapEnt{rootKey, rootIdx, 0} : wh[bslot(p)];
+            const BHeapEnt pe = hget(p);
             if (!(pe.key > key)) break;
-            wh[bslot(cur)] = pe;
+            hset(cur, pe);
             cur = p; p = cur / 2;
         }
         BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
-        wh[bslot(cur)] = ne;
+        hset(cur, ne);
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
     // --- pseudo heap (inline 24-B entries, root cached in registers)
@@ -784,7 +820,7 @@ struct BatchedPolicy {
         int cur = ptail - 1, p = cur / 2;
         const double key = q.dist;
         while (p > 0) {
-            const BPsEnt pe = (p == 1) ? pRoot : ph[p];
+            const BPsEnt pe = ph[p];
             if (!(pe.key > key)) break;
             ph[cur] = pe;
             cur = p; p = cur / 2;
@@ -846,10 +882,14 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
 
 __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
 {
+    extern __shared__ double ldsHeap[];       // (kHeapLds-1) keys then idx per thread
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.lanes) return;
     BatchedPolicy p;
     policy_mesh(p, a);
+    p.lstride = blockDim.x;
+    p.lk = ldsHeap + threadIdx.x;
+    p.li = reinterpret_cast<int*>(ldsHeap + (kHeapLds - 1) * blockDim.x) + threadIdx.x;
     p.wh = a.wheap + (size_t)lane * kHeapSlots;
     p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
     p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
@@ -1019,7 +1059,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         hipEventRecord(e0, 0);
         const int pl = std::min(lanes, n);
         a.lanes = pl;
-        hipLaunchKernelGGL(batched_prop_kernel, dim3((pl + block - 1) / block), dim3(block), 0, 0, a);
+        const size_t ldsBytes = (size_t)(kHeapLds - 1) * block * (sizeof(double) + sizeof(int));
+        hipLaunchKernelGGL(batched_prop_kernel, dim3((pl + block - 1) / block), dim3(block), ldsBytes, 0, a);
         hipEventRecord(e1, 0);
         const int eblocks = std::min(n, 16384);
         if (vcap <= 1024)

@@ -82,6 +82,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
     for (int f0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); f0 < n; f0 += stride) {
         const int f = f0 + lane;
         int q = 0, act = 0;           // act: 0 none, 1 defer, 2 expand
+        bool hasDef = false;
+        unsigned long long defKey = ~0ull;
         if (f < n) {
             const uint64_t e = a.frontCur[f];
             q = (int)((e >> 32) & 0xFFFFu);
@@ -116,7 +118,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                     // put v back into the next frontier
                     if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u)
                         act = 1;
-                    atomicMin(&a.sp[q].minDeferred, (unsigned long long)__double_as_longlong(key));
+                    defKey = (unsigned long long)__double_as_longlong(key);
+                    hasDef = true;
                 } else {
                     act = 2;
                     a.usedCur[f] = db;
@@ -139,9 +142,26 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             int basePos = 0;
             if (lane == 0) basePos = atomicAdd(expCount, __popcll(em));
             basePos = __shfl(basePos, 0);
-            if (act == 2) {
-                a.expList[basePos + __popcll(em & ((1ull << lane) - 1ull))] = f;
-                atomicAdd(&a.sp[q].expanded, 1);
+            if (act == 2) a.expList[basePos + __popcll(em & ((1ull << lane) - 1ull))] = f;
+        }
+        // One update per slot present in this wave instead of one per entry:
+        // the per-slot counters are single addresses every lane would hit.
+        unsigned long long todo = __ballot(hasDef || act == 2);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lq = __shfl(q, leader);
+            const bool mine = (hasDef || act == 2) && q == lq;
+            const unsigned long long grp = __ballot(mine);
+            todo &= ~grp;
+            unsigned long long k = mine && hasDef ? defKey : ~0ull;
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(k, off);
+                k = o < k ? o : k;
+            }
+            const bool anyExp = __ballot(mine && act == 2) != 0ull;
+            if (lane == leader) {
+                if (k != ~0ull) atomicMin(&a.sp[lq].minDeferred, k);
+                if (anyExp) a.sp[lq].expanded = 1;
             }
         }
     }
