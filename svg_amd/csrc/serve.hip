@@ -190,9 +190,7 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     if (!S) return svg_set_error(SVG_E_MEM, "host allocation failed");
     S->g = g;
     int slots = cfg && cfg->max_inflight > 0 ? cfg->max_inflight : 64;
-    // per-slot device bytes: dist, parent, stamp, inq, srcid and two
-    // frontier entries (+ their expansion values) per vertex, twice
-    const size_t perSlot = (size_t)g->d.V * (8 + 4 + 4 + 4 + 4 + 2 * 2 * (8 + 8));
+    const size_t perSlot = solver_bytes_per_slot(g->d.V);
     size_t freeB = 0, totB = 0;
     hipMemGetInfo(&freeB, &totB);
     size_t budget = cfg && cfg->mem_budget > 0 ? (size_t)cfg->mem_budget : freeB / 4;

@@ -27,8 +27,8 @@ struct SlotParam {              // device-resident, one per slot
     int done;
     double thr;                 // current bucket bound (key < thr is expanded)
     double delta;
-    unsigned long long minDeferred;   // bits of the smallest deferred key
-    int expanded;               // entries expanded this round
+    unsigned long long minDeferred;   // bits of the smallest deferred key (last round)
+    int expanded;               // anything expanded in the last round
     int rounds;
     int gen;                    // generation tag of this slot's frontier entries
     int pad;
@@ -36,31 +36,79 @@ struct SlotParam {              // device-resident, one per slot
     long long settled;          // entries expanded
 };
 
+constexpr int kCBlocks = 512;     // classify blocks = deferred-list regions
+constexpr int kPLists = 8;        // push lists (and won lists)
+constexpr int kCntPad = 32;       // counters one 128-B line apart
+constexpr int kAccCopies = 16;    // copies of the per-slot statistics
+
+// Per-slot statistics of a round, spread over kAccCopies copies so the
+// atomics of one slot do not all meet at one address.
+struct SlotAcc {
+    unsigned long long minDeferred;
+    unsigned long long settled;
+    unsigned long long relaxed;
+    int expanded;
+    int pad;
+};
+
+// A row to relax: the frontier entry and the distance it expands with.
+struct ExpEnt {
+    uint64_t entry;
+    unsigned long long dist;
+};
+
+// An improvement (or exact tie) found by a relaxation, kept so the round's
+// parents can be resolved without rescanning the rows.
+struct WonEnt {
+    int32_t slot, x, u, pad;
+    unsigned long long nb;
+};
+
+// Device-resident list sizes.  Index [p] is a round parity.
+struct SolveCounts {
+    int dcnt[2][kCBlocks];            // deferred entries per classify region
+    int dstride[2];                   // region stride of the deferred lists
+    int pcnt[2][kPLists * kCntPad];   // push-list sizes (padded)
+    int ecnt[kCBlocks];               // rows per classify region
+    int estride;
+    int wcnt[kPLists * kCntPad];      // won-list sizes (padded)
+    int overflow;
+};
+
 struct SolverState {
     int V = 0, K = 0, S = 0;
     double delta = 0.0;
+    int roundsPerSync = 4;                // rounds queued between two host checks
     unsigned long long* dist = nullptr;   // S*V bits of double
     int32_t* parent = nullptr;            // S*V
     uint32_t* stamp = nullptr;            // S*V round of last improvement
     uint32_t* inq = nullptr;              // S*V frontier membership
     int32_t* srcid = nullptr;             // S*V
-    uint64_t* front[2] = {nullptr, nullptr};   // entries: gen << 48 | slot << 32 | vertex
-    unsigned long long* used[2] = {nullptr, nullptr};   // per entry: dist used when expanded (0 = deferred)
-    int* fcount = nullptr;                // [0,1] frontier sizes, [2] overflow, [3] expand count
-    int* expList = nullptr;               // entries expanded this round
+    double* hdist = nullptr;              // S*V straight-line distance to the SSSD target
+    uint64_t* dl[2] = {nullptr, nullptr};   // deferred lists (kCBlocks regions), by parity
+    uint64_t* pl[2] = {nullptr, nullptr};   // kPLists push lists, by parity
+    ExpEnt* exp = nullptr;                // rows of the round (kCBlocks regions)
+    SolveCounts* cnt = nullptr;
+    int* ovHost = nullptr;                // pinned copy of the overflow flag
+    size_t frontCap = 0;                  // capacity of a deferred / row list
+    size_t pCap = 0;                      // capacity of one push list
+    WonEnt* won = nullptr;                // kPLists won lists of wonCap entries
+    size_t wonCap = 0;
+    SlotAcc* acc = nullptr;               // kAccCopies * S
     bool anyParent = false;               // some active slot keeps parents
     SlotParam* sp = nullptr;              // S (device)
     SlotParam* spHost = nullptr;          // S (pinned)
     void* stage = nullptr;                // admission staging (device)
     void* stageHost = nullptr;            // admission staging (pinned)
     size_t stageCap = 0;
-    int cur = 0;
+    int cur = 0;                          // parity of the next round's input
     uint32_t round = 1;
-    size_t frontCap = 0;
     long long launches = 0;
     std::vector<int> gens;                // host copy of each slot's generation
 };
 
+// Device bytes one slot costs (for admission by memory).
+size_t solver_bytes_per_slot(int V);
 std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double delta, SolverState& s);
 void solver_destroy(SolverState& s);
 struct Admission {
@@ -74,8 +122,14 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
 // Seeds a slot (device work queued on stream 0).
 std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, const int* src,
                          const double* w, int ns, int target, bool wantParent);
-// One round for every active slot; returns the number of slots still running.
-std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running);
+// nRounds rounds for every active slot, queued back to back, then one host
+// check; returns the number of slots still running.  A slot that finishes
+// early idles through the remaining rounds of the batch.
+std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds, int& running);
+inline std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running)
+{
+    return solver_rounds(s, g, m, s.roundsPerSync, running);
+}
 // Results of a finished slot.
 std::string solver_fetch(SolverState& s, int slot, double* distOut /*V or null*/, double* distT,
                          int32_t* srcidOut, int32_t* parentOut);

@@ -1,5 +1,14 @@
 // Multi-query bucketed frontier engine (see solve.h) and the query entry
 // points of the C ABI (svg_sssd, svg_msad, svg_sssd_batch).
+//
+// A round is three passes: classify (frontier entries -> deferred list and
+// rows to relax), expand (relax the rows, push improved targets) and
+// bookkeep (per-slot bucket bounds).  Device-scope atomics on MI355X are
+// performed at the memory side, where atomics to one address serialise
+// (about 11 ns each), so no pass funnels its output through a single
+// counter: classify writes into static per-block regions sized by its input
+// range, expand buffers its pushes per wave in LDS and appends them to one of
+// kPLists lists, and per-slot statistics are spread over kAccCopies copies.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +25,11 @@ namespace svg {
 constexpr unsigned long long kInfBits = 0x7FEFFFFFFFFFFFFFull;   // DBL_MAX (GraphDistInfo default)
 constexpr int kSolveBlock = 256;
 constexpr int kSolveWaves = kSolveBlock / 64;
+constexpr int kExpandBlocks = 1024;     // 4 waves per SIMD resident (the pass's occupancy)
+constexpr int kPBuf = 512;              // per-wave push buffer (entries)
+constexpr int kChunk = 64;              // won-list chunk
+constexpr uint64_t kSentinel = ~0ull;
+constexpr int kSegs = kCBlocks + kPLists;   // classify input segments
 
 struct SolveArgs {
     int V, K;
@@ -28,16 +42,23 @@ struct SolveArgs {
     uint32_t* stamp;
     uint32_t* inq;
     int32_t* srcid;
-    const uint64_t* frontCur;
-    unsigned long long* usedCur;   // per entry: the distance it was expanded with
-    uint64_t* frontNext;
-    int* expList;                  // entries to expand this round
-    int* fcount;          // [cur, next] as selected by curIdx
-    int curIdx;
+    const double* __restrict__ hdist;   // S*V straight-line distance to the SSSD target
+    // lists of this round (c = input parity)
+    const uint64_t* dIn;                // deferred regions read by classify
+    uint64_t* dOut;                     // deferred regions written by classify
+    const uint64_t* pIn;                // kPLists push lists read by classify
+    uint64_t* pOut;                     // kPLists push lists written by expand
+    ExpEnt* exp;                        // rows to relax (classify regions)
+    SolveCounts* cnt;
+    int c;                              // input parity
+    size_t frontCap;                    // capacity of a deferred / row list
+    size_t pCap;                        // capacity of one push list
     SlotParam* sp;
+    SlotAcc* acc;                       // kAccCopies * S
+    int S;
     uint32_t round;
-    int frontCap;
-    int* overflow;        // set when a frontier write would exceed frontCap
+    WonEnt* won;
+    size_t wonCap;                      // per won list
 };
 
 __device__ __forceinline__ double heur(const double* __restrict__ pos, int v, int t)
@@ -49,43 +70,112 @@ __device__ __forceinline__ double heur(const double* __restrict__ pos, int v, in
     return sqrt(x * x + y * y + z * z);
 }
 
-__global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t* stamp, uint32_t* inq,
-                            int32_t* srcid, const int* slots, int nslots, int V)
-{
-    const size_t total = (size_t)nslots * (size_t)V;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / V);
-        const size_t o = (size_t)slots[k] * V + (i - (size_t)k * V);
-        dist[o] = kInfBits;
-        parent[o] = -1;
-        stamp[o] = 0;
-        inq[o] = 0;
-        srcid[o] = -1;
-    }
-}
-
 __device__ __forceinline__ uint64_t pack_entry(int gen, int slot, int v)
 {
     return ((uint64_t)(uint16_t)gen << 48) | ((uint64_t)(uint16_t)slot << 32) | (uint64_t)(uint32_t)v;
 }
 
-// One round, pass 1: one lane per frontier entry.  Deferred entries move to
-// the next frontier; entries to expand are listed (with the distance they
-// expand with) for pass 2.
+__device__ __forceinline__ unsigned long long lanes_below(int lane)
+{
+    return (1ull << lane) - 1ull;
+}
+
+// Exclusive prefix over n counts into pre[0..n] (pre[n] = total), by one
+// wave; the caller synchronises the block afterwards.
+__device__ void wave_prefix(const int* cnt, int n, int* pre, int lane)
+{
+    const int per = (n + 63) / 64;
+    const int lo = lane * per, hi = min(n, lo + per);
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += cnt[i];
+    int incl = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    int run = incl - s;
+    for (int i = lo; i < hi; ++i) { pre[i] = run; run += cnt[i]; }
+    if (lane == 63) pre[n] = incl;
+}
+
+// Index of the segment holding element g: the last s with pre[s] <= g.
+__device__ __forceinline__ int seg_of(const int* pre, int n, int g)
+{
+    int lo = 0, hi = n;                 // pre[lo] <= g < pre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Resets the admitted slots and, for an A* slot, tabulates the straight-line
+// distance of every vertex to its target (the heuristic of SVG.cu:402).
+__global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t* stamp, uint32_t* inq,
+                            int32_t* srcid, double* hdist, const double* __restrict__ pos,
+                            const int* slots, const int* targets, int nslots, int V)
+{
+    const size_t total = (size_t)nslots * (size_t)V;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / V);
+        const int v = (int)(i - (size_t)k * V);
+        const size_t o = (size_t)slots[k] * V + v;
+        dist[o] = kInfBits;
+        parent[o] = -1;
+        stamp[o] = 0;
+        inq[o] = 0;
+        srcid[o] = -1;
+        const int t = targets[k];
+        hdist[o] = t >= 0 ? heur(pos, v, t) : 0.0;
+    }
+}
+
+// Pass 1: the input (deferred regions of the previous classify, then the
+// push lists) is cut into kCBlocks equal ranges, one per block.  A block's
+// outputs fit in the same range of the output lists, so they are placed with
+// LDS counters only.  No other pass runs concurrently, so frontier flags and
+// distances are read and written plainly: launch boundaries order them
+// against the relaxations of pass 2.
 __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
 {
+    __shared__ int cnts[kSegs];
+    __shared__ int pre[kSegs + 1];
+    __shared__ int sD, sE;
     const int lane = threadIdx.x & 63;
-    const int n = min(a.fcount[a.curIdx], a.frontCap);
-    int* nextCount = &a.fcount[a.curIdx ^ 1];
-    int* expCount = &a.fcount[3];
-    const int stride = gridDim.x * blockDim.x;
-    for (int f0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); f0 < n; f0 += stride) {
-        const int f = f0 + lane;
+    const int wid = threadIdx.x >> 6;
+    SolveCounts* C = a.cnt;
+    // a push list that overflowed (flagged for the host) is read up to its capacity
+    for (int i = threadIdx.x; i < kSegs; i += blockDim.x)
+        cnts[i] = i < kCBlocks ? C->dcnt[a.c][i]
+                               : (int)min((size_t)C->pcnt[a.c][(i - kCBlocks) * kCntPad], a.pCap);
+    if (threadIdx.x == 0) { sD = 0; sE = 0; }
+    __syncthreads();
+    if (wid == 0) wave_prefix(cnts, kSegs, pre, lane);
+    __syncthreads();
+    int N = pre[kSegs];
+    if ((size_t)N + kCBlocks > a.frontCap) {       // cannot happen while the lists hold one
+        N = (int)(a.frontCap - kCBlocks);          // entry per (slot, vertex) and query
+        if (threadIdx.x == 0) atomicOr(&C->overflow, 1);
+    }
+    const int R = (N + kCBlocks - 1) / kCBlocks;
+    const int dStrideIn = C->dstride[a.c];
+    const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) { C->dstride[a.c ^ 1] = R; C->estride = R; }
+    const int g1 = min(N, (b + 1) * R);
+    const size_t outBase = (size_t)b * R;
+    SlotAcc* acc = a.acc + (size_t)(b % kAccCopies) * a.S;
+    for (int g0 = b * R + wid * 64; g0 < g1; g0 += kSolveBlock) {
+        const int g = g0 + lane;
+        uint64_t e = kSentinel;
+        if (g < g1) {
+            const int s = seg_of(pre, kSegs, g);
+            const int off = g - pre[s];
+            e = s < kCBlocks ? a.dIn[(size_t)s * dStrideIn + off] : a.pIn[(size_t)(s - kCBlocks) * a.pCap + off];
+        }
         int q = 0, act = 0;           // act: 0 none, 1 defer, 2 expand
-        bool hasDef = false;
-        unsigned long long defKey = ~0ull;
-        if (f < n) {
-            const uint64_t e = a.frontCur[f];
+        unsigned long long defKey = ~0ull, db = 0;
+        int rowLen = 0;
+        if (e != kSentinel) {
             q = (int)((e >> 32) & 0xFFFFu);
             const int v = (int)(e & 0xFFFFFFFFu);
             const SlotParam& sp = a.sp[q];
@@ -95,158 +185,283 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             } else if (sp.done) {
                 a.inq[o] = 0;
             } else {
-                // Take the entry first, then read the distance: an improvement
-                // that lands after the flag is cleared re-enters the frontier
-                // itself, and one that landed before is visible in the read
-                // (the acquire pairs with the improver's release exchange).
-                __hip_atomic_exchange(&a.inq[o], 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long db =
-                    __hip_atomic_load(&a.dist[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                db = a.dist[o];
                 const double dv = __longlong_as_double((long long)db);
                 double key = dv;
                 bool prune = false;
                 if (sp.kind == Q_SSSD) {
-                    const double ub = __longlong_as_double((long long)__hip_atomic_load(
-                        &a.dist[(size_t)q * a.V + sp.target], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                    key = dv + heur(a.pos, v, sp.target);
+                    const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+                    key = dv + a.hdist[o];
                     prune = key * (1.0 - 1e-12) > ub;
                 }
                 if (prune) {
                     // no path through v can beat the answer any more
+                    a.inq[o] = 0;
                 } else if (key >= sp.thr) {
-                    // deferred: re-arm the flag unless an improvement already
-                    // put v back into the next frontier
-                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-                        act = 1;
+                    // deferred: v stays flagged and moves to the next frontier
+                    act = 1;
                     defKey = (unsigned long long)__double_as_longlong(key);
-                    hasDef = true;
                 } else {
                     act = 2;
-                    a.usedCur[f] = db;
+                    a.inq[o] = 0;
+                    rowLen = a.tail[v];
                 }
             }
         }
         const unsigned long long dm = __ballot(act == 1);
         if (dm) {
-            int basePos = 0;
-            if (lane == 0) basePos = atomicAdd(nextCount, __popcll(dm));
-            basePos = __shfl(basePos, 0);
-            const int dpos = basePos + __popcll(dm & ((1ull << lane) - 1ull));
-            if (act == 1) {
-                if (dpos < a.frontCap) a.frontNext[dpos] = a.frontCur[f];
-                else atomicOr(a.overflow, 1);
-            }
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&sD, __popcll(dm));
+            base = __shfl(base, 0);
+            if (act == 1) a.dOut[outBase + base + __popcll(dm & lanes_below(lane))] = e;
         }
         const unsigned long long em = __ballot(act == 2);
         if (em) {
-            int basePos = 0;
-            if (lane == 0) basePos = atomicAdd(expCount, __popcll(em));
-            basePos = __shfl(basePos, 0);
-            if (act == 2) a.expList[basePos + __popcll(em & ((1ull << lane) - 1ull))] = f;
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&sE, __popcll(em));
+            base = __shfl(base, 0);
+            if (act == 2) {
+                ExpEnt x; x.entry = e; x.dist = db;
+                a.exp[outBase + base + __popcll(em & lanes_below(lane))] = x;
+            }
         }
-        // One update per slot present in this wave instead of one per entry:
-        // the per-slot counters are single addresses every lane would hit.
-        unsigned long long todo = __ballot(hasDef || act == 2);
+        // one update per slot present in this wave, into this block's copy
+        unsigned long long todo = __ballot(act != 0);
         while (todo) {
             const int leader = __ffsll((long long)todo) - 1;
             const int lq = __shfl(q, leader);
-            const bool mine = (hasDef || act == 2) && q == lq;
-            const unsigned long long grp = __ballot(mine);
-            todo &= ~grp;
-            unsigned long long k = mine && hasDef ? defKey : ~0ull;
+            const bool mine = act != 0 && q == lq;
+            todo &= ~__ballot(mine);
+            unsigned long long k = mine ? defKey : ~0ull;
+            long long arcs = mine ? rowLen : 0;
             for (int off = 32; off > 0; off >>= 1) {
                 const unsigned long long o = __shfl_xor(k, off);
                 k = o < k ? o : k;
+                arcs += __shfl_xor(arcs, off);
             }
-            const bool anyExp = __ballot(mine && act == 2) != 0ull;
+            const int nExp = __popcll(__ballot(mine && act == 2));
             if (lane == leader) {
-                if (k != ~0ull) atomicMin(&a.sp[lq].minDeferred, k);
-                if (anyExp) a.sp[lq].expanded = 1;
+                if (k != ~0ull) atomicMin(&acc[lq].minDeferred, k);
+                if (nExp) {
+                    atomicOr(&acc[lq].expanded, 1);
+                    atomicAdd(&acc[lq].settled, (unsigned long long)nExp);
+                    atomicAdd(&acc[lq].relaxed, (unsigned long long)arcs);
+                }
             }
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) { C->dcnt[a.c ^ 1][b] = sD; C->ecnt[b] = sE; }
 }
 
-// One round, pass 2: one wave per listed entry, 64 arcs of its row per step.
-// A plain read of the target's distance filters the relaxations first; it
-// can only be stale-high (distances only fall), so no improvement is missed.
+// Appends the wave's buffered pushes to push list `list` with one atomic.
+__device__ __forceinline__ void flush_pushes(const SolveArgs& a, uint64_t* buf, int& n, int list, int lane)
+{
+    if (n == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&a.cnt->pcnt[a.c ^ 1][list * kCntPad], n);
+    base = __shfl(base, 0);
+    uint64_t* dst = a.pOut + (size_t)list * a.pCap;
+    for (int j = lane; j < n; j += 64) {
+        if ((size_t)(base + j) < a.pCap) dst[base + j] = buf[j];
+        else atomicOr(&a.cnt->overflow, 1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    n = 0;
+}
+
+// Reserves room for the lanes of `mask` in the wave's won-list chunk.
+__device__ __forceinline__ int won_pos(const SolveArgs& a, WonEnt* list, int* counter, int& base, int& used,
+                                       unsigned long long mask, int lane)
+{
+    const int cnt = __popcll(mask);
+    if (used + cnt > kChunk) {
+        for (int j = used + lane; j < kChunk; j += 64)
+            if ((size_t)(base + j) < a.wonCap) list[base + j].slot = -1;
+        int nb = 0;
+        if (lane == 0) nb = atomicAdd(counter, kChunk);
+        base = __shfl(nb, 0);
+        used = 0;
+    }
+    const int pos = base + used + __popcll(mask & lanes_below(lane));
+    used += cnt;
+    return pos;
+}
+
+// Pass 2: one wave per row, rows dealt round-robin over all waves.  Up to
+// kSteps 64-arc steps are processed in phases so each phase's memory
+// operations are in flight together: arc loads, target-distance loads, the
+// A* bound's loads, the atomicMins, the frontier-flag exchanges.  The plain
+// read of a target's distance filters the relaxations first; it can only be
+// stale-high (distances only fall), so no improvement is missed.
+constexpr int kSteps = 8;
+
 __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
 {
+    __shared__ int pre[kCBlocks + 1];
+    __shared__ uint64_t pbuf[kSolveWaves][kPBuf];
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    const int n = a.fcount[3];
-    int* nextCount = &a.fcount[a.curIdx ^ 1];
-    for (int k = wave; k < n; k += nw) {
-        const int f = a.expList[k];
-        const uint64_t e = a.frontCur[f];
-        const int qq = (int)((e >> 32) & 0xFFFFu);
-        const int vv = (int)(e & 0xFFFFFFFFu);
-        const double dvv = __longlong_as_double((long long)a.usedCur[f]);
+    const int wid = threadIdx.x >> 6;
+    SolveCounts* C = a.cnt;
+    if (wid == 0) wave_prefix(C->ecnt, kCBlocks, pre, lane);
+    __syncthreads();
+    const int n = pre[kCBlocks];
+    const int stride = C->estride;
+    const int wave = blockIdx.x * kSolveWaves + wid;
+    const int nw = gridDim.x * kSolveWaves;
+    const int list = blockIdx.x % kPLists;
+    uint64_t* buf = pbuf[wid];
+    int nBuf = 0;
+    WonEnt* wonL = a.won + (size_t)list * a.wonCap;
+    int* wonC = &C->wcnt[list * kCntPad];
+    int wBase = 0, wUsed = kChunk;
+    for (int g = wave; g < n; g += nw) {
+        // the row's parameters are wave-uniform: keep them in scalar registers
+        const int sgi = seg_of(pre, kCBlocks, g);
+        const ExpEnt ex = a.exp[(size_t)sgi * stride + (g - pre[sgi])];
+        const uint64_t e = ex.entry;
+        const int qq = __builtin_amdgcn_readfirstlane((int)((e >> 32) & 0xFFFFu));
+        const int vv = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFFFFFu));
+        const double dvv = __longlong_as_double((long long)ex.dist);
         const SlotParam& sp = a.sp[qq];
-        const int kind = sp.kind;
-        const int t = sp.target;
-        const int wantParent = sp.wantParent;
-        const int gen = sp.gen;
+        const int kind = __builtin_amdgcn_readfirstlane(sp.kind);
+        const int t = __builtin_amdgcn_readfirstlane(sp.target);
+        const int wantParent = __builtin_amdgcn_readfirstlane(sp.wantParent);
+        const int gen = __builtin_amdgcn_readfirstlane(sp.gen);
         const size_t sb = (size_t)qq * a.V;
-        const int tl = a.tail[vv];
-        const int tlr = (tl + 63) & ~63;
+        const int tl = __builtin_amdgcn_readfirstlane(a.tail[vv]);
         const size_t row = (size_t)vv * a.K;
         double ub = DBL_MAX;
         if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
-        for (int i = lane; i < tlr; i += 64) {
-            bool push = false;
-            int x = -1;
-            if (i < tl) {
-                x = a.adj[row + i];
-                const double nd = dvv + a.w[row + i];
-                const unsigned long long nb = (unsigned long long)__double_as_longlong(nd);
-                bool ok = nb < a.dist[sb + x];
-                if (ok && kind == Q_SSSD) ok = !((nd + heur(a.pos, x, t) * (1.0 - 1e-12)) > ub);
-                if (ok) {
-                    const unsigned long long old = atomicMin(&a.dist[sb + x], nb);
-                    if (nb < old) {
-                        a.stamp[sb + x] = a.round;
-                        if (wantParent) a.parent[sb + x] = INT_MAX;
-                        push = __hip_atomic_exchange(&a.inq[sb + x], 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        for (int b = 0; b < tl; b += 64 * kSteps) {
+            int xs[kSteps];
+            unsigned long long nb[kSteps], val[kSteps];
+            // arcs: loads without branches (an index past the row re-reads
+            // its last arc and is masked), so all of them are in flight at once
+            double ws[kSteps];
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st) {
+                const int i = min(b + st * 64 + lane, tl - 1);
+                xs[st] = a.adj[row + i];
+                ws[st] = a.w[row + i];
+            }
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st) {
+                nb[st] = (unsigned long long)__double_as_longlong(dvv + ws[st]);
+                val[st] = a.dist[sb + xs[st]];
+            }
+            // candidates (with parents kept, an exact tie is one too)
+            unsigned cand = 0;
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st)
+                if (b + st * 64 + lane < tl && (wantParent ? nb[st] <= val[st] : nb[st] < val[st]))
+                    cand |= 1u << st;
+            // A*: drop a candidate that even the straight line to the target
+            // cannot bring under the current answer (SVG.cu:399-405)
+            if (kind == Q_SSSD && __ballot(cand != 0)) {
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st)
+                    if ((cand >> st) & 1u) val[st] = (unsigned long long)__double_as_longlong(a.hdist[sb + xs[st]]);
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st)
+                    if ((cand >> st) & 1u) {
+                        const double nd = __longlong_as_double((long long)nb[st]);
+                        const double h = __longlong_as_double((long long)val[st]);
+                        if ((nd + h * (1.0 - 1e-12)) > ub) cand &= ~(1u << st);
+                    }
+            }
+            if (!__ballot(cand != 0)) continue;
+            // atomicMins; val becomes the previous distance
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st)
+                if ((cand >> st) & 1u) val[st] = atomicMin(&a.dist[sb + xs[st]], nb[st]);
+            unsigned imp = 0, win = 0;
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st)
+                if ((cand >> st) & 1u) {
+                    if (nb[st] < val[st]) imp |= 1u << st;
+                    else if (nb[st] == val[st] && wantParent) win |= 1u << st;
+                }
+            if (wantParent) win |= imp;
+            // frontier flags of the improved targets
+            unsigned push = 0;
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st)
+                if ((imp >> st) & 1u) {
+                    const size_t o = sb + xs[st];
+                    if (wantParent) {
+                        a.stamp[o] = a.round;
+                        a.parent[o] = INT_MAX;
+                    }
+                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+                        push |= 1u << st;
+                }
+            // pushes go through the wave's LDS buffer
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st) {
+                const bool pu = (push >> st) & 1u;
+                const unsigned long long pm = __ballot(pu);
+                if (pm) {
+                    if (nBuf + 64 > kPBuf) flush_pushes(a, buf, nBuf, list, lane);
+                    if (pu) buf[nBuf + __popcll(pm & lanes_below(lane))] = pack_entry(gen, qq, xs[st]);
+                    nBuf += __popcll(pm);
+                    __builtin_amdgcn_wave_barrier();
+                }
+                const bool wi = (win >> st) & 1u;
+                const unsigned long long wm = __ballot(wi);
+                if (wm) {
+                    const int wpos = won_pos(a, wonL, wonC, wBase, wUsed, wm, lane);
+                    if (wi && (size_t)wpos < a.wonCap) {
+                        WonEnt we;
+                        we.slot = qq; we.x = xs[st]; we.u = vv; we.pad = 0; we.nb = nb[st];
+                        wonL[wpos] = we;
                     }
                 }
             }
-            const unsigned long long pm = __ballot(push);
-            if (pm) {
-                int basePos = 0;
-                if (lane == 0) basePos = atomicAdd(nextCount, __popcll(pm));
-                basePos = __shfl(basePos, 0);
-                const int ppos = basePos + __popcll(pm & ((1ull << lane) - 1ull));
-                if (push) {
-                    if (ppos < a.frontCap) a.frontNext[ppos] = pack_entry(gen, qq, x);
-                    else atomicOr(a.overflow, 1);
-                }
-            }
-        }
-        if (lane == 0) {
-            atomicAdd((unsigned long long*)&a.sp[qq].relaxed, (unsigned long long)tl);
-            atomicAdd((unsigned long long*)&a.sp[qq].settled, 1ull);
         }
     }
+    flush_pushes(a, buf, nBuf, list, lane);
+    for (int j = wUsed + lane; j < kChunk; j += 64)
+        if ((size_t)(wBase + j) < a.wonCap) wonL[wBase + j].slot = -1;
 }
 
 // Parents of the vertices improved this round: the smallest u whose
-// expansion value reproduces the new distance exactly.
+// expansion value reproduces the new distance exactly.  The round's
+// improvements and exact ties are the only candidates; if a won list
+// overflowed, the relaxed rows are scanned again instead.
 __global__ void __launch_bounds__(kSolveBlock) k_parent_fix(SolveArgs a)
 {
+    SolveCounts* C = a.cnt;
+    bool over = false;
+    for (int l = 0; l < kPLists; ++l) over |= (size_t)C->wcnt[l * kCntPad] > a.wonCap;
+    if (!over) {
+        for (int l = 0; l < kPLists; ++l) {
+            const int nWon = C->wcnt[l * kCntPad];
+            const WonEnt* L = a.won + (size_t)l * a.wonCap;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nWon; i += gridDim.x * blockDim.x) {
+                const WonEnt we = L[i];
+                if (we.slot < 0) continue;
+                const size_t o = (size_t)we.slot * a.V + we.x;
+                if (a.stamp[o] == a.round && a.dist[o] == we.nb) atomicMin(&a.parent[o], we.u);
+            }
+        }
+        return;
+    }
+    __shared__ int pre[kCBlocks + 1];
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    const int n = a.fcount[3];
-    for (int k = wave; k < n; k += nw) {
-        const int f = a.expList[k];
-        const uint64_t e = a.frontCur[f];
-        const int qq = (int)((e >> 32) & 0xFFFFu);
+    const int wid = threadIdx.x >> 6;
+    if (wid == 0) wave_prefix(C->ecnt, kCBlocks, pre, lane);
+    __syncthreads();
+    const int n = pre[kCBlocks];
+    const int stride = C->estride;
+    const int nw = gridDim.x * kSolveWaves;
+    for (int g = blockIdx.x * kSolveWaves + wid; g < n; g += nw) {
+        const int sgi = seg_of(pre, kCBlocks, g);
+        const ExpEnt ex = a.exp[(size_t)sgi * stride + (g - pre[sgi])];
+        const int qq = (int)((ex.entry >> 32) & 0xFFFFu);
         if (!a.sp[qq].wantParent) continue;
-        const int vv = (int)(e & 0xFFFFFFFFu);
-        const double dvv = __longlong_as_double((long long)a.usedCur[f]);
+        const int vv = (int)(ex.entry & 0xFFFFFFFFu);
+        const double dvv = __longlong_as_double((long long)ex.dist);
         const size_t sb = (size_t)qq * a.V;
         const int tl = a.tail[vv];
         const size_t row = (size_t)vv * a.K;
@@ -260,30 +475,48 @@ __global__ void __launch_bounds__(kSolveBlock) k_parent_fix(SolveArgs a)
     }
 }
 
-__global__ void k_bookkeep(SolveArgs a, int S)
+// Pass 3: per slot, fold the statistic copies; a slot with nothing expanded
+// this round moves its bucket bound to the smallest deferred key + delta, or
+// finishes.  Consumed push lists are emptied for the next round's pushes.
+__global__ void k_bookkeep(SolveArgs a)
 {
-    for (int q = threadIdx.x; q < S; q += blockDim.x) {
+    for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
+        unsigned long long md = ~0ull, settled = 0, relaxed = 0;
+        int expanded = 0;
+        for (int k = 0; k < kAccCopies; ++k) {
+            SlotAcc& x = a.acc[(size_t)k * a.S + q];
+            md = x.minDeferred < md ? x.minDeferred : md;
+            settled += x.settled;
+            relaxed += x.relaxed;
+            expanded |= x.expanded;
+            x.minDeferred = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0;
+        }
         SlotParam& sp = a.sp[q];
         if (sp.kind < 0 || sp.done) continue;
         sp.rounds++;
-        if (sp.expanded == 0) {
-            if (sp.minDeferred == 0xFFFFFFFFFFFFFFFFull) {
+        sp.settled += (long long)settled;
+        sp.relaxed += (long long)relaxed;
+        sp.expanded = expanded;
+        sp.minDeferred = md;
+        if (!expanded) {
+            if (md == ~0ull) {
                 sp.done = 1;
             } else {
-                const double md = __longlong_as_double((long long)sp.minDeferred);
+                const double mdd = __longlong_as_double((long long)md);
                 bool finished = false;
                 if (sp.kind == Q_SSSD) {
                     const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
-                    finished = md * (1.0 - 1e-12) > ub;
+                    finished = mdd * (1.0 - 1e-12) > ub;
                 }
                 if (finished) sp.done = 1;
-                else sp.thr = md + sp.delta;
+                else sp.thr = mdd + sp.delta;
             }
         }
-        sp.expanded = 0;
-        sp.minDeferred = 0xFFFFFFFFFFFFFFFFull;
     }
-    if (threadIdx.x == 0) { a.fcount[a.curIdx] = 0; a.fcount[3] = 0; }
+    if (threadIdx.x < kPLists) {
+        a.cnt->pcnt[a.c][threadIdx.x * kCntPad] = 0;
+        a.cnt->wcnt[threadIdx.x * kCntPad] = 0;
+    }
 }
 
 __global__ void k_resolve_src(const int32_t* parent, int32_t* srcid, const unsigned long long* dist, int V,
@@ -339,9 +572,15 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
     a.V = s.V; a.K = s.K;
     a.tail = g.tail; a.adj = g.adj; a.w = g.w; a.pos = m.pos;
     a.dist = s.dist; a.parent = s.parent; a.stamp = s.stamp; a.inq = s.inq; a.srcid = s.srcid;
-    a.frontCur = s.front[s.cur]; a.usedCur = s.used[s.cur]; a.frontNext = s.front[s.cur ^ 1];
-    a.fcount = s.fcount; a.curIdx = s.cur; a.sp = s.sp; a.round = s.round;
-    a.frontCap = (int)s.frontCap; a.overflow = s.fcount + 2; a.expList = s.expList;
+    a.hdist = s.hdist;
+    const int c = s.cur;
+    a.c = c;
+    a.dIn = s.dl[c]; a.dOut = s.dl[c ^ 1];
+    a.pIn = s.pl[c]; a.pOut = s.pl[c ^ 1];
+    a.exp = s.exp; a.cnt = s.cnt;
+    a.frontCap = s.frontCap; a.pCap = s.pCap;
+    a.sp = s.sp; a.acc = s.acc; a.S = s.S; a.round = s.round;
+    a.won = s.won; a.wonCap = s.wonCap;
     return a;
 }
 
@@ -351,26 +590,48 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
         if (_e != hipSuccess) return std::string(#x ": ") + hipGetErrorString(_e); \
     } while (0)
 
+size_t solver_bytes_per_slot(int V)
+{
+    // dist, parent, stamp, inq, srcid, hdist; deferred lists (2) and rows
+    // (16 B) at two entries per vertex; push lists (2 x kPLists) at one;
+    // won lists at two entries per vertex
+    return (size_t)V * (8 + 4 + 4 + 4 + 4 + 8 + 2 * 2 * 8 + 2 * 16 + 2 * kPLists * 8 + 2 * sizeof(WonEnt));
+}
+
 std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double delta, SolverState& s)
 {
     (void)m;
     if (slots < 1 || slots > 4096) return "slots must be in [1, 4096]";
     s.V = g.V; s.K = g.K; s.S = slots; s.delta = delta;
     const size_t sv = (size_t)slots * (size_t)g.V;
+    // live entries are at most one per (slot, vertex); entries of a slot's
+    // previous query may still be listed once more
     s.frontCap = 2 * sv + 65536;
+    s.pCap = sv + 65536;
     if (s.frontCap >= 0x7fffffffull) return "slots * V too large for the frontier";
     HTRY(hipMalloc(&s.dist, sizeof(unsigned long long) * sv));
     HTRY(hipMalloc(&s.parent, sizeof(int32_t) * sv));
     HTRY(hipMalloc(&s.stamp, sizeof(uint32_t) * sv));
     HTRY(hipMalloc(&s.inq, sizeof(uint32_t) * sv));
     HTRY(hipMalloc(&s.srcid, sizeof(int32_t) * sv));
+    HTRY(hipMalloc(&s.hdist, sizeof(double) * sv));
     for (int i = 0; i < 2; ++i) {
-        HTRY(hipMalloc(&s.front[i], sizeof(uint64_t) * s.frontCap));
-        HTRY(hipMalloc(&s.used[i], sizeof(unsigned long long) * s.frontCap));
+        HTRY(hipMalloc(&s.dl[i], sizeof(uint64_t) * s.frontCap));
+        HTRY(hipMalloc(&s.pl[i], sizeof(uint64_t) * s.pCap * kPLists));
     }
-    HTRY(hipMalloc(&s.expList, sizeof(int) * s.frontCap));
-    HTRY(hipMalloc(&s.fcount, sizeof(int) * 4));
-    HTRY(hipMemset(s.fcount, 0, sizeof(int) * 4));
+    HTRY(hipMalloc(&s.exp, sizeof(ExpEnt) * s.frontCap));
+    HTRY(hipMalloc(&s.cnt, sizeof(SolveCounts)));
+    HTRY(hipMemset(s.cnt, 0, sizeof(SolveCounts)));
+    HTRY(hipHostMalloc(&s.ovHost, sizeof(int), hipHostMallocDefault));
+    *s.ovHost = 0;
+    s.wonCap = std::min<size_t>(s.frontCap / kPLists + 65536, 0x7ffffff0ull);
+    HTRY(hipMalloc(&s.won, sizeof(WonEnt) * s.wonCap * kPLists));
+    HTRY(hipMalloc(&s.acc, sizeof(SlotAcc) * (size_t)kAccCopies * slots));
+    {
+        std::vector<SlotAcc> a0((size_t)kAccCopies * slots);
+        for (auto& x : a0) { x.minDeferred = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0; x.pad = 0; }
+        HTRY(hipMemcpy(s.acc, a0.data(), sizeof(SlotAcc) * a0.size(), hipMemcpyHostToDevice));
+    }
     HTRY(hipMalloc(&s.sp, sizeof(SlotParam) * (size_t)slots));
     HTRY(hipHostMalloc(&s.spHost, sizeof(SlotParam) * (size_t)slots, hipHostMallocDefault));
     std::vector<SlotParam> init((size_t)slots);
@@ -385,15 +646,17 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
 
 void solver_destroy(SolverState& s)
 {
-    hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid);
-    for (int i = 0; i < 2; ++i) { hipFree(s.front[i]); hipFree(s.used[i]); }
-    hipFree(s.fcount); hipFree(s.expList); hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
+    hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid); hipFree(s.hdist);
+    for (int i = 0; i < 2; ++i) { hipFree(s.dl[i]); hipFree(s.pl[i]); }
+    hipFree(s.exp); hipFree(s.cnt); hipHostFree(s.ovHost); hipFree(s.won); hipFree(s.acc);
+    hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
     s = SolverState();
 }
 
 // Seeds of the admitted slots, one block each, in order (SVG.cu:246-249,
 // 359-365): a repeated source keeps its last weight, as the reference's
-// array writes do.  The first bucket bound is the smallest seed key + delta.
+// array writes do.  Seeds join push list 0 of the next round's input.  The
+// first bucket bound is the smallest seed key + delta.
 __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* __restrict__ kinds,
                         const int* __restrict__ targets, const int* __restrict__ wantP,
                         const int* __restrict__ gens, const int* __restrict__ off,
@@ -403,7 +666,7 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
     const int k = blockIdx.x;
     const int slot = slots[k], kind = kinds[k], target = targets[k], gen = gens[k];
     const size_t base = (size_t)slot * a.V;
-    uint64_t* front = (uint64_t*)a.frontCur;
+    uint64_t* front = const_cast<uint64_t*>(a.pIn);
     double mk = DBL_MAX;
     for (int i = off[k]; i < off[k + 1]; ++i) {
         const int v = seeds[i];
@@ -413,9 +676,9 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
         a.srcid[base + v] = v;
         if (a.inq[base + v] == 0) {
             a.inq[base + v] = 1;
-            const int pos = atomicAdd(&a.fcount[a.curIdx], 1);
-            if (pos < a.frontCap) front[pos] = pack_entry(gen, slot, v);
-            else atomicOr(a.overflow, 1);
+            const int pos = atomicAdd(&a.cnt->pcnt[a.c][0], 1);
+            if ((size_t)pos < a.pCap) front[pos] = pack_entry(gen, slot, v);
+            else atomicOr(&a.cnt->overflow, 1);
         }
         const double key = kind == Q_SSSD ? w + heur(a.pos, v, target) : w;
         mk = key < mk ? key : mk;
@@ -478,7 +741,7 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const double* dW = (const double*)((const char*)s.stage + intBytes);
     const size_t total = (size_t)n * s.V;
     hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, 0,
-                       s.dist, s.parent, s.stamp, s.inq, s.srcid, dSlots, n, s.V);
+                       s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, n, s.V);
     DevGraph dummy;
     SolveArgs args = make_args(s, dummy, m);
     hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
@@ -503,24 +766,27 @@ std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, c
     return solver_admit_many(s, m, adm);
 }
 
-std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running)
+std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds, int& running)
 {
-    SolveArgs a = make_args(s, g, m);
     s.anyParent = false;
     for (int q = 0; q < s.S; ++q)
         if (s.spHost[q].kind >= 0 && !s.spHost[q].done && s.spHost[q].wantParent) s.anyParent = true;
-    hipLaunchKernelGGL(k_classify, dim3(512), dim3(kSolveBlock), 0, 0, a);
-    hipLaunchKernelGGL(k_expand, dim3(2048), dim3(kSolveBlock), 0, 0, a);
-    if (s.anyParent) hipLaunchKernelGGL(k_parent_fix, dim3(2048), dim3(kSolveBlock), 0, 0, a);
-    hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(256), 0, 0, a, s.S);
-    s.launches += s.anyParent ? 4 : 3;
-    hipError_t e = hipMemcpy(s.spHost, s.sp, sizeof(SlotParam) * (size_t)s.S, hipMemcpyDeviceToHost);
+    if (nRounds < 1) nRounds = 1;
+    for (int r = 0; r < nRounds; ++r) {
+        SolveArgs a = make_args(s, g, m);
+        hipLaunchKernelGGL(k_classify, dim3(kCBlocks), dim3(kSolveBlock), 0, 0, a);
+        hipLaunchKernelGGL(k_expand, dim3(kExpandBlocks), dim3(kSolveBlock), 0, 0, a);
+        if (s.anyParent) hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, 0, a);
+        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(256), 0, 0, a);
+        s.launches += s.anyParent ? 4 : 3;
+        s.cur ^= 1;
+        s.round++;
+    }
+    hipError_t e = hipMemcpyAsync(s.spHost, s.sp, sizeof(SlotParam) * (size_t)s.S, hipMemcpyDeviceToHost, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.ovHost, &s.cnt->overflow, sizeof(int), hipMemcpyDeviceToHost, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(0);
     if (e != hipSuccess) return std::string("solver round: ") + hipGetErrorString(e);
-    int ov = 0;
-    hipMemcpy(&ov, s.fcount + 2, sizeof(int), hipMemcpyDeviceToHost);
-    if (ov) return "solver frontier overflow";
-    s.cur ^= 1;
-    s.round++;
+    if (*s.ovHost) return "solver frontier overflow";
     running = 0;
     for (int q = 0; q < s.S; ++q)
         if (s.spHost[q].kind >= 0 && !s.spHost[q].done) ++running;
