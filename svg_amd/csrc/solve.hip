@@ -393,7 +393,10 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                         a.stamp[o] = a.round;
                         a.parent[o] = INT_MAX;
                     }
-                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+                    // relaxed: nothing in this pass orders other data by the
+                    // flag (an acquire/release here would write back and
+                    // invalidate the L2 at every exchange)
+                    if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
                         push |= 1u << st;
                 }
             // pushes go through the wave's LDS buffer
