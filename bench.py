@@ -66,12 +66,12 @@ def make_requests(V, n, qps, seed=21, msad_frac=0.3):
     return arrival, reqs
 
 
-def serve_replay(g, V, n, qps, inflight, delta_scale=1.0):
+def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4):
     import ctypes as C
     from svg_amd import _lib as L
     arrival, reqs = make_requests(V, n, qps)
     lib = L.lib()
-    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale)
+    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync)
     srv = C.c_void_p()
     L.check(lib.svg_server_start(g.g, C.byref(cfg), C.byref(srv)))
     arr = (L.Request * n)()
@@ -89,12 +89,13 @@ def serve_replay(g, V, n, qps, inflight, delta_scale=1.0):
             n_msad += 1
             w = np.zeros(len(src), np.float64)
             dout = np.empty(V, np.float64)
-            keep += [src, w, dout]
+            sout = np.empty(V, np.int32)      # nearest-source ids, as SolveMSAD returns
+            keep += [src, w, dout, sout]
             r.src = src.ctypes.data_as(C.POINTER(C.c_int32))
             r.w = w.ctypes.data_as(C.POINTER(C.c_double))
             r.ns = len(src)
             r.dist_out = dout.ctypes.data_as(C.POINTER(C.c_double))
-            r.srcid_out = None
+            r.srcid_out = sout.ctypes.data_as(C.POINTER(C.c_int32))
     lat = np.zeros(n, np.float64)
     t0 = time.perf_counter()
     L.check(lib.svg_server_replay(srv, arr, np.ascontiguousarray(arrival), n, lat))
@@ -110,13 +111,18 @@ def serve_replay(g, V, n, qps, inflight, delta_scale=1.0):
         "mean_ms": float(lat[ok].mean()), "max_ms": float(lat[ok].max()),
         "msad_fraction": n_msad / n, "inflight_slots": inflight, "launches": int(st.launches),
         "max_concurrent": int(st.max_batch_seen), "delta_scale": delta_scale,
+        "rounds_per_sync": rounds_per_sync, "msad_returns": "distances and nearest-source ids",
         "latency_from": "scheduled arrival to host completion",
     }
+    kinds = np.array([r[0] for r in reqs])
     for k, name in ((0, "sssd"), (1, "msad")):
         nk = max(1, st.n_kind[k])
+        sel = ok & (kinds == k)
         out[name] = {"n": int(st.n_kind[k]), "rounds_per_query": st.rounds_kind[k] / nk,
                      "expanded_per_query": st.settled_kind[k] / nk,
-                     "arcs_per_query": st.relaxed_kind[k] / nk}
+                     "arcs_per_query": st.relaxed_kind[k] / nk,
+                     "p50_ms": float(np.percentile(lat[sel], 50)) if sel.any() else None,
+                     "p99_ms": float(np.percentile(lat[sel], 99)) if sel.any() else None}
     return out, (arrival, reqs, sssd_out)
 
 
@@ -150,6 +156,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=64)
     ap.add_argument("--delta-scale", default="1.0",
                     help="bucket width / mean arc; a comma list runs one replay per value")
+    ap.add_argument("--rounds-per-sync", default="4",
+                    help="frontier rounds between host checks; a comma list runs one replay per value")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (byte-counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -244,14 +252,15 @@ def main():
     if args.requests > 0:
         runs = []
         for ds in [float(x) for x in args.delta_scale.split(",")]:
-            sv, _ = serve_replay(g_keep, V, args.requests, args.qps, args.inflight, ds)
-            runs.append(sv)
-            log(f"serving (delta x{ds}): p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms "
-                f"qps {sv['achieved_qps']:.1f}; sssd {sv['sssd']}; msad {sv['msad']}")
+            for rps in [int(x) for x in args.rounds_per_sync.split(",")]:
+                sv, _ = serve_replay(g_keep, V, args.requests, args.qps, args.inflight, ds, rps)
+                runs.append(sv)
+                log(f"serving (delta x{ds}, {rps} rounds/sync): p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms "
+                    f"qps {sv['achieved_qps']:.1f}; sssd {sv['sssd']}; msad {sv['msad']}")
         res["serving"] = min(runs, key=lambda r: r["p99_ms"])
         if len(runs) > 1:
-            res["serving_sweep"] = [{k: r[k] for k in ("delta_scale", "p50_ms", "p99_ms", "achieved_qps")}
-                                    for r in runs]
+            res["serving_sweep"] = [{k: r[k] for k in ("delta_scale", "rounds_per_sync", "p50_ms", "p99_ms",
+                                                       "achieved_qps")} for r in runs]
 
     # ---- CPU baseline: the oracle on a bounded sample of the same build
     if not args.no_cpu:

@@ -128,6 +128,8 @@ typedef struct {
     long long mem_budget;   /* bytes of device memory for request state (0 = auto) */
     int max_batch;          /* requests merged into one frontier launch */
     double delta_scale;     /* bucket width = delta_scale x mean arc length (0 = 1) */
+    int rounds_per_sync;    /* frontier rounds queued between two checks for finished
+                               requests and new admissions (0 = 4) */
 } svg_server_cfg;
 
 typedef struct {

@@ -204,6 +204,7 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     const double delta = graph_mean_arc(g->d) * scale;
     std::string e = solver_create(g->d, g->mesh->d, slots, delta, S->st);
     if (!e.empty()) { solver_destroy(S->st); delete S; return svg_set_error(SVG_E_MEM, e); }
+    if (cfg && cfg->rounds_per_sync > 0) S->st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
     if (hipMalloc(&S->gSlots, sizeof(int) * 2 * (size_t)slots) != hipSuccess ||
         hipMalloc(&S->gOut, sizeof(double) * (size_t)slots) != hipSuccess ||
         hipHostMalloc(&S->pinned, sizeof(int) * 2 * (size_t)slots, hipHostMallocDefault) != hipSuccess) {

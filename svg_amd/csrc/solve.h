@@ -44,7 +44,8 @@ constexpr int kAccCopies = 16;    // copies of the per-slot statistics
 // Per-slot statistics of a round, spread over kAccCopies copies so the
 // atomics of one slot do not all meet at one address.
 struct SlotAcc {
-    unsigned long long minDeferred;
+    unsigned long long minDeferred;   // smallest key deferred by classify
+    unsigned long long minImp;        // smallest key a relaxation lowered a target to
     unsigned long long settled;
     unsigned long long relaxed;
     int expanded;
@@ -69,11 +70,15 @@ struct SolveCounts {
     int dcnt[2][kCBlocks];            // deferred entries per classify region
     int dstride[2];                   // region stride of the deferred lists
     int pcnt[2][kPLists * kCntPad];   // push-list sizes (padded)
-    int ecnt[kCBlocks];               // rows per classify region
-    int estride;
-    int wcnt[kPLists * kCntPad];      // won-list sizes (padded)
+    int ecnt[2][kCBlocks];            // rows per classify region
+    int estride[2];
+    int wcnt[2][kPLists * kCntPad];   // won-list sizes (padded)
     int overflow;
 };
+
+// Slots up to this count are bookkept inside the next round's classify pass
+// (one LDS table per block); more slots use a bookkeeping pass per round.
+constexpr int kLdsSlots = 256;
 
 struct SolverState {
     int V = 0, K = 0, S = 0;
@@ -87,14 +92,16 @@ struct SolverState {
     double* hdist = nullptr;              // S*V straight-line distance to the SSSD target
     uint64_t* dl[2] = {nullptr, nullptr};   // deferred lists (kCBlocks regions), by parity
     uint64_t* pl[2] = {nullptr, nullptr};   // kPLists push lists, by parity
-    ExpEnt* exp = nullptr;                // rows of the round (kCBlocks regions)
+    ExpEnt* exp[2] = {nullptr, nullptr};  // rows of a round (kCBlocks regions), by parity
     SolveCounts* cnt = nullptr;
     int* ovHost = nullptr;                // pinned copy of the overflow flag
     size_t frontCap = 0;                  // capacity of a deferred / row list
     size_t pCap = 0;                      // capacity of one push list
-    WonEnt* won = nullptr;                // kPLists won lists of wonCap entries
+    WonEnt* won[2] = {nullptr, nullptr};  // kPLists won lists of wonCap entries, by parity
     size_t wonCap = 0;
-    SlotAcc* acc = nullptr;               // kAccCopies * S
+    SlotAcc* acc[2] = {nullptr, nullptr}; // kAccCopies * S statistics of a round, by parity
+    double* thr = nullptr;                // 2 * S bucket bound of a round, by parity
+    int* doneP = nullptr;                 // 2 * S finished flag of a round, by parity
     bool anyParent = false;               // some active slot keeps parents
     SlotParam* sp = nullptr;              // S (device)
     SlotParam* spHost = nullptr;          // S (pinned)

@@ -48,16 +48,23 @@ struct SolveArgs {
     uint64_t* dOut;                     // deferred regions written by classify
     const uint64_t* pIn;                // kPLists push lists read by classify
     uint64_t* pOut;                     // kPLists push lists written by expand
-    ExpEnt* exp;                        // rows to relax (classify regions)
+    ExpEnt* exp;                        // rows to relax this round (classify regions)
+    const ExpEnt* expPrev;              // rows of the previous round
     SolveCounts* cnt;
-    int c;                              // input parity
+    int c;                              // parity of this round
+    int doBook;                         // classify bookkeeps the previous round (S <= kLdsSlots)
+    int doParent;                       // classify resolves the previous round's parents
     size_t frontCap;                    // capacity of a deferred / row list
     size_t pCap;                        // capacity of one push list
     SlotParam* sp;
-    SlotAcc* acc;                       // kAccCopies * S
+    SlotAcc* acc;                       // kAccCopies * S, this round's statistics
+    SlotAcc* accPrev;                   // the previous round's
+    double* thr;                        // 2 * S bucket bounds by parity
+    int* doneP;                         // 2 * S finished flags by parity
     int S;
     uint32_t round;
-    WonEnt* won;
+    WonEnt* won;                        // this round's won lists
+    const WonEnt* wonPrev;              // the previous round's
     size_t wonCap;                      // per won list
 };
 
@@ -130,6 +137,106 @@ __global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t*
     }
 }
 
+// Bookkeeping of one slot after a round with statistics `acc` (kAccCopies
+// copies): with nothing left below the bound the bound moves to the smallest
+// key + delta, or the slot finishes.  Keys of the next round's entries are
+// the deferred ones (at least minDeferred, or lower if relaxed since, which
+// minImp covers) and the relaxed targets (at least minImp), so when none is
+// below the bound the next round would expand nothing and the bound moves
+// now instead of a round later.  With `write` the slot's record is updated.
+__device__ void book_slot(const SolveArgs& a, int q, const SlotAcc* acc, double thrOld, int doneOld,
+                          double& thrNew, int& doneNew, bool write)
+{
+    unsigned long long md = ~0ull, mi = ~0ull, settled = 0, relaxed = 0;
+    int expanded = 0;
+    for (int k = 0; k < kAccCopies; ++k) {
+        const SlotAcc& x = acc[(size_t)k * a.S + q];
+        md = x.minDeferred < md ? x.minDeferred : md;
+        mi = x.minImp < mi ? x.minImp : mi;
+        settled += x.settled;
+        relaxed += x.relaxed;
+        expanded |= x.expanded;
+    }
+    thrNew = thrOld;
+    doneNew = doneOld;
+    SlotParam& sp = a.sp[q];
+    if (sp.kind < 0 || doneOld) return;
+    const unsigned long long nextMin = md < mi ? md : mi;
+    const double nm = __longlong_as_double((long long)nextMin);
+    if (nextMin == ~0ull) {
+        doneNew = 1;
+    } else if (!(nm < thrOld)) {
+        bool finished = false;
+        if (sp.kind == Q_SSSD) {
+            const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+            finished = nm * (1.0 - 1e-12) > ub;
+        }
+        if (finished) doneNew = 1;
+        else thrNew = nm + sp.delta;
+    }
+    if (write) {
+        sp.rounds++;
+        sp.settled += (long long)settled;
+        sp.relaxed += (long long)relaxed;
+        sp.expanded = expanded;
+        sp.minDeferred = md;
+        sp.thr = thrNew;
+        sp.done = doneNew;
+    }
+}
+
+// Parents of the vertices improved in round `rnd` (parity p): the smallest u
+// whose expansion value reproduces the new distance exactly.  The round's
+// improvements and exact ties are the only candidates; if a won list
+// overflowed, the round's rows are scanned again instead.  `pre` is LDS of
+// kCBlocks + 1 ints; every block of the grid calls this.
+__device__ void parent_part(const SolveArgs& a, int p, uint32_t rnd, const WonEnt* won, const ExpEnt* rows,
+                            int* pre)
+{
+    SolveCounts* C = a.cnt;
+    bool over = false;
+    for (int l = 0; l < kPLists; ++l) over |= (size_t)C->wcnt[p][l * kCntPad] > a.wonCap;
+    if (!over) {
+        for (int l = 0; l < kPLists; ++l) {
+            const int nWon = C->wcnt[p][l * kCntPad];
+            const WonEnt* L = won + (size_t)l * a.wonCap;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nWon; i += gridDim.x * blockDim.x) {
+                const WonEnt we = L[i];
+                if (we.slot < 0) continue;
+                const size_t o = (size_t)we.slot * a.V + we.x;
+                if (a.stamp[o] == rnd && a.dist[o] == we.nb) atomicMin(&a.parent[o], we.u);
+            }
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    if (wid == 0) wave_prefix(C->ecnt[p], kCBlocks, pre, lane);
+    __syncthreads();
+    const int n = pre[kCBlocks];
+    const int stride = C->estride[p];
+    const int nw = gridDim.x * kSolveWaves;
+    for (int g = blockIdx.x * kSolveWaves + wid; g < n; g += nw) {
+        const int sgi = seg_of(pre, kCBlocks, g);
+        const ExpEnt ex = rows[(size_t)sgi * stride + (g - pre[sgi])];
+        const int qq = (int)((ex.entry >> 32) & 0xFFFFu);
+        if (!a.sp[qq].wantParent) continue;
+        const int vv = (int)(ex.entry & 0xFFFFFFFFu);
+        const double dvv = __longlong_as_double((long long)ex.dist);
+        const size_t sb = (size_t)qq * a.V;
+        const int tl = a.tail[vv];
+        const size_t row = (size_t)vv * a.K;
+        for (int i = lane; i < tl; i += 64) {
+            const int x = a.adj[row + i];
+            if (a.stamp[sb + x] != rnd) continue;
+            const double nd = dvv + a.w[row + i];
+            if ((unsigned long long)__double_as_longlong(nd) == a.dist[sb + x])
+                atomicMin(&a.parent[sb + x], vv);
+        }
+    }
+    __syncthreads();
+}
+
 // Pass 1: the input (deferred regions of the previous classify, then the
 // push lists) is cut into kCBlocks equal ranges, one per block.  A block's
 // outputs fit in the same range of the output lists, so they are placed with
@@ -141,9 +248,23 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
     __shared__ int cnts[kSegs];
     __shared__ int pre[kSegs + 1];
     __shared__ int sD, sE;
+    __shared__ double lThr[kLdsSlots];
+    __shared__ int lDone[kLdsSlots];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     SolveCounts* C = a.cnt;
+    // The previous round's bookkeeping, computed identically by every block
+    // (block 0 records it); the previous round's parents.
+    if (a.doBook) {
+        for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
+            double t; int d;
+            book_slot(a, q, a.accPrev, a.thr[(a.c ^ 1) * a.S + q], a.doneP[(a.c ^ 1) * a.S + q], t, d,
+                      blockIdx.x == 0);
+            lThr[q] = t; lDone[q] = d;
+            if (blockIdx.x == 0) { a.thr[a.c * a.S + q] = t; a.doneP[a.c * a.S + q] = d; }
+        }
+    }
+    if (a.doParent) parent_part(a, a.c ^ 1, a.round - 1, a.wonPrev, a.expPrev, pre);
     // a push list that overflowed (flagged for the host) is read up to its capacity
     for (int i = threadIdx.x; i < kSegs; i += blockDim.x)
         cnts[i] = i < kCBlocks ? C->dcnt[a.c][i]
@@ -160,7 +281,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
     const int R = (N + kCBlocks - 1) / kCBlocks;
     const int dStrideIn = C->dstride[a.c];
     const int b = blockIdx.x;
-    if (b == 0 && threadIdx.x == 0) { C->dstride[a.c ^ 1] = R; C->estride = R; }
+    if (b == 0 && threadIdx.x == 0) { C->dstride[a.c ^ 1] = R; C->estride[a.c] = R; }
     const int g1 = min(N, (b + 1) * R);
     const size_t outBase = (size_t)b * R;
     SlotAcc* acc = a.acc + (size_t)(b % kAccCopies) * a.S;
@@ -180,9 +301,11 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             const int v = (int)(e & 0xFFFFFFFFu);
             const SlotParam& sp = a.sp[q];
             const size_t o = (size_t)q * a.V + v;
+            const double thr = a.doBook ? lThr[q] : a.thr[a.c * a.S + q];
+            const int done = a.doBook ? lDone[q] : a.doneP[a.c * a.S + q];
             if (sp.kind < 0 || (int)(e >> 48) != (sp.gen & 0xFFFF)) {
                 // entry of an earlier query in this slot: dropped
-            } else if (sp.done) {
+            } else if (done) {
                 a.inq[o] = 0;
             } else {
                 db = a.dist[o];
@@ -197,7 +320,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 if (prune) {
                     // no path through v can beat the answer any more
                     a.inq[o] = 0;
-                } else if (key >= sp.thr) {
+                } else if (key >= thr) {
                     // deferred: v stays flagged and moves to the next frontier
                     act = 1;
                     defKey = (unsigned long long)__double_as_longlong(key);
@@ -251,7 +374,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) { C->dcnt[a.c ^ 1][b] = sD; C->ecnt[b] = sE; }
+    if (threadIdx.x == 0) { C->dcnt[a.c ^ 1][b] = sD; C->ecnt[a.c][b] = sE; }
 }
 
 // Appends the wave's buffered pushes to push list `list` with one atomic.
@@ -303,17 +426,26 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     SolveCounts* C = a.cnt;
-    if (wid == 0) wave_prefix(C->ecnt, kCBlocks, pre, lane);
+    // state the classify pass has consumed is cleared for the next round
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAccCopies * a.S; i += gridDim.x * blockDim.x) {
+        SlotAcc& x = a.accPrev[i];
+        x.minDeferred = ~0ull; x.minImp = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kPLists) {
+        C->pcnt[a.c][threadIdx.x * kCntPad] = 0;
+        C->wcnt[a.c ^ 1][threadIdx.x * kCntPad] = 0;
+    }
+    if (wid == 0) wave_prefix(C->ecnt[a.c], kCBlocks, pre, lane);
     __syncthreads();
     const int n = pre[kCBlocks];
-    const int stride = C->estride;
+    const int stride = C->estride[a.c];
     const int wave = blockIdx.x * kSolveWaves + wid;
     const int nw = gridDim.x * kSolveWaves;
     const int list = blockIdx.x % kPLists;
     uint64_t* buf = pbuf[wid];
     int nBuf = 0;
     WonEnt* wonL = a.won + (size_t)list * a.wonCap;
-    int* wonC = &C->wcnt[list * kCntPad];
+    int* wonC = &C->wcnt[a.c][list * kCntPad];
     int wBase = 0, wUsed = kChunk;
     for (int g = wave; g < n; g += nw) {
         // the row's parameters are wave-uniform: keep them in scalar registers
@@ -333,6 +465,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
         const size_t row = (size_t)vv * a.K;
         double ub = DBL_MAX;
         if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
+        unsigned long long rowMin = ~0ull;        // smallest candidate key of the row
         for (int b = 0; b < tl; b += 64 * kSteps) {
             int xs[kSteps];
             unsigned long long nb[kSteps], val[kSteps];
@@ -357,7 +490,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                 if (b + st * 64 + lane < tl && (wantParent ? nb[st] <= val[st] : nb[st] < val[st]))
                     cand |= 1u << st;
             // A*: drop a candidate that even the straight line to the target
-            // cannot bring under the current answer (SVG.cu:399-405)
+            // cannot bring under the current answer (SVG.cu:399-405).  The
+            // smallest key among the candidates feeds the bucket look-ahead.
             if (kind == Q_SSSD && __ballot(cand != 0)) {
 #pragma unroll
                 for (int st = 0; st < kSteps; ++st)
@@ -367,8 +501,17 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                     if ((cand >> st) & 1u) {
                         const double nd = __longlong_as_double((long long)nb[st]);
                         const double h = __longlong_as_double((long long)val[st]);
-                        if ((nd + h * (1.0 - 1e-12)) > ub) cand &= ~(1u << st);
+                        if ((nd + h * (1.0 - 1e-12)) > ub) {
+                            cand &= ~(1u << st);
+                        } else {
+                            const unsigned long long kb = (unsigned long long)__double_as_longlong(nd + h);
+                            rowMin = kb < rowMin ? kb : rowMin;
+                        }
                     }
+            } else {
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st)
+                    if ((cand >> st) & 1u) rowMin = nb[st] < rowMin ? nb[st] : rowMin;
             }
             if (!__ballot(cand != 0)) continue;
             // atomicMins; val becomes the previous distance
@@ -422,103 +565,42 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                 }
             }
         }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(rowMin, off);
+            rowMin = o < rowMin ? o : rowMin;
+        }
+        if (lane == 0 && rowMin != ~0ull)
+            atomicMin(&a.acc[(size_t)(blockIdx.x % kAccCopies) * a.S + qq].minImp, rowMin);
     }
     flush_pushes(a, buf, nBuf, list, lane);
     for (int j = wUsed + lane; j < kChunk; j += 64)
         if ((size_t)(wBase + j) < a.wonCap) wonL[wBase + j].slot = -1;
 }
 
-// Parents of the vertices improved this round: the smallest u whose
-// expansion value reproduces the new distance exactly.  The round's
-// improvements and exact ties are the only candidates; if a won list
-// overflowed, the relaxed rows are scanned again instead.
+// Parents of this round's improvements, as a pass of its own: after the last
+// round of a batch, and every round when the slots are bookkept by a pass.
 __global__ void __launch_bounds__(kSolveBlock) k_parent_fix(SolveArgs a)
 {
-    SolveCounts* C = a.cnt;
-    bool over = false;
-    for (int l = 0; l < kPLists; ++l) over |= (size_t)C->wcnt[l * kCntPad] > a.wonCap;
-    if (!over) {
-        for (int l = 0; l < kPLists; ++l) {
-            const int nWon = C->wcnt[l * kCntPad];
-            const WonEnt* L = a.won + (size_t)l * a.wonCap;
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nWon; i += gridDim.x * blockDim.x) {
-                const WonEnt we = L[i];
-                if (we.slot < 0) continue;
-                const size_t o = (size_t)we.slot * a.V + we.x;
-                if (a.stamp[o] == a.round && a.dist[o] == we.nb) atomicMin(&a.parent[o], we.u);
-            }
-        }
-        return;
-    }
     __shared__ int pre[kCBlocks + 1];
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    if (wid == 0) wave_prefix(C->ecnt, kCBlocks, pre, lane);
-    __syncthreads();
-    const int n = pre[kCBlocks];
-    const int stride = C->estride;
-    const int nw = gridDim.x * kSolveWaves;
-    for (int g = blockIdx.x * kSolveWaves + wid; g < n; g += nw) {
-        const int sgi = seg_of(pre, kCBlocks, g);
-        const ExpEnt ex = a.exp[(size_t)sgi * stride + (g - pre[sgi])];
-        const int qq = (int)((ex.entry >> 32) & 0xFFFFu);
-        if (!a.sp[qq].wantParent) continue;
-        const int vv = (int)(ex.entry & 0xFFFFFFFFu);
-        const double dvv = __longlong_as_double((long long)ex.dist);
-        const size_t sb = (size_t)qq * a.V;
-        const int tl = a.tail[vv];
-        const size_t row = (size_t)vv * a.K;
-        for (int i = lane; i < tl; i += 64) {
-            const int x = a.adj[row + i];
-            if (a.stamp[sb + x] != a.round) continue;
-            const double nd = dvv + a.w[row + i];
-            if ((unsigned long long)__double_as_longlong(nd) == a.dist[sb + x])
-                atomicMin(&a.parent[sb + x], vv);
-        }
-    }
+    parent_part(a, a.c, a.round, a.won, a.exp, pre);
 }
 
-// Pass 3: per slot, fold the statistic copies; a slot with nothing expanded
-// this round moves its bucket bound to the smallest deferred key + delta, or
-// finishes.  Consumed push lists are emptied for the next round's pushes.
+// Bookkeeping of this round as a pass of its own (one block): after the last
+// round of a batch, so the host sees current states, and every round when
+// there are more slots than the classify pass bookkeeps in LDS.  It records
+// the next round's bounds and clears the consumed statistics.
 __global__ void k_bookkeep(SolveArgs a)
 {
     for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
-        unsigned long long md = ~0ull, settled = 0, relaxed = 0;
-        int expanded = 0;
-        for (int k = 0; k < kAccCopies; ++k) {
-            SlotAcc& x = a.acc[(size_t)k * a.S + q];
-            md = x.minDeferred < md ? x.minDeferred : md;
-            settled += x.settled;
-            relaxed += x.relaxed;
-            expanded |= x.expanded;
-            x.minDeferred = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0;
-        }
-        SlotParam& sp = a.sp[q];
-        if (sp.kind < 0 || sp.done) continue;
-        sp.rounds++;
-        sp.settled += (long long)settled;
-        sp.relaxed += (long long)relaxed;
-        sp.expanded = expanded;
-        sp.minDeferred = md;
-        if (!expanded) {
-            if (md == ~0ull) {
-                sp.done = 1;
-            } else {
-                const double mdd = __longlong_as_double((long long)md);
-                bool finished = false;
-                if (sp.kind == Q_SSSD) {
-                    const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
-                    finished = mdd * (1.0 - 1e-12) > ub;
-                }
-                if (finished) sp.done = 1;
-                else sp.thr = mdd + sp.delta;
-            }
-        }
+        double t; int d;
+        book_slot(a, q, a.acc, a.thr[a.c * a.S + q], a.doneP[a.c * a.S + q], t, d, true);
+        a.thr[(a.c ^ 1) * a.S + q] = t;
+        a.doneP[(a.c ^ 1) * a.S + q] = d;
     }
-    if (threadIdx.x < kPLists) {
-        a.cnt->pcnt[a.c][threadIdx.x * kCntPad] = 0;
-        a.cnt->wcnt[threadIdx.x * kCntPad] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAccCopies * a.S; i += blockDim.x) {
+        SlotAcc& x = a.acc[i];
+        x.minDeferred = ~0ull; x.minImp = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0;
     }
 }
 
@@ -580,10 +662,12 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
     a.c = c;
     a.dIn = s.dl[c]; a.dOut = s.dl[c ^ 1];
     a.pIn = s.pl[c]; a.pOut = s.pl[c ^ 1];
-    a.exp = s.exp; a.cnt = s.cnt;
+    a.exp = s.exp[c]; a.expPrev = s.exp[c ^ 1]; a.cnt = s.cnt;
     a.frontCap = s.frontCap; a.pCap = s.pCap;
-    a.sp = s.sp; a.acc = s.acc; a.S = s.S; a.round = s.round;
-    a.won = s.won; a.wonCap = s.wonCap;
+    a.sp = s.sp; a.acc = s.acc[c]; a.accPrev = s.acc[c ^ 1]; a.S = s.S; a.round = s.round;
+    a.thr = s.thr; a.doneP = s.doneP;
+    a.won = s.won[c]; a.wonPrev = s.won[c ^ 1]; a.wonCap = s.wonCap;
+    a.doBook = 0; a.doParent = 0;
     return a;
 }
 
@@ -595,10 +679,11 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
 
 size_t solver_bytes_per_slot(int V)
 {
-    // dist, parent, stamp, inq, srcid, hdist; deferred lists (2) and rows
-    // (16 B) at two entries per vertex; push lists (2 x kPLists) at one;
-    // won lists at two entries per vertex
-    return (size_t)V * (8 + 4 + 4 + 4 + 4 + 8 + 2 * 2 * 8 + 2 * 16 + 2 * kPLists * 8 + 2 * sizeof(WonEnt));
+    // dist, parent, stamp, inq, srcid, hdist; deferred lists (2) and row
+    // lists (2, 16 B) at two entries per vertex; push lists (2 x kPLists) at
+    // one; won lists (2 x kPLists) at a quarter
+    return (size_t)V * (8 + 4 + 4 + 4 + 4 + 8 + 2 * 2 * 8 + 2 * 2 * 16 + 2 * kPLists * 8 +
+                        2 * kPLists * sizeof(WonEnt) / 4);
 }
 
 std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double delta, SolverState& s)
@@ -622,18 +707,27 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
         HTRY(hipMalloc(&s.dl[i], sizeof(uint64_t) * s.frontCap));
         HTRY(hipMalloc(&s.pl[i], sizeof(uint64_t) * s.pCap * kPLists));
     }
-    HTRY(hipMalloc(&s.exp, sizeof(ExpEnt) * s.frontCap));
+    for (int i = 0; i < 2; ++i) HTRY(hipMalloc(&s.exp[i], sizeof(ExpEnt) * s.frontCap));
     HTRY(hipMalloc(&s.cnt, sizeof(SolveCounts)));
     HTRY(hipMemset(s.cnt, 0, sizeof(SolveCounts)));
     HTRY(hipHostMalloc(&s.ovHost, sizeof(int), hipHostMallocDefault));
     *s.ovHost = 0;
-    s.wonCap = std::min<size_t>(s.frontCap / kPLists + 65536, 0x7ffffff0ull);
-    HTRY(hipMalloc(&s.won, sizeof(WonEnt) * s.wonCap * kPLists));
-    HTRY(hipMalloc(&s.acc, sizeof(SlotAcc) * (size_t)kAccCopies * slots));
+    s.wonCap = std::min<size_t>(sv / 4 + 65536, 0x7ffffff0ull);
+    for (int i = 0; i < 2; ++i) HTRY(hipMalloc(&s.won[i], sizeof(WonEnt) * s.wonCap * kPLists));
     {
         std::vector<SlotAcc> a0((size_t)kAccCopies * slots);
-        for (auto& x : a0) { x.minDeferred = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0; x.pad = 0; }
-        HTRY(hipMemcpy(s.acc, a0.data(), sizeof(SlotAcc) * a0.size(), hipMemcpyHostToDevice));
+        for (auto& x : a0) { x.minDeferred = ~0ull; x.minImp = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0; x.pad = 0; }
+        for (int i = 0; i < 2; ++i) {
+            HTRY(hipMalloc(&s.acc[i], sizeof(SlotAcc) * a0.size()));
+            HTRY(hipMemcpy(s.acc[i], a0.data(), sizeof(SlotAcc) * a0.size(), hipMemcpyHostToDevice));
+        }
+    }
+    HTRY(hipMalloc(&s.thr, sizeof(double) * 2 * (size_t)slots));
+    HTRY(hipMemset(s.thr, 0, sizeof(double) * 2 * (size_t)slots));
+    HTRY(hipMalloc(&s.doneP, sizeof(int) * 2 * (size_t)slots));
+    {
+        std::vector<int> d1((size_t)2 * slots, 1);
+        HTRY(hipMemcpy(s.doneP, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice));
     }
     HTRY(hipMalloc(&s.sp, sizeof(SlotParam) * (size_t)slots));
     HTRY(hipHostMalloc(&s.spHost, sizeof(SlotParam) * (size_t)slots, hipHostMallocDefault));
@@ -651,7 +745,8 @@ void solver_destroy(SolverState& s)
 {
     hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid); hipFree(s.hdist);
     for (int i = 0; i < 2; ++i) { hipFree(s.dl[i]); hipFree(s.pl[i]); }
-    hipFree(s.exp); hipFree(s.cnt); hipHostFree(s.ovHost); hipFree(s.won); hipFree(s.acc);
+    for (int i = 0; i < 2; ++i) { hipFree(s.exp[i]); hipFree(s.won[i]); hipFree(s.acc[i]); }
+    hipFree(s.cnt); hipHostFree(s.ovHost); hipFree(s.thr); hipFree(s.doneP);
     hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
     s = SolverState();
 }
@@ -691,6 +786,8 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
     p.thr = mk + delta; p.delta = delta; p.minDeferred = ~0ull;
     p.expanded = 0; p.rounds = 0; p.gen = gen; p.pad = 0; p.relaxed = 0; p.settled = 0;
     a.sp[slot] = p;
+    a.thr[a.c * a.S + slot] = p.thr;
+    a.doneP[a.c * a.S + slot] = 0;
 }
 
 std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vector<Admission>& adm)
@@ -775,13 +872,22 @@ std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, i
     for (int q = 0; q < s.S; ++q)
         if (s.spHost[q].kind >= 0 && !s.spHost[q].done && s.spHost[q].wantParent) s.anyParent = true;
     if (nRounds < 1) nRounds = 1;
+    // With few slots, a round is two launches: classify (which also does the
+    // previous round's bookkeeping and parents) and expand; the batch ends
+    // with the last round's parents and bookkeeping as passes of their own.
+    const bool inClassify = s.S <= kLdsSlots;
     for (int r = 0; r < nRounds; ++r) {
         SolveArgs a = make_args(s, g, m);
+        a.doBook = inClassify && r > 0;
+        a.doParent = a.doBook && s.anyParent;
         hipLaunchKernelGGL(k_classify, dim3(kCBlocks), dim3(kSolveBlock), 0, 0, a);
         hipLaunchKernelGGL(k_expand, dim3(kExpandBlocks), dim3(kSolveBlock), 0, 0, a);
-        if (s.anyParent) hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, 0, a);
-        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(256), 0, 0, a);
-        s.launches += s.anyParent ? 4 : 3;
+        s.launches += 2;
+        if (!inClassify || r == nRounds - 1) {
+            if (s.anyParent) { hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, 0, a); s.launches++; }
+            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, 0, a);
+            s.launches++;
+        }
         s.cur ^= 1;
         s.round++;
     }
