@@ -39,12 +39,6 @@ struct Ticket {
     int state = 0;           // 0 queued, 1 running, 2 done, 3 failed
 };
 
-__global__ void k_gather_targets(const unsigned long long* dist, const int* slots, const int* targets, int n,
-                                 int V, double* out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = __longlong_as_double((long long)dist[(size_t)slots[i] * V + targets[i]]);
-}
 
 }  // namespace
 
@@ -69,30 +63,90 @@ struct svg_server_s {
     Clock::time_point firstSubmit, lastDone;
     bool anySubmit = false;
     long long nKind[2] = {0, 0}, roundsKind[2] = {0, 0}, settledKind[2] = {0, 0}, relaxedKind[2] = {0, 0};
-    // gather staging
-    int* gSlots = nullptr;
-    int* gTargets = nullptr;
-    double* gOut = nullptr;
-    void* pinned = nullptr;
+    // pipelining: two batches of rounds in flight, their slot records copied
+    // into alternating pinned buffers; finished MSAD results leave through a
+    // second stream into per-slot pinned buffers
+    SlotParam* hs[2] = {nullptr, nullptr};
+    hipEvent_t batchEv[2] = {nullptr, nullptr};
+    hipStream_t fetchStream = nullptr;
+    std::vector<hipEvent_t> fetchEv;
+    double* fDist = nullptr;        // slots * V (pinned)
+    int32_t* fSrc = nullptr;        // slots * V (pinned)
 };
 
+namespace {
+
+enum : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FETCHING = 2 };
+
+void fail_all(svg_server_s* S, const std::string& e)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->error = e;
+    for (int q = 0; q < S->slots; ++q)
+        if (S->slotTicket[q] != -1) { S->tickets[S->slotTicket[q]]->state = 3; S->slotTicket[q] = -1; }
+    for (long long id : S->queue) S->tickets[id]->state = 3;
+    S->queue.clear();
+    S->cvDone.notify_all();
+}
+
+void finish_ticket(svg_server_s* S, int q, const SlotParam& sp)
+{
+    Ticket& T = *S->tickets[S->slotTicket[q]];
+    const int kd = T.kind == Q_MSAD ? 1 : 0;
+    S->nKind[kd] += 1;
+    S->roundsKind[kd] += sp.rounds;
+    S->settledKind[kd] += sp.settled;
+    S->relaxedKind[kd] += sp.relaxed;
+    solver_release_slot(S->st, q);
+    const Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    T.done = now;
+    T.state = 2;
+    S->lat.push_back(std::chrono::duration<double, std::milli>(now - T.submit).count());
+    S->lastDone = now;
+    S->slotTicket[q] = -1;
+    S->cvDone.notify_all();
+}
+
+}  // namespace
+
+// The worker keeps the device busy: it queues batch k of rounds, then reads
+// the slot records of batch k-1 (complete by then) while batch k runs.  A
+// request admitted between two batches joins the next one; its record is
+// trusted from the first batch queued after its admission.
 static void worker_loop(svg_server_s* S)
 {
     svg_ensure_device();
     const DevMesh& m = S->g->mesh->d;
-    std::vector<int> active;
+    std::vector<int> state((size_t)S->slots, SLOT_FREE);
+    std::vector<long long> admitBatch((size_t)S->slots, 0);
+    std::vector<SlotParam> fetchRec((size_t)S->slots);
+    long long k = 0;                 // batches queued so far
+    long long nextRead = 0;          // batches before this one have been read
+    const size_t V = (size_t)S->st.V;
     for (;;) {
+        // results that have arrived in pinned memory
+        for (int q = 0; q < S->slots; ++q) {
+            if (state[q] != SLOT_FETCHING || hipEventQuery(S->fetchEv[q]) != hipSuccess) continue;
+            Ticket& T = *S->tickets[S->slotTicket[q]];
+            if (T.distOut) std::memcpy(T.distOut, S->fDist + q * V, sizeof(double) * V);
+            if (T.srcidOut) std::memcpy(T.srcidOut, S->fSrc + q * V, sizeof(int32_t) * V);
+            finish_ticket(S, q, fetchRec[q]);
+            state[q] = SLOT_FREE;
+        }
+        // admissions into free slots
         std::vector<Admission> adm;
-        std::vector<long long> admT;
+        int running = 0, fetching = 0;
+        for (int q = 0; q < S->slots; ++q) { running += state[q] == SLOT_RUNNING; fetching += state[q] == SLOT_FETCHING; }
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (active.empty() && S->queue.empty()) {
+            if (running == 0 && fetching == 0 && nextRead == k && S->queue.empty()) {
                 if (S->stopping) break;
                 S->cvWork.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
             for (int q = 0; q < S->slots && !S->queue.empty() && (int)adm.size() < S->maxBatch; ++q) {
-                if (S->slotTicket[q] != -1) continue;
+                if (state[q] != SLOT_FREE) continue;
                 const long long id = S->queue.front();
                 S->queue.pop_front();
                 Ticket& T = *S->tickets[id];
@@ -104,81 +158,72 @@ static void worker_loop(svg_server_s* S)
                 T.state = 1;
                 S->slotTicket[q] = id;
                 adm.push_back(std::move(a));
-                admT.push_back(id);
             }
         }
         if (!adm.empty()) {
             std::string e = solver_admit_many(S->st, m, adm);
-            if (!e.empty()) {
-                std::lock_guard<std::mutex> lk(S->mu);
-                S->error = e;
-                for (long long id : admT) S->tickets[id]->state = 3;
-                for (const Admission& a : adm) S->slotTicket[a.slot] = -1;
-                S->cvDone.notify_all();
-                continue;
+            if (!e.empty()) { fail_all(S, e); return; }
+            for (const Admission& a : adm) { state[a.slot] = SLOT_RUNNING; admitBatch[a.slot] = k; }
+            running += (int)adm.size();
+            S->maxBatchSeen = std::max(S->maxBatchSeen, running);
+        }
+        // queue the next batch while the previous one may still run
+        if (running > 0) {
+            std::string e = solver_enqueue_rounds(S->st, S->g->d, m, S->st.roundsPerSync);
+            if (e.empty()) e = solver_enqueue_status(S->st, S->hs[k & 1]);
+            if (e.empty() && hipEventRecord(S->batchEv[k & 1], 0) != hipSuccess) e = "event record failed";
+            if (!e.empty()) { fail_all(S, e); return; }
+            ++k;
+        }
+        // read the records of the batch before the one just queued (or of the
+        // last batch, when nothing new was queued)
+        const long long upto = running > 0 ? k - 2 : k - 1;
+        for (; nextRead <= upto; ++nextRead) {
+            const int pb = (int)(nextRead & 1);
+            if (hipEventSynchronize(S->batchEv[pb]) != hipSuccess) { fail_all(S, "batch wait failed"); return; }
+            const SlotParam* rec = S->hs[pb];
+            if (rec[S->slots].kind) { fail_all(S, "solver frontier overflow"); return; }
+            for (int q = 0; q < S->slots; ++q) {
+                if (state[q] != SLOT_RUNNING || admitBatch[q] > nextRead || !rec[q].done) continue;
+                Ticket& T = *S->tickets[S->slotTicket[q]];
+                if (T.kind == Q_SSSD) {
+                    if (T.distOut) *T.distOut = rec[q].result;
+                    finish_ticket(S, q, rec[q]);
+                    state[q] = SLOT_FREE;
+                } else {
+                    std::string e;
+                    if (hipStreamWaitEvent(S->fetchStream, S->batchEv[pb], 0) != hipSuccess) e = "stream wait failed";
+                    if (e.empty())
+                        e = solver_enqueue_fetch(S->st, q, S->fetchStream, T.distOut ? S->fDist + q * V : nullptr,
+                                                 T.srcidOut ? S->fSrc + q * V : nullptr, nullptr);
+                    if (e.empty() && hipEventRecord(S->fetchEv[q], S->fetchStream) != hipSuccess)
+                        e = "event record failed";
+                    if (!e.empty()) { fail_all(S, e); return; }
+                    fetchRec[q] = rec[q];
+                    state[q] = SLOT_FETCHING;
+                }
             }
-            for (const Admission& a : adm) active.push_back(a.slot);
-            S->maxBatchSeen = std::max(S->maxBatchSeen, (int)active.size());
         }
-        if (active.empty()) continue;
-        int running = 0;
-        std::string e = solver_round(S->st, S->g->d, m, running);
-        if (!e.empty()) {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->error = e;
-            for (int q : active) { S->tickets[S->slotTicket[q]]->state = 3; S->slotTicket[q] = -1; }
-            active.clear();
-            S->cvDone.notify_all();
-            continue;
+        // only results in flight: wait for the oldest rather than spin
+        if (running == 0 && nextRead == k && fetching > 0) {
+            for (int q = 0; q < S->slots; ++q)
+                if (state[q] == SLOT_FETCHING) { hipEventSynchronize(S->fetchEv[q]); break; }
         }
-        // retire finished slots: SSSD answers are gathered with one launch
-        std::vector<int> fin, finS, finT;
-        for (int q : active)
-            if (S->st.spHost[q].done) fin.push_back(q);
-        if (fin.empty()) continue;
-        for (int q : fin) {
-            Ticket& T = *S->tickets[S->slotTicket[q]];
-            if (T.kind == Q_SSSD) { finS.push_back(q); finT.push_back(T.t); }
-        }
-        std::vector<double> sssdOut(finS.size());
-        if (!finS.empty()) {
-            const int n = (int)finS.size();
-            std::memcpy(S->pinned, finS.data(), sizeof(int) * (size_t)n);
-            std::memcpy((int*)S->pinned + n, finT.data(), sizeof(int) * (size_t)n);
-            hipMemcpyAsync(S->gSlots, S->pinned, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice, 0);
-            hipLaunchKernelGGL(k_gather_targets, dim3((n + 63) / 64), dim3(64), 0, 0, S->st.dist, S->gSlots,
-                               S->gSlots + n, n, S->st.V, S->gOut);
-            hipMemcpy(sssdOut.data(), S->gOut, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
-        }
-        size_t k = 0;
-        for (int q : fin) {
-            const long long id = S->slotTicket[q];
-            Ticket& T = *S->tickets[id];
-            if (T.kind == Q_SSSD) {
-                if (T.distOut) *T.distOut = sssdOut[k];
-                ++k;
-            } else {
-                solver_fetch(S->st, q, T.distOut, nullptr, T.srcidOut, nullptr);
-            }
-            const SlotParam& sp = S->st.spHost[q];
-            const int kd = T.kind == Q_MSAD ? 1 : 0;
-            S->nKind[kd] += 1;
-            S->roundsKind[kd] += sp.rounds;
-            S->settledKind[kd] += sp.settled;
-            S->relaxedKind[kd] += sp.relaxed;
-            solver_release_slot(S->st, q);
-            const Clock::time_point now = Clock::now();
-            std::lock_guard<std::mutex> lk(S->mu);
-            T.done = now;
-            T.state = 2;
-            S->lat.push_back(std::chrono::duration<double, std::milli>(now - T.submit).count());
-            S->lastDone = now;
-            S->slotTicket[q] = -1;
-        }
-        active.erase(std::remove_if(active.begin(), active.end(),
-                                    [&](int q) { return S->slotTicket[q] == -1; }), active.end());
-        S->cvDone.notify_all();
     }
+}
+
+static void server_free(svg_server_s* S)
+{
+    solver_destroy(S->st);
+    for (int i = 0; i < 2; ++i) {
+        if (S->hs[i]) hipHostFree(S->hs[i]);
+        if (S->batchEv[i]) hipEventDestroy(S->batchEv[i]);
+    }
+    for (hipEvent_t e : S->fetchEv) if (e) hipEventDestroy(e);
+    if (S->fetchStream) hipStreamDestroy(S->fetchStream);
+    if (S->fDist) hipHostFree(S->fDist);
+    if (S->fSrc) hipHostFree(S->fSrc);
+    delete S;
 }
 
 extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_server_t* out)
@@ -205,10 +250,18 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     std::string e = solver_create(g->d, g->mesh->d, slots, delta, S->st);
     if (!e.empty()) { solver_destroy(S->st); delete S; return svg_set_error(SVG_E_MEM, e); }
     if (cfg && cfg->rounds_per_sync > 0) S->st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
-    if (hipMalloc(&S->gSlots, sizeof(int) * 2 * (size_t)slots) != hipSuccess ||
-        hipMalloc(&S->gOut, sizeof(double) * (size_t)slots) != hipSuccess ||
-        hipHostMalloc(&S->pinned, sizeof(int) * 2 * (size_t)slots, hipHostMallocDefault) != hipSuccess) {
-        solver_destroy(S->st); delete S;
+    bool ok = true;
+    for (int i = 0; i < 2 && ok; ++i) {
+        ok = hipHostMalloc(&S->hs[i], sizeof(SlotParam) * (size_t)(slots + 1), hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&S->batchEv[i], hipEventDisableTiming) == hipSuccess;
+    }
+    ok = ok && hipStreamCreateWithFlags(&S->fetchStream, hipStreamNonBlocking) == hipSuccess;
+    S->fetchEv.assign((size_t)slots, nullptr);
+    for (int q = 0; q < slots && ok; ++q) ok = hipEventCreateWithFlags(&S->fetchEv[q], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipHostMalloc(&S->fDist, sizeof(double) * (size_t)slots * g->d.V, hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->fSrc, sizeof(int32_t) * (size_t)slots * g->d.V, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        server_free(S);
         return svg_set_error(SVG_E_MEM, "server staging allocation failed");
     }
     S->slotTicket.assign((size_t)slots, -1);
@@ -305,9 +358,7 @@ extern "C" int svg_server_stop(svg_server_t S)
     S->cvWork.notify_all();
     if (S->worker.joinable()) S->worker.join();
     svg_ensure_device();
-    solver_destroy(S->st);
-    hipFree(S->gSlots); hipFree(S->gOut); hipHostFree(S->pinned);
-    delete S;
+    server_free(S);
     return SVG_OK;
 }
 

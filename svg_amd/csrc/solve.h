@@ -9,6 +9,8 @@
 // lower bound to the target (SVG.cu:402) cannot beat the current answer.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -34,6 +36,7 @@ struct SlotParam {              // device-resident, one per slot
     int pad;
     long long relaxed;          // arcs scanned
     long long settled;          // entries expanded
+    double result;              // SSSD: distance to the target at the last bookkeeping
 };
 
 constexpr int kCBlocks = 512;     // classify blocks = deferred-list regions
@@ -94,7 +97,7 @@ struct SolverState {
     uint64_t* pl[2] = {nullptr, nullptr};   // kPLists push lists, by parity
     ExpEnt* exp[2] = {nullptr, nullptr};  // rows of a round (kCBlocks regions), by parity
     SolveCounts* cnt = nullptr;
-    int* ovHost = nullptr;                // pinned copy of the overflow flag
+    int* ovDev = nullptr;                 // overflow flag: the record after the last slot's
     size_t frontCap = 0;                  // capacity of a deferred / row list
     size_t pCap = 0;                      // capacity of one push list
     WonEnt* won[2] = {nullptr, nullptr};  // kPLists won lists of wonCap entries, by parity
@@ -103,11 +106,13 @@ struct SolverState {
     double* thr = nullptr;                // 2 * S bucket bound of a round, by parity
     int* doneP = nullptr;                 // 2 * S finished flag of a round, by parity
     bool anyParent = false;               // some active slot keeps parents
-    SlotParam* sp = nullptr;              // S (device)
-    SlotParam* spHost = nullptr;          // S (pinned)
-    void* stage = nullptr;                // admission staging (device)
-    void* stageHost = nullptr;            // admission staging (pinned)
-    size_t stageCap = 0;
+    SlotParam* sp = nullptr;              // S + 1 (device; the last record carries ovDev)
+    SlotParam* spHost = nullptr;          // S + 1 (pinned)
+    void* stage[2] = {nullptr, nullptr};        // admission staging (device), alternating
+    void* stageHost[2] = {nullptr, nullptr};    // admission staging (pinned)
+    size_t stageCap[2] = {0, 0};
+    hipEvent_t stageEv[2] = {nullptr, nullptr}; // staging copy done
+    int stageIdx = 0;
     int cur = 0;                          // parity of the next round's input
     uint32_t round = 1;
     long long launches = 0;
@@ -129,9 +134,20 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
 // Seeds a slot (device work queued on stream 0).
 std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, const int* src,
                          const double* w, int ns, int target, bool wantParent);
+// Queues nRounds rounds for every active slot on stream 0, ending with the
+// bookkeeping pass, so the slot records are current when they complete.  A
+// slot that finishes early idles through the remaining rounds.
+std::string solver_enqueue_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds);
+// Queues a copy of the S + 1 slot records (the last one's `kind` is the
+// overflow flag) into pinned memory.
+std::string solver_enqueue_status(SolverState& s, SlotParam* hostDst);
+// Queues, on `stream`, the results of a finished slot into pinned buffers:
+// distances, nearest-source ids (resolved from the parent chains first),
+// parents.  The caller orders `stream` after the rounds that finished it.
+std::string solver_enqueue_fetch(SolverState& s, int slot, hipStream_t stream, double* distDst,
+                                 int32_t* srcidDst, int32_t* parentDst);
 // nRounds rounds for every active slot, queued back to back, then one host
-// check; returns the number of slots still running.  A slot that finishes
-// early idles through the remaining rounds of the batch.
+// check; returns the number of slots still running.
 std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds, int& running);
 inline std::string solver_round(SolverState& s, const DevGraph& g, const DevMesh& m, int& running)
 {

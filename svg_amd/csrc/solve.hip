@@ -66,6 +66,7 @@ struct SolveArgs {
     WonEnt* won;                        // this round's won lists
     const WonEnt* wonPrev;              // the previous round's
     size_t wonCap;                      // per won list
+    int* ovFlag;                        // set when a list would overflow
 };
 
 __device__ __forceinline__ double heur(const double* __restrict__ pos, int v, int t)
@@ -182,6 +183,7 @@ __device__ void book_slot(const SolveArgs& a, int q, const SlotAcc* acc, double 
         sp.minDeferred = md;
         sp.thr = thrNew;
         sp.done = doneNew;
+        if (sp.kind == Q_SSSD) sp.result = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
     }
 }
 
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
     int N = pre[kSegs];
     if ((size_t)N + kCBlocks > a.frontCap) {       // cannot happen while the lists hold one
         N = (int)(a.frontCap - kCBlocks);          // entry per (slot, vertex) and query
-        if (threadIdx.x == 0) atomicOr(&C->overflow, 1);
+        if (threadIdx.x == 0) atomicOr(a.ovFlag, 1);
     }
     const int R = (N + kCBlocks - 1) / kCBlocks;
     const int dStrideIn = C->dstride[a.c];
@@ -387,7 +389,7 @@ __device__ __forceinline__ void flush_pushes(const SolveArgs& a, uint64_t* buf, 
     uint64_t* dst = a.pOut + (size_t)list * a.pCap;
     for (int j = lane; j < n; j += 64) {
         if ((size_t)(base + j) < a.pCap) dst[base + j] = buf[j];
-        else atomicOr(&a.cnt->overflow, 1);
+        else atomicOr(a.ovFlag, 1);
     }
     __builtin_amdgcn_wave_barrier();
     n = 0;
@@ -668,6 +670,7 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
     a.thr = s.thr; a.doneP = s.doneP;
     a.won = s.won[c]; a.wonPrev = s.won[c ^ 1]; a.wonCap = s.wonCap;
     a.doBook = 0; a.doParent = 0;
+    a.ovFlag = s.ovDev;
     return a;
 }
 
@@ -710,8 +713,6 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
     for (int i = 0; i < 2; ++i) HTRY(hipMalloc(&s.exp[i], sizeof(ExpEnt) * s.frontCap));
     HTRY(hipMalloc(&s.cnt, sizeof(SolveCounts)));
     HTRY(hipMemset(s.cnt, 0, sizeof(SolveCounts)));
-    HTRY(hipHostMalloc(&s.ovHost, sizeof(int), hipHostMallocDefault));
-    *s.ovHost = 0;
     s.wonCap = std::min<size_t>(sv / 4 + 65536, 0x7ffffff0ull);
     for (int i = 0; i < 2; ++i) HTRY(hipMalloc(&s.won[i], sizeof(WonEnt) * s.wonCap * kPLists));
     {
@@ -729,12 +730,15 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
         std::vector<int> d1((size_t)2 * slots, 1);
         HTRY(hipMemcpy(s.doneP, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice));
     }
-    HTRY(hipMalloc(&s.sp, sizeof(SlotParam) * (size_t)slots));
-    HTRY(hipHostMalloc(&s.spHost, sizeof(SlotParam) * (size_t)slots, hipHostMallocDefault));
-    std::vector<SlotParam> init((size_t)slots);
+    HTRY(hipMalloc(&s.sp, sizeof(SlotParam) * (size_t)(slots + 1)));
+    HTRY(hipHostMalloc(&s.spHost, sizeof(SlotParam) * (size_t)(slots + 1), hipHostMallocDefault));
+    std::vector<SlotParam> init((size_t)slots + 1);
     for (auto& p : init) { std::memset(&p, 0, sizeof(p)); p.kind = -1; p.done = 1; p.minDeferred = ~0ull; }
-    HTRY(hipMemcpy(s.sp, init.data(), sizeof(SlotParam) * (size_t)slots, hipMemcpyHostToDevice));
-    std::memcpy(s.spHost, init.data(), sizeof(SlotParam) * (size_t)slots);
+    init[slots].kind = 0;                           // the overflow flag
+    HTRY(hipMemcpy(s.sp, init.data(), sizeof(SlotParam) * init.size(), hipMemcpyHostToDevice));
+    std::memcpy(s.spHost, init.data(), sizeof(SlotParam) * init.size());
+    s.ovDev = &s.sp[slots].kind;
+    for (int i = 0; i < 2; ++i) HTRY(hipEventCreateWithFlags(&s.stageEv[i], hipEventDisableTiming));
     s.cur = 0;
     s.round = 1;
     s.gens.assign((size_t)slots, 0);
@@ -746,8 +750,12 @@ void solver_destroy(SolverState& s)
     hipFree(s.dist); hipFree(s.parent); hipFree(s.stamp); hipFree(s.inq); hipFree(s.srcid); hipFree(s.hdist);
     for (int i = 0; i < 2; ++i) { hipFree(s.dl[i]); hipFree(s.pl[i]); }
     for (int i = 0; i < 2; ++i) { hipFree(s.exp[i]); hipFree(s.won[i]); hipFree(s.acc[i]); }
-    hipFree(s.cnt); hipHostFree(s.ovHost); hipFree(s.thr); hipFree(s.doneP);
-    hipFree(s.sp); hipHostFree(s.spHost); hipFree(s.stage); hipHostFree(s.stageHost);
+    hipFree(s.cnt); hipFree(s.thr); hipFree(s.doneP);
+    hipFree(s.sp); hipHostFree(s.spHost);
+    for (int i = 0; i < 2; ++i) {
+        hipFree(s.stage[i]); hipHostFree(s.stageHost[i]);
+        if (s.stageEv[i]) hipEventDestroy(s.stageEv[i]);
+    }
     s = SolverState();
 }
 
@@ -776,7 +784,7 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
             a.inq[base + v] = 1;
             const int pos = atomicAdd(&a.cnt->pcnt[a.c][0], 1);
             if ((size_t)pos < a.pCap) front[pos] = pack_entry(gen, slot, v);
-            else atomicOr(&a.cnt->overflow, 1);
+            else atomicOr(a.ovFlag, 1);
         }
         const double key = kind == Q_SSSD ? w + heur(a.pos, v, target) : w;
         mk = key < mk ? key : mk;
@@ -785,6 +793,7 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
     p.kind = kind; p.target = target; p.wantParent = wantP[k]; p.done = 0;
     p.thr = mk + delta; p.delta = delta; p.minDeferred = ~0ull;
     p.expanded = 0; p.rounds = 0; p.gen = gen; p.pad = 0; p.relaxed = 0; p.settled = 0;
+    p.result = DBL_MAX;
     a.sp[slot] = p;
     a.thr[a.c * a.S + slot] = p.thr;
     a.doneP[a.c * a.S + slot] = 0;
@@ -794,7 +803,10 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
 {
     const int n = (int)adm.size();
     if (n == 0) return std::string();
-    HTRY(hipStreamSynchronize(0));      // the pinned staging buffer is free again
+    // the staging pair used two admissions ago is free once its copy is done
+    const int si = s.stageIdx;
+    s.stageIdx ^= 1;
+    HTRY(hipEventSynchronize(s.stageEv[si]));
     std::vector<int> ints;          // slots, kinds, targets, wantP, gens, off[n+1], seeds
     ints.reserve((size_t)6 * n + 64);
     size_t nseeds = 0;
@@ -819,18 +831,20 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     for (const Admission& a : adm) ints.insert(ints.end(), a.src.begin(), a.src.end());
     const size_t intBytes = ((ints.size() * sizeof(int) + 15) / 16) * 16;
     const size_t bytes = intBytes + nseeds * sizeof(double);
-    if (bytes > s.stageCap) {
-        hipFree(s.stage);
-        s.stageCap = std::max(bytes, (size_t)1 << 20);
-        HTRY(hipMalloc(&s.stage, s.stageCap));
-        hipHostFree(s.stageHost);
-        HTRY(hipHostMalloc(&s.stageHost, s.stageCap, hipHostMallocDefault));
+    if (bytes > s.stageCap[si]) {
+        HTRY(hipStreamSynchronize(0));              // the old buffer may still be read
+        hipFree(s.stage[si]);
+        s.stageCap[si] = std::max(bytes, (size_t)1 << 20);
+        HTRY(hipMalloc(&s.stage[si], s.stageCap[si]));
+        hipHostFree(s.stageHost[si]);
+        HTRY(hipHostMalloc(&s.stageHost[si], s.stageCap[si], hipHostMallocDefault));
     }
-    std::memcpy(s.stageHost, ints.data(), ints.size() * sizeof(int));
-    double* wh = (double*)((char*)s.stageHost + intBytes);
+    std::memcpy(s.stageHost[si], ints.data(), ints.size() * sizeof(int));
+    double* wh = (double*)((char*)s.stageHost[si] + intBytes);
     for (const Admission& a : adm) for (double w : a.w) *wh++ = w;
-    HTRY(hipMemcpyAsync(s.stage, s.stageHost, bytes, hipMemcpyHostToDevice, 0));
-    const int* d = (const int*)s.stage;
+    HTRY(hipMemcpyAsync(s.stage[si], s.stageHost[si], bytes, hipMemcpyHostToDevice, 0));
+    HTRY(hipEventRecord(s.stageEv[si], 0));
+    const int* d = (const int*)s.stage[si];
     const int* dSlots = d;
     const int* dKinds = d + n;
     const int* dTargets = d + 2 * n;
@@ -838,7 +852,7 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const int* dGens = d + 4 * n;
     const int* dOff = d + 5 * n;
     const int* dSeeds = d + 6 * n + 1;
-    const double* dW = (const double*)((const char*)s.stage + intBytes);
+    const double* dW = (const double*)((const char*)s.stage[si] + intBytes);
     const size_t total = (size_t)n * s.V;
     hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, 0,
                        s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, n, s.V);
@@ -866,7 +880,7 @@ std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, c
     return solver_admit_many(s, m, adm);
 }
 
-std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds, int& running)
+std::string solver_enqueue_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds)
 {
     s.anyParent = false;
     for (int q = 0; q < s.S; ++q)
@@ -891,14 +905,43 @@ std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, i
         s.cur ^= 1;
         s.round++;
     }
-    hipError_t e = hipMemcpyAsync(s.spHost, s.sp, sizeof(SlotParam) * (size_t)s.S, hipMemcpyDeviceToHost, 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.ovHost, &s.cnt->overflow, sizeof(int), hipMemcpyDeviceToHost, 0);
-    if (e == hipSuccess) e = hipStreamSynchronize(0);
-    if (e != hipSuccess) return std::string("solver round: ") + hipGetErrorString(e);
-    if (*s.ovHost) return "solver frontier overflow";
+    HTRY(hipGetLastError());
+    return std::string();
+}
+
+std::string solver_enqueue_status(SolverState& s, SlotParam* hostDst)
+{
+    HTRY(hipMemcpyAsync(hostDst, s.sp, sizeof(SlotParam) * (size_t)(s.S + 1), hipMemcpyDeviceToHost, 0));
+    return std::string();
+}
+
+std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds, int& running)
+{
+    std::string e = solver_enqueue_rounds(s, g, m, nRounds);
+    if (e.empty()) e = solver_enqueue_status(s, s.spHost);
+    if (!e.empty()) return e;
+    const hipError_t he = hipStreamSynchronize(0);
+    if (he != hipSuccess) return std::string("solver round: ") + hipGetErrorString(he);
+    if (s.spHost[s.S].kind) return "solver frontier overflow";
     running = 0;
     for (int q = 0; q < s.S; ++q)
         if (s.spHost[q].kind >= 0 && !s.spHost[q].done) ++running;
+    return std::string();
+}
+
+std::string solver_enqueue_fetch(SolverState& s, int slot, hipStream_t stream, double* distDst,
+                                 int32_t* srcidDst, int32_t* parentDst)
+{
+    const size_t sb = (size_t)slot * s.V;
+    if (distDst)
+        HTRY(hipMemcpyAsync(distDst, s.dist + sb, sizeof(double) * (size_t)s.V, hipMemcpyDeviceToHost, stream));
+    if (srcidDst) {
+        hipLaunchKernelGGL(k_resolve_src, dim3((s.V + 255) / 256), dim3(256), 0, stream, s.parent, s.srcid, s.dist,
+                           s.V, slot);
+        HTRY(hipMemcpyAsync(srcidDst, s.srcid + sb, sizeof(int32_t) * (size_t)s.V, hipMemcpyDeviceToHost, stream));
+    }
+    if (parentDst)
+        HTRY(hipMemcpyAsync(parentDst, s.parent + sb, sizeof(int32_t) * (size_t)s.V, hipMemcpyDeviceToHost, stream));
     return std::string();
 }
 
