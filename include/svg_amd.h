@@ -130,6 +130,8 @@ typedef struct {
     double delta_scale;     /* bucket width = delta_scale x mean arc length (0 = 1) */
     int rounds_per_sync;    /* frontier rounds queued between two checks for finished
                                requests and new admissions (0 = 4) */
+    double msad_delta_scale; /* bucket width of MSAD requests / mean arc length
+                               (0 = delta_scale) */
 } svg_server_cfg;
 
 typedef struct {

@@ -250,6 +250,7 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     std::string e = solver_create(g->d, g->mesh->d, slots, delta, S->st);
     if (!e.empty()) { solver_destroy(S->st); delete S; return svg_set_error(SVG_E_MEM, e); }
     if (cfg && cfg->rounds_per_sync > 0) S->st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
+    if (cfg && cfg->msad_delta_scale > 0.0) S->st.deltaMsad = delta / scale * cfg->msad_delta_scale;
     bool ok = true;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipHostMalloc(&S->hs[i], sizeof(SlotParam) * (size_t)(slots + 1), hipHostMallocDefault) == hipSuccess &&

@@ -85,7 +85,8 @@ constexpr int kLdsSlots = 256;
 
 struct SolverState {
     int V = 0, K = 0, S = 0;
-    double delta = 0.0;
+    double delta = 0.0;                   // bucket width of SSSD queries
+    double deltaMsad = 0.0;               // bucket width of MSAD queries
     int roundsPerSync = 4;                // rounds queued between two host checks
     unsigned long long* dist = nullptr;   // S*V bits of double
     int32_t* parent = nullptr;            // S*V

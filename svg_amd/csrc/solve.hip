@@ -196,18 +196,24 @@ __device__ void parent_part(const SolveArgs& a, int p, uint32_t rnd, const WonEn
                             int* pre)
 {
     SolveCounts* C = a.cnt;
+    int cnt[kPLists];
     bool over = false;
-    for (int l = 0; l < kPLists; ++l) over |= (size_t)C->wcnt[p][l * kCntPad] > a.wonCap;
+    int total = 0;
+#pragma unroll
+    for (int l = 0; l < kPLists; ++l) {
+        cnt[l] = C->wcnt[p][l * kCntPad];
+        over |= (size_t)cnt[l] > a.wonCap;
+        total += cnt[l];
+    }
     if (!over) {
-        for (int l = 0; l < kPLists; ++l) {
-            const int nWon = C->wcnt[p][l * kCntPad];
-            const WonEnt* L = won + (size_t)l * a.wonCap;
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nWon; i += gridDim.x * blockDim.x) {
-                const WonEnt we = L[i];
-                if (we.slot < 0) continue;
-                const size_t o = (size_t)we.slot * a.V + we.x;
-                if (a.stamp[o] == rnd && a.dist[o] == we.nb) atomicMin(&a.parent[o], we.u);
-            }
+        // one pass over the concatenated lists
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+            int l = 0, off = i;
+            while (off >= cnt[l]) { off -= cnt[l]; ++l; }
+            const WonEnt we = won[(size_t)l * a.wonCap + off];
+            if (we.slot < 0) continue;
+            const size_t o = (size_t)we.slot * a.V + we.x;
+            if (a.stamp[o] == rnd && a.dist[o] == we.nb) atomicMin(&a.parent[o], we.u);
         }
         return;
     }
@@ -250,20 +256,31 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
     __shared__ int cnts[kSegs];
     __shared__ int pre[kSegs + 1];
     __shared__ int sD, sE;
-    __shared__ double lThr[kLdsSlots];
-    __shared__ int lDone[kLdsSlots];
+    // per-slot table (up to kLdsSlots slots): bound, state, generation and,
+    // for A*, the current answer (no distance changes during this pass)
+    __shared__ double lThr[kLdsSlots], lUb[kLdsSlots];
+    __shared__ int lDone[kLdsSlots], lKind[kLdsSlots], lGen[kLdsSlots];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     SolveCounts* C = a.cnt;
-    // The previous round's bookkeeping, computed identically by every block
-    // (block 0 records it); the previous round's parents.
-    if (a.doBook) {
+    const bool table = a.S <= kLdsSlots;
+    // With doBook, the previous round's bookkeeping, computed identically by
+    // every block (block 0 records it); then the previous round's parents.
+    if (table) {
         for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
+            const SlotParam& sp = a.sp[q];
             double t; int d;
-            book_slot(a, q, a.accPrev, a.thr[(a.c ^ 1) * a.S + q], a.doneP[(a.c ^ 1) * a.S + q], t, d,
-                      blockIdx.x == 0);
-            lThr[q] = t; lDone[q] = d;
-            if (blockIdx.x == 0) { a.thr[a.c * a.S + q] = t; a.doneP[a.c * a.S + q] = d; }
+            if (a.doBook) {
+                book_slot(a, q, a.accPrev, a.thr[(a.c ^ 1) * a.S + q], a.doneP[(a.c ^ 1) * a.S + q], t, d,
+                          blockIdx.x == 0);
+                if (blockIdx.x == 0) { a.thr[a.c * a.S + q] = t; a.doneP[a.c * a.S + q] = d; }
+            } else {
+                t = a.thr[a.c * a.S + q];
+                d = a.doneP[a.c * a.S + q];
+            }
+            const int kind = sp.kind;
+            lThr[q] = t; lDone[q] = d; lKind[q] = kind; lGen[q] = sp.gen;
+            lUb[q] = kind == Q_SSSD ? __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]) : 0.0;
         }
     }
     if (a.doParent) parent_part(a, a.c ^ 1, a.round - 1, a.wonPrev, a.expPrev, pre);
@@ -301,11 +318,17 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
         if (e != kSentinel) {
             q = (int)((e >> 32) & 0xFFFFu);
             const int v = (int)(e & 0xFFFFFFFFu);
-            const SlotParam& sp = a.sp[q];
             const size_t o = (size_t)q * a.V + v;
-            const double thr = a.doBook ? lThr[q] : a.thr[a.c * a.S + q];
-            const int done = a.doBook ? lDone[q] : a.doneP[a.c * a.S + q];
-            if (sp.kind < 0 || (int)(e >> 48) != (sp.gen & 0xFFFF)) {
+            double thr, ub = 0.0;
+            int done, kind, gen;
+            if (table) {
+                thr = lThr[q]; done = lDone[q]; kind = lKind[q]; gen = lGen[q]; ub = lUb[q];
+            } else {
+                const SlotParam& sp = a.sp[q];
+                thr = a.thr[a.c * a.S + q]; done = a.doneP[a.c * a.S + q]; kind = sp.kind; gen = sp.gen;
+                if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+            }
+            if (kind < 0 || (int)(e >> 48) != (gen & 0xFFFF)) {
                 // entry of an earlier query in this slot: dropped
             } else if (done) {
                 a.inq[o] = 0;
@@ -314,8 +337,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 const double dv = __longlong_as_double((long long)db);
                 double key = dv;
                 bool prune = false;
-                if (sp.kind == Q_SSSD) {
-                    const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+                if (kind == Q_SSSD) {
                     key = dv + a.hdist[o];
                     prune = key * (1.0 - 1e-12) > ub;
                 }
@@ -425,9 +447,16 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
 {
     __shared__ int pre[kCBlocks + 1];
     __shared__ uint64_t pbuf[kSolveWaves][kPBuf];
+    __shared__ int4 lSlot[kLdsSlots];           // kind, target, wantParent, gen
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     SolveCounts* C = a.cnt;
+    const bool table = a.S <= kLdsSlots;
+    if (table)
+        for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
+            const SlotParam& sp = a.sp[q];
+            lSlot[q] = make_int4(sp.kind, sp.target, sp.wantParent, sp.gen);
+        }
     // state the classify pass has consumed is cleared for the next round
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAccCopies * a.S; i += gridDim.x * blockDim.x) {
         SlotAcc& x = a.accPrev[i];
@@ -457,11 +486,17 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
         const int qq = __builtin_amdgcn_readfirstlane((int)((e >> 32) & 0xFFFFu));
         const int vv = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFFFFFu));
         const double dvv = __longlong_as_double((long long)ex.dist);
-        const SlotParam& sp = a.sp[qq];
-        const int kind = __builtin_amdgcn_readfirstlane(sp.kind);
-        const int t = __builtin_amdgcn_readfirstlane(sp.target);
-        const int wantParent = __builtin_amdgcn_readfirstlane(sp.wantParent);
-        const int gen = __builtin_amdgcn_readfirstlane(sp.gen);
+        int4 sl;
+        if (table) {
+            sl = lSlot[qq];
+        } else {
+            const SlotParam& sp = a.sp[qq];
+            sl = make_int4(sp.kind, sp.target, sp.wantParent, sp.gen);
+        }
+        const int kind = __builtin_amdgcn_readfirstlane(sl.x);
+        const int t = __builtin_amdgcn_readfirstlane(sl.y);
+        const int wantParent = __builtin_amdgcn_readfirstlane(sl.z);
+        const int gen = __builtin_amdgcn_readfirstlane(sl.w);
         const size_t sb = (size_t)qq * a.V;
         const int tl = __builtin_amdgcn_readfirstlane(a.tail[vv]);
         const size_t row = (size_t)vv * a.K;
@@ -693,7 +728,7 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
 {
     (void)m;
     if (slots < 1 || slots > 4096) return "slots must be in [1, 4096]";
-    s.V = g.V; s.K = g.K; s.S = slots; s.delta = delta;
+    s.V = g.V; s.K = g.K; s.S = slots; s.delta = delta; s.deltaMsad = delta;
     const size_t sv = (size_t)slots * (size_t)g.V;
     // live entries are at most one per (slot, vertex); entries of a slot's
     // previous query may still be listed once more
@@ -766,11 +801,13 @@ void solver_destroy(SolverState& s)
 __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* __restrict__ kinds,
                         const int* __restrict__ targets, const int* __restrict__ wantP,
                         const int* __restrict__ gens, const int* __restrict__ off,
-                        const int* __restrict__ seeds, const double* __restrict__ wts, double delta)
+                        const int* __restrict__ seeds, const double* __restrict__ wts, double deltaSssd,
+                        double deltaMsad)
 {
     if (threadIdx.x != 0) return;
     const int k = blockIdx.x;
     const int slot = slots[k], kind = kinds[k], target = targets[k], gen = gens[k];
+    const double delta = kind == Q_SSSD ? deltaSssd : deltaMsad;
     const size_t base = (size_t)slot * a.V;
     uint64_t* front = const_cast<uint64_t*>(a.pIn);
     double mk = DBL_MAX;
@@ -859,7 +896,7 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     DevGraph dummy;
     SolveArgs args = make_args(s, dummy, m);
     hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
-                       dSeeds, dW, s.delta);
+                       dSeeds, dW, s.delta, s.deltaMsad);
     s.launches += 2;
     for (const Admission& a : adm) {
         SlotParam& p = s.spHost[a.slot];
