@@ -40,6 +40,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, K):
+    """HBM bytes per launch of the propagation kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (both counters
+    scaled by a 1 GiB copy measured in the same process); None if absent."""
+    import re
+    names = {"bumpy_torus": f"torus72k_K{K}"}
+    if workload not in names:
+        return None
+    p = os.path.join(ROOT, "profiles", f"batched_prop_kernel_{names[workload]}.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch: (\d+)", open(p).read())
+    return int(m.group(1)) if m else None
+
+
 def prop_bytes(c):
     return sum(c[k] * b for k, b in BYTES_PROP.items())
 
@@ -230,7 +245,8 @@ def main():
         "counters": c,
         "roofline": {"bound": "hbm", "kernel": "batched_prop_kernel",
                      "achieved": prop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": prop_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": prop_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, K),
+                     "traffic_source": "profiles/ rocprofv3 FETCH_SIZE + WRITE_SIZE summary, copy-calibrated",
                      "algorithmic_bytes": pb, "kernel_seconds": st["prop_seconds"],
                      "extract_achieved_gbs": eb / st["extract_seconds"] / 1e9},
     }
