@@ -208,14 +208,16 @@ def main():
         g.ConstructSVG(mode=mode, **kn)
         return g
 
+    # the autotuner learns from the warm-up build(s) of this mesh: source
+    # order, table sizes, block shape (rows do not depend on it)
     for i in range(args.warmup):
-        g = build(S.SVG_MODE_BATCHED)
+        g = build(S.SVG_MODE_BATCHED, autotune=1)
         log(f"warmup {i}: build {g.stats()['seconds']:.4f} s")
         g.FreeSVGStructure()
     times, stats = [], []
     wall0 = time.perf_counter()
     for i in range(args.steps):
-        g = build(S.SVG_MODE_BATCHED)
+        g = build(S.SVG_MODE_BATCHED, autotune=1)
         st = g.stats()
         times.append(st["seconds"])
         stats.append(st)
@@ -241,7 +243,7 @@ def main():
         "build_s": build_s, "build_s_all": times, "wall_s_steps": wall,
         "batched": {k: st[k] for k in ("prop_seconds", "extract_seconds", "redo_seconds", "redone",
                                        "flagged_drop", "flagged_overflow", "flagged_error", "lanes",
-                                       "prop_block", "chunk", "vert_cap", "split_cap", "arcs")},
+                                       "prop_block", "chunk", "vert_cap", "split_cap", "tuned", "arcs")},
         "counters": c,
         "roofline": {"bound": "hbm", "kernel": "batched_prop_kernel",
                      "achieved": prop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

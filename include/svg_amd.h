@@ -69,7 +69,8 @@ typedef struct {
     int chunk;              /* batched: sources with resident tables (0 = auto) */
     int vert_cap;           /* batched: vertex-table slots per source (0 = auto) */
     int split_cap;          /* batched: split-table slots per source (0 = auto) */
-    int autotune;           /* batched: search lanes / block on a source sample */
+    int autotune;           /* batched: learn from each build of this mesh (per K and
+                               coefficients) the source order, table sizes and block shape */
 } svg_build_params;
 
 typedef struct {
@@ -84,6 +85,8 @@ typedef struct {
     int flagged_overflow;   /* sources whose reference tables would overflow */
     int flagged_error;
     int lanes, prop_block, chunk, vert_cap, split_cap;
+    int tuned;              /* 1: source order, table sizes and block shape came from the
+                               autotuner's record of earlier builds of this mesh */
     long long counters[16]; /* event counters (win push/pop, pseudo push/pop, propagate,
                                children, split/vert writes, fan half-edges, pseudo
                                neighbours, -, -, backtrace hops, reached, -, -) */

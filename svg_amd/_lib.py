@@ -35,7 +35,7 @@ class BuildStats(C.Structure):
                 ("flagged_drop", C.c_int), ("flagged_overflow", C.c_int),
                 ("flagged_error", C.c_int), ("lanes", C.c_int), ("prop_block", C.c_int),
                 ("chunk", C.c_int), ("vert_cap", C.c_int), ("split_cap", C.c_int),
-                ("counters", C.c_longlong * 16)]
+                ("tuned", C.c_int), ("counters", C.c_longlong * 16)]
 
 
 class ServerCfg(C.Structure):

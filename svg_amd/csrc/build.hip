@@ -851,11 +851,12 @@ struct BArgs {
     int* work;                  // work counter
     uint32_t* status;           // chunk
     int32_t* vcount;            // chunk
-    int32_t* cnt;               // chunk * 12 event counters
+    int32_t* cnt;               // chunk * kNumBatchedCounters event counters
     int hopCap;
+    const int32_t* order;       // sources in processing order (null: by id)
 };
 
-constexpr int kNumBatchedCounters = 12;
+constexpr int kNumBatchedCounters = 13;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
@@ -897,7 +898,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     for (;;) {
         const int slot = atomicAdd(a.work, 1);
         if (slot >= a.nsrc) break;
-        const int src = a.srcBase + slot;
+        const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
         policy_tables(p, a, slot, src);
         p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
         p.nrecIdx = -1;
@@ -909,7 +910,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
         int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
         c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
         c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
-        c[10] = 0; c[11] = p.vcount;
+        c[10] = 0; c[11] = p.vcount; c[12] = p.scount;
     }
 }
 
@@ -923,7 +924,7 @@ __global__ void __launch_bounds__(64) batched_extract_kernel(BArgs a)
     BatchedPolicy p;
     policy_mesh(p, a);
     for (int slot = blockIdx.x; slot < a.nsrc; slot += gridDim.x) {
-        const int src = a.srcBase + slot;
+        const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
         const uint32_t stt = a.status[slot];
         const int K = a.K;
         if (stt & (SF_CAPACITY | SF_REF_OVERFLOW)) {
@@ -1004,19 +1005,65 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     const int srefcap = (int)((unsigned)K * c.splitCoef);
     unsigned vcap = c.vertCap > 0 ? next_pow2((unsigned)c.vertCap) : next_pow2(2u * (unsigned)vrefcap + 2u);
     unsigned scap = c.splitCap > 0 ? next_pow2((unsigned)c.splitCap) : next_pow2(4u * (unsigned)K);
+    int block = c.propBlock > 0 ? c.propBlock : 64;
+    // Autotuner (per mesh, K and coefficients; results never depend on it):
+    // once a build has recorded every source's event count, the cheapest
+    // sources are the ones left to start late; when the default tables would
+    // not fit at once they are sized to the peak occupancy recorded (a source
+    // that still overflows is recomputed exactly on the plain path); the
+    // block shape is the fastest of 64/128/256 threads, each tried once on
+    // successive builds.
+    TuneRecord* tune = c.autotune ? c.tune : nullptr;
+    std::vector<int32_t> order;
+    int blockSlot = -1;
+    if (tune) {
+        static const int kBlocks[3] = {64, 128, 256};
+        if (c.propBlock <= 0) {
+            int best = -1;
+            for (int i = 0; i < 3 && blockSlot < 0; ++i) if (tune->blockMs[i] <= 0.0) blockSlot = i;
+            if (blockSlot < 0) {
+                for (int i = 0; i < 3; ++i) if (best < 0 || tune->blockMs[i] < tune->blockMs[best]) best = i;
+                blockSlot = best;
+            }
+            block = kBlocks[blockSlot];
+        }
+        const int nLanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
+        if ((int)tune->cost.size() == V && nLanes < V) {
+            // The first nLanes sources start together; the others start as
+            // lanes free up.  The cheapest V - nLanes go last, and each group
+            // keeps id order, so the lanes of a wave stay on nearby sources
+            // (their mesh data shares cache lines).
+            std::vector<int32_t> byCost((size_t)V);
+            for (int i = 0; i < V; ++i) byCost[i] = i;
+            std::nth_element(byCost.begin(), byCost.begin() + (V - nLanes), byCost.end(),
+                             [&](int x, int y) { return tune->cost[x] < tune->cost[y] || (tune->cost[x] == tune->cost[y] && x < y); });
+            std::vector<char> late((size_t)V, 0);
+            for (int i = 0; i < V - nLanes; ++i) late[byCost[i]] = 1;
+            order.reserve((size_t)V);
+            for (int i = 0; i < V; ++i) if (!late[i]) order.push_back(i);
+            for (int i = 0; i < V; ++i) if (late[i]) order.push_back(i);
+        }
+        if ((int)tune->cost.size() == V) st.tuned = 1;
+    }
     if (vcap < 64) vcap = 64;
     if (scap < 64) scap = 64;
     if (vcap > 8192) return "vertex table larger than the extraction sort buffer (8192)";
     const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
-    const int block = c.propBlock > 0 ? c.propBlock : 64;
     size_t freeB = 0, totB = 0;
     hipMemGetInfo(&freeB, &totB);
     const size_t perLane = sizeof(BHeapEnt) * kHeapSlots +
                            (sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
-    const size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
+    size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
     int chunk = c.chunk > 0 ? c.chunk : V;
+    const size_t budget = freeB > ((size_t)lanes * perLane + (4ull << 30)) ? freeB - (size_t)lanes * perLane - (4ull << 30) : 0;
+    if (st.tuned && (size_t)chunk * perSrc > budget && c.vertCap <= 0 && c.splitCap <= 0) {
+        // the default tables would not all fit at once: size them to the
+        // peak occupancy recorded (+3/4 load headroom) to keep one pass
+        vcap = std::min(8192u, std::max(64u, next_pow2((unsigned)((tune->maxVert + 1) * 16 / 9 + 8))));
+        scap = std::max(64u, next_pow2((unsigned)((tune->maxSplit + 1) * 16 / 9 + 8)));
+        perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
+    }
     {
-        const size_t budget = freeB > ((size_t)lanes * perLane + (4ull << 30)) ? freeB - (size_t)lanes * perLane - (4ull << 30) : 0;
         const size_t maxChunk = budget / perSrc;
         if ((size_t)chunk > maxChunk) chunk = (int)std::max<size_t>(maxChunk, 1);
     }
@@ -1042,6 +1089,16 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         hipMalloc(&a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)chunk) != hipSuccess) {
         err = "batched build: out of device memory";
     }
+    int32_t* dOrder = nullptr;
+    if (err.empty() && !order.empty()) {
+        if (hipMalloc(&dOrder, sizeof(int32_t) * (size_t)V) != hipSuccess ||
+            hipMemcpy(dOrder, order.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice) != hipSuccess)
+            err = "batched build: out of device memory";
+        a.order = dOrder;
+    }
+    std::vector<int32_t> newCost;
+    int newMaxV = 0, newMaxS = 0;
+    if (tune) newCost.assign((size_t)V, 0);
     st.allocSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::vector<int32_t> redo;
     st.status.assign((size_t)V, 0);
@@ -1081,8 +1138,9 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         hipMemcpy(hs.data(), a.status, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost);
         hipMemcpy(hc.data(), a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)n, hipMemcpyDeviceToHost);
         for (int i = 0; i < n; ++i) {
-            st.status[base + i] = hs[i];
-            if (hs[i] & (SF_CAPACITY | SF_REF_OVERFLOW)) redo.push_back(base + i);
+            const int src = order.empty() ? base + i : order[(size_t)base + i];
+            st.status[src] = hs[i];
+            if (hs[i] & (SF_CAPACITY | SF_REF_OVERFLOW)) redo.push_back(src);
             if (hs[i] & (SF_WIN_DROP | SF_PSEUDO_DROP)) st.flaggedDrop++;
             if (hs[i] & SF_REF_OVERFLOW) st.flaggedOverflow++;
             if (hs[i] & SF_ERROR) st.flaggedError++;
@@ -1093,8 +1151,22 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
             st.counters[CNT_SPLIT_WRITE] += cc[6]; st.counters[CNT_VERT_WRITE] += cc[7];
             st.counters[CNT_FAN_HALFEDGE] += cc[8]; st.counters[CNT_PSEUDO_NEIGH] += cc[9];
             st.counters[CNT_BT_HOPS] += cc[10]; st.counters[CNT_REACHED] += cc[11];
+            if (tune) {
+                newCost[src] = cc[1] + cc[3];
+                // a source that hit the capacity stopped early: its peak is unknown
+                const int grow = (hs[i] & SF_CAPACITY) ? 2 : 1;
+                newMaxV = std::max(newMaxV, cc[11] * grow);
+                newMaxS = std::max(newMaxS, cc[12] * grow);
+            }
         }
     }
+    if (tune && err.empty()) {
+        tune->cost.swap(newCost);
+        tune->maxVert = newMaxV;
+        tune->maxSplit = newMaxS;
+        if (blockSlot >= 0) tune->blockMs[blockSlot] = st.propSeconds * 1e3;
+    }
+    hipFree(dOrder);
     hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
     hipFree(a.vtab); hipFree(a.rlist); hipFree(a.stab); hipFree(a.wheap); hipFree(a.wrec);
     hipFree(a.wfree); hipFree(a.pheap); hipFree(a.work); hipFree(a.status); hipFree(a.vcount); hipFree(a.cnt);

@@ -173,8 +173,23 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     std::string e = alloc_graph(m->h.nv, p->K, g->d);
     if (!e.empty()) { free_graph(g->d); delete g; return svg_set_error(SVG_E_MEM, e); }
     BuildStats st;
-    if (c.mode == SVG_MODE_PLAIN) e = build_plain(m->h, m->d, c, g->d, st, nullptr, 0);
-    else e = build_batched(m->h, m->d, c, g->d, st);
+    if (c.mode == SVG_MODE_BATCHED && c.autotune) {
+        std::lock_guard<std::mutex> lk(m->mu);
+        TuneRecord* r = nullptr;
+        for (TuneRecord& t : m->tunes)
+            if (t.K == c.K && t.splitCoef == c.splitCoef && t.vertCoef == c.vertCoef) r = &t;
+        if (!r) {
+            m->tunes.emplace_back();
+            r = &m->tunes.back();
+            r->K = c.K; r->splitCoef = c.splitCoef; r->vertCoef = c.vertCoef;
+        }
+        c.tune = r;
+        e = build_batched(m->h, m->d, c, g->d, st);
+    } else if (c.mode == SVG_MODE_PLAIN) {
+        e = build_plain(m->h, m->d, c, g->d, st, nullptr, 0);
+    } else {
+        e = build_batched(m->h, m->d, c, g->d, st);
+    }
     if (!e.empty()) { free_graph(g->d); delete g; return svg_set_error(SVG_E_HIP, e); }
     std::vector<int32_t> tails((size_t)m->h.nv);
     hipMemcpy(tails.data(), g->d.tail, sizeof(int32_t) * tails.size(), hipMemcpyDeviceToHost);
@@ -197,7 +212,7 @@ extern "C" int svg_build_stats_get(svg_graph_t g, svg_build_stats* st)
     st->arcs = s.arcs; st->redone = s.redone;
     st->flagged_drop = s.flaggedDrop; st->flagged_overflow = s.flaggedOverflow; st->flagged_error = s.flaggedError;
     st->lanes = s.lanes; st->prop_block = s.propBlock; st->chunk = s.chunk;
-    st->vert_cap = s.vertCap; st->split_cap = s.splitCap;
+    st->vert_cap = s.vertCap; st->split_cap = s.splitCap; st->tuned = s.tuned;
     for (size_t i = 0; i < s.counters.size() && i < 16; ++i) st->counters[i] = s.counters[i];
     return SVG_OK;
 }

@@ -10,6 +10,17 @@
 
 namespace svg {
 
+// What the batched autotuner has learned about one (mesh, K, coefficients):
+// every batched build records each source's event count and its tables'
+// peak occupancy, and the propagation time of the block shape it used.
+struct TuneRecord {
+    int K = 0;
+    unsigned splitCoef = 0, vertCoef = 0;
+    std::vector<int32_t> cost;     // per source: window + pseudo pops
+    int maxVert = 0, maxSplit = 0; // largest table occupancy seen
+    double blockMs[3] = {0.0, 0.0, 0.0};   // propagation time with 64 / 128 / 256 threads
+};
+
 struct BuildConfig {
     int K = 0;
     unsigned splitCoef = 10;     // SVG::SetParameters (kernel.cu:422)
@@ -25,6 +36,7 @@ struct BuildConfig {
     int vertCap = 0;             // per-source vertex table slots (power of two)
     int splitCap = 0;            // per-source split table slots (power of two)
     int autotune = 0;
+    TuneRecord* tune = nullptr;  // autotune: what earlier builds of this mesh recorded
 };
 
 struct BuildStats {
@@ -37,6 +49,7 @@ struct BuildStats {
     int redone = 0;              // sources recomputed on the plain path
     int flaggedDrop = 0, flaggedOverflow = 0, flaggedError = 0;
     int lanes = 0, propBlock = 0, chunk = 0, vertCap = 0, splitCap = 0;
+    int tuned = 0;               // 1: order, capacities and block came from the autotuner
     std::vector<long long> counters;   // summed event counters (CNT_NUM)
     std::vector<uint32_t> status;      // per-vertex status bits (SF_*)
 };

@@ -15,6 +15,7 @@ struct svg_mesh_s {
     svg::HostMesh h;
     svg::DevMesh d;
     std::mutex mu;
+    std::vector<svg::TuneRecord> tunes;   // batched-build autotuner state, per (K, coefficients)
 };
 
 struct svg_graph_s {

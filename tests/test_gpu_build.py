@@ -67,25 +67,41 @@ def test_build_sphere30k_sample():
     assert not bad, bad[:10]
 
 
-def test_build_torus72k_sample_and_properties():
-    """bumpy_torus (V=72,000, K=500): oracle parity on 64 sources, and
-    size-independent properties on every row."""
+def _compare_all(tails, h, orows, otails, K):
+    """Vectorised bitwise comparison of every row (full-graph parity)."""
+    assert np.array_equal(tails, otails), f"{int(np.sum(tails != otails))} row lengths differ"
+    mask = np.arange(K)[None, :] < tails[:, None]
+    for gk, ok in (("adj", "adj"), ("dist", "geodDist"), ("src_edge", "nextToSrcEdge"),
+                   ("dst_edge", "nextToDstEdge")):
+        a, b = h[gk][mask], orows[ok][mask]
+        assert np.array_equal(a, b), f"{gk}: {int(np.sum(a != b))} entries differ"
+    for gk, ok, ek in (("src_x", "nextToSrcX", "nextToSrcEdge"), ("dst_x", "nextToDstX", "nextToDstEdge")):
+        m = mask & (orows[ek] != -1)
+        a, b = h[gk][m], orows[ok][m]
+        assert np.array_equal(a, b), f"{gk}: {int(np.sum(a != b))} entries differ"
+
+
+def test_build_torus72k_full_and_autotuned():
+    """bumpy_torus (V=72,000, K=500), the headline build: every row bit-equal
+    to the oracle, drop flags equal, size-independent properties; then two
+    more builds with the autotuner (the second one runs with the learned
+    source order, table sizes and block shape) give the same rows."""
     xyz, tri = mg.bumpy_torus()
     K = 500
-    srcs = np.linspace(0, len(xyz) - 1, 64).astype(int)
-    print("case torus72k", flush=True)
-    _, srcs, orows, otails, ocnt = oracle_build(xyz, tri, K, sources=srcs)
+    V = len(xyz)
+    print("case torus72k oracle (all sources)", flush=True)
+    om = O.Mesh(xyz, tri)
+    orows, otails, ocnt, secs = O.svg_build(om, K)
+    print(f"oracle build {secs:.1f} s", flush=True)
     m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
     tails, h = gpu_rows(g)
-    bad = compare_rows((tails, h), orows, otails, srcs, K)
-    assert not bad, bad[:10]
-    # Half of these sources fill the 1023-entry window heap and drop pushes,
-    # as the reference does (PriorityQueue.cuh:53); the flag must agree.
+    _compare_all(tails, h, orows, otails, K)
+    # Many sources fill the 1023-entry window heap and drop pushes, as the
+    # reference does (PriorityQueue.cuh:53); the flag must agree.
     st = g.status()
     odrop = ocnt[:, O.COUNTERS.index("win_drop")] > 0
-    assert np.array_equal((st[srcs] & 1) > 0, odrop)
+    assert np.array_equal((st & 1) > 0, odrop)
     assert odrop.any()
-    V = len(xyz)
     assert np.all(tails <= K) and np.all(tails > 0)
     mask = np.arange(K)[None, :] < tails[:, None]
     adj = np.where(mask, h["adj"], np.iinfo(np.int32).max)
@@ -94,3 +110,14 @@ def test_build_torus72k_sample_and_properties():
     eu = np.linalg.norm(xyz[np.where(mask, h["adj"], 0)] - xyz[:, None, :], axis=2)
     assert np.all(h["dist"][mask] >= eu[mask] * (1 - 1e-12)), "arc >= chord"
     assert not np.any(st & (4 | 8 | 16)), "table overflow, capacity or errors"
+    g.FreeSVGStructure()
+    for it in range(2):
+        print(f"case torus72k autotuned build {it}", flush=True)
+        g2 = S.SVG()
+        g2.AssignMesh(m)
+        g2.SetParameters(K, 10, 2)
+        g2.ConstructSVG(mode=S.SVG_MODE_BATCHED, autotune=1)
+        assert g2.stats()["tuned"] == it          # the first one only records
+        t2, h2 = gpu_rows(g2)
+        _compare_all(t2, h2, orows, otails, K)
+        g2.FreeSVGStructure()
