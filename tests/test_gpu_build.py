@@ -121,3 +121,22 @@ def test_build_torus72k_full_and_autotuned():
         t2, h2 = gpu_rows(g2)
         _compare_all(t2, h2, orows, otails, K)
         g2.FreeSVGStructure()
+
+
+@pytest.mark.parametrize("K", [100, 500])
+def test_build_terrain750k_sample(K):
+    """aniso_terrain (866x866 grid, V=749,956, skinny/obtuse triangles, open
+    boundary) at K=100 and K=500 (the reference runs out of memory at 500):
+    every source built in one pass, oracle parity on 48 spread sources,
+    every row sorted and within K."""
+    xyz, tri = mg.aniso_terrain()
+    print(f"case terrain750k K={K}", flush=True)
+    m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
+    srcs = np.linspace(0, len(xyz) - 1, 48).astype(int)
+    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
+    tails, h = gpu_rows(g)
+    bad = compare_rows((tails, h), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+    assert np.all(tails <= K) and np.all(tails > 0)
+    st = g.status()
+    assert not np.any(st & 16), "error flags"
