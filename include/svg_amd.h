@@ -135,6 +135,8 @@ typedef struct {
                                requests and new admissions (0 = 4) */
     double msad_delta_scale; /* bucket width of MSAD requests / mean arc length
                                (0 = delta_scale) */
+    int streams;            /* slot groups, each with its own HIP stream and worker, whose
+                               rounds run concurrently (0 = 4) */
 } svg_server_cfg;
 
 typedef struct {

@@ -1,12 +1,17 @@
 // Serving loop on a device-resident SVG: continuous batching of SSSD / MSAD
 // requests into shared frontier rounds (solve.h).
 //
-// A worker thread owns the slot engine.  Between rounds it admits queued
-// requests into free slots (up to max_batch per round); a slot is the unit of
-// device memory, so the number of slots follows the memory budget.  Requests
-// therefore join while others are mid-search, share every expand launch with
-// them, and leave as soon as their own search converges.  The reference
-// answers one query at a time on the host (SVG.cu:369-413; kernel.cu:63-92).
+// The request slots are split into groups ("streams"), each a slot engine
+// with its own HIP stream and worker thread.  Between rounds a worker admits
+// queued requests into its free slots (up to max_batch per round); a slot is
+// the unit of device memory, so the number of slots follows the memory
+// budget.  Requests therefore join while others are mid-search, share every
+// expand launch of their group, and leave as soon as their own search
+// converges.  The groups' rounds run concurrently on the device: the passes
+// of one round are short and latency-bound, so a second group's passes fill
+// the time the first group's waves spend waiting on memory, and each round
+// carries only its own group's frontier.  The reference answers one query at
+// a time on the host (SVG.cu:369-413; kernel.cu:63-92).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,21 +44,39 @@ struct Ticket {
     int state = 0;           // 0 queued, 1 running, 2 done, 3 failed
 };
 
+// One slot group: a slot engine, its stream and worker, and the pinned
+// buffers its results leave through.
+struct Group {
+    SolverState st;
+    int slots = 0;
+    std::thread worker;
+    std::vector<Ticket*> slotTicket;      // guarded by the server mutex
+    int running = 0;                      // slots searching (guarded by the server mutex)
+    std::atomic<long long> launches{0};
+    // pipelining: two batches of rounds in flight, their slot records copied
+    // into alternating pinned buffers; finished MSAD results leave through a
+    // second stream into per-slot pinned buffers
+    SlotParam* hs[2] = {nullptr, nullptr};
+    hipEvent_t batchEv[2] = {nullptr, nullptr};
+    hipStream_t stream = nullptr;         // admissions, rounds, slot records
+    hipStream_t fetchStream = nullptr;
+    std::vector<hipEvent_t> fetchEv;
+    double* fDist = nullptr;              // slots * V (pinned)
+    int32_t* fSrc = nullptr;              // slots * V (pinned)
+};
 
 }  // namespace
 
 struct svg_server_s {
     svg_graph_s* g = nullptr;
-    SolverState st;
+    std::vector<std::unique_ptr<Group>> groups;
     int slots = 0;
     int maxBatch = 0;
     size_t queueCap = 0;
-    std::thread worker;
     std::mutex mu;
     std::condition_variable cvWork, cvDone;
-    std::deque<long long> queue;
+    std::deque<Ticket*> queue;
     std::vector<std::unique_ptr<Ticket>> tickets;
-    std::vector<long long> slotTicket;
     bool stopping = false;
     std::string error;
     // stats
@@ -63,15 +86,6 @@ struct svg_server_s {
     Clock::time_point firstSubmit, lastDone;
     bool anySubmit = false;
     long long nKind[2] = {0, 0}, roundsKind[2] = {0, 0}, settledKind[2] = {0, 0}, relaxedKind[2] = {0, 0};
-    // pipelining: two batches of rounds in flight, their slot records copied
-    // into alternating pinned buffers; finished MSAD results leave through a
-    // second stream into per-slot pinned buffers
-    SlotParam* hs[2] = {nullptr, nullptr};
-    hipEvent_t batchEv[2] = {nullptr, nullptr};
-    hipStream_t fetchStream = nullptr;
-    std::vector<hipEvent_t> fetchEv;
-    double* fDist = nullptr;        // slots * V (pinned)
-    int32_t* fSrc = nullptr;        // slots * V (pinned)
 };
 
 namespace {
@@ -81,98 +95,121 @@ enum : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FETCHING = 2 };
 void fail_all(svg_server_s* S, const std::string& e)
 {
     std::lock_guard<std::mutex> lk(S->mu);
-    S->error = e;
-    for (int q = 0; q < S->slots; ++q)
-        if (S->slotTicket[q] != -1) { S->tickets[S->slotTicket[q]]->state = 3; S->slotTicket[q] = -1; }
-    for (long long id : S->queue) S->tickets[id]->state = 3;
+    if (S->error.empty()) S->error = e;
+    for (auto& G : S->groups)
+        for (Ticket*& T : G->slotTicket)
+            if (T) { T->state = 3; T = nullptr; }
+    for (Ticket* T : S->queue) T->state = 3;
     S->queue.clear();
     S->cvDone.notify_all();
 }
 
-void finish_ticket(svg_server_s* S, int q, const SlotParam& sp)
+void finish_ticket(svg_server_s* S, Group& G, int q, const SlotParam& sp)
 {
-    Ticket& T = *S->tickets[S->slotTicket[q]];
-    const int kd = T.kind == Q_MSAD ? 1 : 0;
+    solver_release_slot(G.st, q);
+    const Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    Ticket* T = G.slotTicket[q];
+    if (!T) return;                       // failed meanwhile
+    const int kd = T->kind == Q_MSAD ? 1 : 0;
     S->nKind[kd] += 1;
     S->roundsKind[kd] += sp.rounds;
     S->settledKind[kd] += sp.settled;
     S->relaxedKind[kd] += sp.relaxed;
-    solver_release_slot(S->st, q);
-    const Clock::time_point now = Clock::now();
-    std::lock_guard<std::mutex> lk(S->mu);
-    T.done = now;
-    T.state = 2;
-    S->lat.push_back(std::chrono::duration<double, std::milli>(now - T.submit).count());
+    T->done = now;
+    T->state = 2;
+    S->lat.push_back(std::chrono::duration<double, std::milli>(now - T->submit).count());
     S->lastDone = now;
-    S->slotTicket[q] = -1;
+    G.slotTicket[q] = nullptr;
     S->cvDone.notify_all();
+}
+
+// A group takes a queued request only while no other group with a free slot
+// is searching fewer requests, so concurrent requests spread over the groups.
+bool least_loaded(const svg_server_s* S, const Group* G, int mine)
+{
+    for (const auto& O : S->groups)
+        if (O.get() != G && O->running < O->slots && O->running < mine) return false;
+    return true;
 }
 
 }  // namespace
 
-// The worker keeps the device busy: it queues batch k of rounds, then reads
+// The worker keeps its stream busy: it queues batch k of rounds, then reads
 // the slot records of batch k-1 (complete by then) while batch k runs.  A
 // request admitted between two batches joins the next one; its record is
 // trusted from the first batch queued after its admission.
-static void worker_loop(svg_server_s* S)
+static void worker_loop(svg_server_s* S, Group* Gp)
 {
+    Group& G = *Gp;
     svg_ensure_device();
     const DevMesh& m = S->g->mesh->d;
-    std::vector<int> state((size_t)S->slots, SLOT_FREE);
-    std::vector<long long> admitBatch((size_t)S->slots, 0);
-    std::vector<SlotParam> fetchRec((size_t)S->slots);
+    std::vector<int> state((size_t)G.slots, SLOT_FREE);
+    std::vector<long long> admitBatch((size_t)G.slots, 0);
+    std::vector<SlotParam> fetchRec((size_t)G.slots);
     long long k = 0;                 // batches queued so far
     long long nextRead = 0;          // batches before this one have been read
-    const size_t V = (size_t)S->st.V;
+    const size_t V = (size_t)G.st.V;
     for (;;) {
-        // results that have arrived in pinned memory
-        for (int q = 0; q < S->slots; ++q) {
-            if (state[q] != SLOT_FETCHING || hipEventQuery(S->fetchEv[q]) != hipSuccess) continue;
-            Ticket& T = *S->tickets[S->slotTicket[q]];
-            if (T.distOut) std::memcpy(T.distOut, S->fDist + q * V, sizeof(double) * V);
-            if (T.srcidOut) std::memcpy(T.srcidOut, S->fSrc + q * V, sizeof(int32_t) * V);
-            finish_ticket(S, q, fetchRec[q]);
+        // results that have arrived in pinned memory (a slot's ticket is only
+        // changed by this worker, or cleared by fail_all)
+        for (int q = 0; q < G.slots; ++q) {
+            if (state[q] != SLOT_FETCHING || hipEventQuery(G.fetchEv[q]) != hipSuccess) continue;
+            Ticket* T;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                T = G.slotTicket[q];
+            }
+            if (T && T->distOut) std::memcpy(T->distOut, G.fDist + q * V, sizeof(double) * V);
+            if (T && T->srcidOut) std::memcpy(T->srcidOut, G.fSrc + q * V, sizeof(int32_t) * V);
+            finish_ticket(S, G, q, fetchRec[q]);
             state[q] = SLOT_FREE;
         }
         // admissions into free slots
         std::vector<Admission> adm;
         int running = 0, fetching = 0;
-        for (int q = 0; q < S->slots; ++q) { running += state[q] == SLOT_RUNNING; fetching += state[q] == SLOT_FETCHING; }
+        for (int q = 0; q < G.slots; ++q) { running += state[q] == SLOT_RUNNING; fetching += state[q] == SLOT_FETCHING; }
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (running == 0 && fetching == 0 && nextRead == k && S->queue.empty()) {
-                if (S->stopping) break;
+            if (!S->error.empty()) return;
+            G.running = running;
+            if (running == 0 && fetching == 0 && nextRead == k && (S->queue.empty() || !least_loaded(S, &G, 0))) {
+                if (S->stopping && S->queue.empty()) break;
                 S->cvWork.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
-            for (int q = 0; q < S->slots && !S->queue.empty() && (int)adm.size() < S->maxBatch; ++q) {
+            for (int q = 0; q < G.slots && !S->queue.empty() && (int)adm.size() < S->maxBatch; ++q) {
                 if (state[q] != SLOT_FREE) continue;
-                const long long id = S->queue.front();
+                if (!least_loaded(S, &G, running + (int)adm.size())) break;
+                Ticket* T = S->queue.front();
                 S->queue.pop_front();
-                Ticket& T = *S->tickets[id];
                 Admission a;
-                a.slot = q; a.kind = T.kind; a.target = T.kind == Q_SSSD ? T.t : -1;
-                a.wantParent = T.kind == Q_MSAD && T.srcidOut != nullptr;
-                if (T.kind == Q_SSSD) { a.src.push_back(T.s); a.w.push_back(0.0); }
-                else { a.src = T.src; a.w = T.w; }
-                T.state = 1;
-                S->slotTicket[q] = id;
+                a.slot = q; a.kind = T->kind; a.target = T->kind == Q_SSSD ? T->t : -1;
+                a.wantParent = T->kind == Q_MSAD && T->srcidOut != nullptr;
+                if (T->kind == Q_SSSD) { a.src.push_back(T->s); a.w.push_back(0.0); }
+                else { a.src = T->src; a.w = T->w; }
+                T->state = 1;
+                G.slotTicket[q] = T;
                 adm.push_back(std::move(a));
             }
+            G.running = running + (int)adm.size();
+            int total = 0;
+            for (const auto& O : S->groups) total += O->running;
+            S->maxBatchSeen = std::max(S->maxBatchSeen, total);
         }
         if (!adm.empty()) {
-            std::string e = solver_admit_many(S->st, m, adm);
+            std::string e = solver_admit_many(G.st, m, adm);
             if (!e.empty()) { fail_all(S, e); return; }
             for (const Admission& a : adm) { state[a.slot] = SLOT_RUNNING; admitBatch[a.slot] = k; }
             running += (int)adm.size();
-            S->maxBatchSeen = std::max(S->maxBatchSeen, running);
         }
         // queue the next batch while the previous one may still run
         if (running > 0) {
-            std::string e = solver_enqueue_rounds(S->st, S->g->d, m, S->st.roundsPerSync);
-            if (e.empty()) e = solver_enqueue_status(S->st, S->hs[k & 1]);
-            if (e.empty() && hipEventRecord(S->batchEv[k & 1], 0) != hipSuccess) e = "event record failed";
+            std::string e = solver_enqueue_rounds(G.st, S->g->d, m, G.st.roundsPerSync);
+            if (e.empty()) e = solver_enqueue_status(G.st, G.hs[k & 1]);
+            if (e.empty() && hipEventRecord(G.batchEv[k & 1], G.stream) != hipSuccess) e = "event record failed";
             if (!e.empty()) { fail_all(S, e); return; }
+            G.launches.store(G.st.launches, std::memory_order_relaxed);
             ++k;
         }
         // read the records of the batch before the one just queued (or of the
@@ -180,23 +217,28 @@ static void worker_loop(svg_server_s* S)
         const long long upto = running > 0 ? k - 2 : k - 1;
         for (; nextRead <= upto; ++nextRead) {
             const int pb = (int)(nextRead & 1);
-            if (hipEventSynchronize(S->batchEv[pb]) != hipSuccess) { fail_all(S, "batch wait failed"); return; }
-            const SlotParam* rec = S->hs[pb];
-            if (rec[S->slots].kind) { fail_all(S, "solver frontier overflow"); return; }
-            for (int q = 0; q < S->slots; ++q) {
+            if (hipEventSynchronize(G.batchEv[pb]) != hipSuccess) { fail_all(S, "batch wait failed"); return; }
+            const SlotParam* rec = G.hs[pb];
+            if (rec[G.slots].kind) { fail_all(S, "solver frontier overflow"); return; }
+            for (int q = 0; q < G.slots; ++q) {
                 if (state[q] != SLOT_RUNNING || admitBatch[q] > nextRead || !rec[q].done) continue;
-                Ticket& T = *S->tickets[S->slotTicket[q]];
-                if (T.kind == Q_SSSD) {
-                    if (T.distOut) *T.distOut = rec[q].result;
-                    finish_ticket(S, q, rec[q]);
+                Ticket* T;
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    T = G.slotTicket[q];
+                }
+                if (!T) return;                   // failed meanwhile
+                if (T->kind == Q_SSSD) {
+                    if (T->distOut) *T->distOut = rec[q].result;
+                    finish_ticket(S, G, q, rec[q]);
                     state[q] = SLOT_FREE;
                 } else {
                     std::string e;
-                    if (hipStreamWaitEvent(S->fetchStream, S->batchEv[pb], 0) != hipSuccess) e = "stream wait failed";
+                    if (hipStreamWaitEvent(G.fetchStream, G.batchEv[pb], 0) != hipSuccess) e = "stream wait failed";
                     if (e.empty())
-                        e = solver_enqueue_fetch(S->st, q, S->fetchStream, T.distOut ? S->fDist + q * V : nullptr,
-                                                 T.srcidOut ? S->fSrc + q * V : nullptr, nullptr);
-                    if (e.empty() && hipEventRecord(S->fetchEv[q], S->fetchStream) != hipSuccess)
+                        e = solver_enqueue_fetch(G.st, q, G.fetchStream, T->distOut ? G.fDist + q * V : nullptr,
+                                                 T->srcidOut ? G.fSrc + q * V : nullptr, nullptr);
+                    if (e.empty() && hipEventRecord(G.fetchEv[q], G.fetchStream) != hipSuccess)
                         e = "event record failed";
                     if (!e.empty()) { fail_all(S, e); return; }
                     fetchRec[q] = rec[q];
@@ -206,23 +248,29 @@ static void worker_loop(svg_server_s* S)
         }
         // only results in flight: wait for the oldest rather than spin
         if (running == 0 && nextRead == k && fetching > 0) {
-            for (int q = 0; q < S->slots; ++q)
-                if (state[q] == SLOT_FETCHING) { hipEventSynchronize(S->fetchEv[q]); break; }
+            for (int q = 0; q < G.slots; ++q)
+                if (state[q] == SLOT_FETCHING) { hipEventSynchronize(G.fetchEv[q]); break; }
         }
     }
 }
 
+static void group_free(Group& G)
+{
+    solver_destroy(G.st);
+    for (int i = 0; i < 2; ++i) {
+        if (G.hs[i]) hipHostFree(G.hs[i]);
+        if (G.batchEv[i]) hipEventDestroy(G.batchEv[i]);
+    }
+    for (hipEvent_t e : G.fetchEv) if (e) hipEventDestroy(e);
+    if (G.fetchStream) hipStreamDestroy(G.fetchStream);
+    if (G.stream) hipStreamDestroy(G.stream);
+    if (G.fDist) hipHostFree(G.fDist);
+    if (G.fSrc) hipHostFree(G.fSrc);
+}
+
 static void server_free(svg_server_s* S)
 {
-    solver_destroy(S->st);
-    for (int i = 0; i < 2; ++i) {
-        if (S->hs[i]) hipHostFree(S->hs[i]);
-        if (S->batchEv[i]) hipEventDestroy(S->batchEv[i]);
-    }
-    for (hipEvent_t e : S->fetchEv) if (e) hipEventDestroy(e);
-    if (S->fetchStream) hipStreamDestroy(S->fetchStream);
-    if (S->fDist) hipHostFree(S->fDist);
-    if (S->fSrc) hipHostFree(S->fSrc);
+    for (auto& G : S->groups) group_free(*G);
     delete S;
 }
 
@@ -242,31 +290,44 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     const int memSlots = (int)std::min<size_t>(4096, budget / perSlot);
     if (memSlots < 1) { delete S; return svg_set_error(SVG_E_MEM, "memory budget below one request slot"); }
     slots = std::min(slots, memSlots);
+    const int nGroups = std::max(1, std::min(cfg && cfg->streams > 0 ? cfg->streams : 4, std::min(slots, 8)));
     S->slots = slots;
     S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
     S->queueCap = (size_t)std::max(1024, 64 * slots);
     const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
     const double delta = graph_mean_arc(g->d) * scale;
-    std::string e = solver_create(g->d, g->mesh->d, slots, delta, S->st);
-    if (!e.empty()) { solver_destroy(S->st); delete S; return svg_set_error(SVG_E_MEM, e); }
-    if (cfg && cfg->rounds_per_sync > 0) S->st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
-    if (cfg && cfg->msad_delta_scale > 0.0) S->st.deltaMsad = delta / scale * cfg->msad_delta_scale;
-    bool ok = true;
-    for (int i = 0; i < 2 && ok; ++i) {
-        ok = hipHostMalloc(&S->hs[i], sizeof(SlotParam) * (size_t)(slots + 1), hipHostMallocDefault) == hipSuccess &&
-             hipEventCreateWithFlags(&S->batchEv[i], hipEventDisableTiming) == hipSuccess;
+    for (int gi = 0; gi < nGroups; ++gi) {
+        S->groups.emplace_back(new Group());
+        Group& G = *S->groups.back();
+        G.slots = slots / nGroups + (gi < slots % nGroups ? 1 : 0);
+        std::string e;
+        if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) e = "stream creation failed";
+        G.st.stream = G.stream;
+        if (e.empty()) e = solver_create(g->d, g->mesh->d, G.slots, delta, G.st);
+        if (!e.empty()) { server_free(S); return svg_set_error(SVG_E_MEM, e); }
+        if (cfg && cfg->rounds_per_sync > 0) G.st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
+        // the groups' expand passes share the device: together about two
+        // machine-fills of waves
+        G.st.expandBlocks = std::max(256, std::min(1024, 2048 / nGroups));
+        if (cfg && cfg->msad_delta_scale > 0.0) G.st.deltaMsad = delta / scale * cfg->msad_delta_scale;
+        bool ok = true;
+        for (int i = 0; i < 2 && ok; ++i) {
+            ok = hipHostMalloc(&G.hs[i], sizeof(SlotParam) * (size_t)(G.slots + 1), hipHostMallocDefault) == hipSuccess &&
+                 hipEventCreateWithFlags(&G.batchEv[i], hipEventDisableTiming) == hipSuccess;
+        }
+        ok = ok && hipStreamCreateWithFlags(&G.fetchStream, hipStreamNonBlocking) == hipSuccess;
+        G.fetchEv.assign((size_t)G.slots, nullptr);
+        for (int q = 0; q < G.slots && ok; ++q)
+            ok = hipEventCreateWithFlags(&G.fetchEv[q], hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipHostMalloc(&G.fDist, sizeof(double) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc(&G.fSrc, sizeof(int32_t) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess;
+        if (!ok) {
+            server_free(S);
+            return svg_set_error(SVG_E_MEM, "server staging allocation failed");
+        }
+        G.slotTicket.assign((size_t)G.slots, nullptr);
     }
-    ok = ok && hipStreamCreateWithFlags(&S->fetchStream, hipStreamNonBlocking) == hipSuccess;
-    S->fetchEv.assign((size_t)slots, nullptr);
-    for (int q = 0; q < slots && ok; ++q) ok = hipEventCreateWithFlags(&S->fetchEv[q], hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipHostMalloc(&S->fDist, sizeof(double) * (size_t)slots * g->d.V, hipHostMallocDefault) == hipSuccess &&
-         hipHostMalloc(&S->fSrc, sizeof(int32_t) * (size_t)slots * g->d.V, hipHostMallocDefault) == hipSuccess;
-    if (!ok) {
-        server_free(S);
-        return svg_set_error(SVG_E_MEM, "server staging allocation failed");
-    }
-    S->slotTicket.assign((size_t)slots, -1);
-    S->worker = std::thread(worker_loop, S);
+    for (auto& G : S->groups) G->worker = std::thread(worker_loop, S, G.get());
     *out = S;
     return SVG_OK;
 }
@@ -295,9 +356,9 @@ extern "C" long long svg_server_submit(svg_server_t S, const svg_request* r)
     if (S->queue.size() >= S->queueCap) { S->refused++; return svg_set_error(SVG_E_BUSY, "queue full"); }
     if (!S->anySubmit) { S->firstSubmit = T->submit; S->anySubmit = true; }
     const long long id = (long long)S->tickets.size();
+    S->queue.push_back(T.get());
     S->tickets.push_back(std::move(T));
-    S->queue.push_back(id);
-    S->cvWork.notify_one();
+    S->cvWork.notify_all();
     return id;
 }
 
@@ -339,7 +400,8 @@ extern "C" int svg_server_stats_get(svg_server_t S, svg_server_stats* st)
     const double span = S->anySubmit ? std::chrono::duration<double>(S->lastDone - S->firstSubmit).count() : 0.0;
     st->qps = span > 0 ? (double)S->lat.size() / span : 0.0;
     st->max_batch_seen = S->maxBatchSeen;
-    st->launches = S->st.launches;
+    st->launches = 0;
+    for (const auto& G : S->groups) st->launches += G->launches.load(std::memory_order_relaxed);
     for (int k = 0; k < 2; ++k) {
         st->n_kind[k] = S->nKind[k];
         st->rounds_kind[k] = S->roundsKind[k];
@@ -357,7 +419,8 @@ extern "C" int svg_server_stop(svg_server_t S)
         S->stopping = true;
     }
     S->cvWork.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (auto& G : S->groups)
+        if (G->worker.joinable()) G->worker.join();
     svg_ensure_device();
     server_free(S);
     return SVG_OK;

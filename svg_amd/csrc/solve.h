@@ -88,12 +88,15 @@ struct SolverState {
     double delta = 0.0;                   // bucket width of SSSD queries
     double deltaMsad = 0.0;               // bucket width of MSAD queries
     int roundsPerSync = 4;                // rounds queued between two host checks
+    hipStream_t stream = 0;               // every launch and copy of this state (set before create)
+    int expandBlocks = 1024;              // expand grid: 1024 = 4 waves per SIMD, the pass's occupancy;
+                                          // states sharing the device concurrently take fewer
     unsigned long long* dist = nullptr;   // S*V bits of double
     int32_t* parent = nullptr;            // S*V
     uint32_t* stamp = nullptr;            // S*V round of last improvement
     uint32_t* inq = nullptr;              // S*V frontier membership
     int32_t* srcid = nullptr;             // S*V
-    double* hdist = nullptr;              // S*V straight-line distance to the SSSD target
+    float* hdist = nullptr;               // S*V straight-line distance to the SSSD target (rounded down)
     uint64_t* dl[2] = {nullptr, nullptr};   // deferred lists (kCBlocks regions), by parity
     uint64_t* pl[2] = {nullptr, nullptr};   // kPLists push lists, by parity
     ExpEnt* exp[2] = {nullptr, nullptr};  // rows of a round (kCBlocks regions), by parity
@@ -130,12 +133,12 @@ struct Admission {
     std::vector<int> src;
     std::vector<double> w;
 };
-// Seeds several slots with one init launch and one seed launch (stream 0).
+// Seeds several slots with one init launch and one seed launch (on s.stream).
 std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vector<Admission>& adm);
-// Seeds a slot (device work queued on stream 0).
+// Seeds a slot (device work queued on s.stream).
 std::string solver_admit(SolverState& s, const DevMesh& m, int slot, int kind, const int* src,
                          const double* w, int ns, int target, bool wantParent);
-// Queues nRounds rounds for every active slot on stream 0, ending with the
+// Queues nRounds rounds for every active slot on s.stream, ending with the
 // bookkeeping pass, so the slot records are current when they complete.  A
 // slot that finishes early idles through the remaining rounds.
 std::string solver_enqueue_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, int nRounds);

@@ -25,7 +25,6 @@ namespace svg {
 constexpr unsigned long long kInfBits = 0x7FEFFFFFFFFFFFFFull;   // DBL_MAX (GraphDistInfo default)
 constexpr int kSolveBlock = 256;
 constexpr int kSolveWaves = kSolveBlock / 64;
-constexpr int kExpandBlocks = 1024;     // 4 waves per SIMD resident (the pass's occupancy)
 constexpr int kPBuf = 512;              // per-wave push buffer (entries)
 constexpr int kChunk = 64;              // won-list chunk
 constexpr uint64_t kSentinel = ~0ull;
@@ -42,7 +41,7 @@ struct SolveArgs {
     uint32_t* stamp;
     uint32_t* inq;
     int32_t* srcid;
-    const double* __restrict__ hdist;   // S*V straight-line distance to the SSSD target
+    const float* __restrict__ hdist;    // S*V straight-line distance to the SSSD target, rounded down
     // lists of this round (c = input parity)
     const uint64_t* dIn;                // deferred regions read by classify
     uint64_t* dOut;                     // deferred regions written by classify
@@ -120,7 +119,7 @@ __device__ __forceinline__ int seg_of(const int* pre, int n, int g)
 // Resets the admitted slots and, for an A* slot, tabulates the straight-line
 // distance of every vertex to its target (the heuristic of SVG.cu:402).
 __global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t* stamp, uint32_t* inq,
-                            int32_t* srcid, double* hdist, const double* __restrict__ pos,
+                            int32_t* srcid, float* hdist, const double* __restrict__ pos,
                             const int* slots, const int* targets, int nslots, int V)
 {
     const size_t total = (size_t)nslots * (size_t)V;
@@ -134,7 +133,8 @@ __global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t*
         inq[o] = 0;
         srcid[o] = -1;
         const int t = targets[k];
-        hdist[o] = t >= 0 ? heur(pos, v, t) : 0.0;
+        // rounded toward zero, it stays a lower bound (admissible) in 4 bytes
+        hdist[o] = t >= 0 ? __double2float_rz(heur(pos, v, t)) : 0.0f;
     }
 }
 
@@ -191,9 +191,9 @@ __device__ void book_slot(const SolveArgs& a, int q, const SlotAcc* acc, double 
 // whose expansion value reproduces the new distance exactly.  The round's
 // improvements and exact ties are the only candidates; if a won list
 // overflowed, the round's rows are scanned again instead.  `pre` is LDS of
-// kCBlocks + 1 ints; every block of the grid calls this.
+// kCBlocks + 1 ints; blocks vb = 0 .. nvb-1 of the grid call this.
 __device__ void parent_part(const SolveArgs& a, int p, uint32_t rnd, const WonEnt* won, const ExpEnt* rows,
-                            int* pre)
+                            int* pre, int vb, int nvb)
 {
     SolveCounts* C = a.cnt;
     int cnt[kPLists];
@@ -207,7 +207,7 @@ __device__ void parent_part(const SolveArgs& a, int p, uint32_t rnd, const WonEn
     }
     if (!over) {
         // one pass over the concatenated lists
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        for (int i = vb * blockDim.x + threadIdx.x; i < total; i += nvb * blockDim.x) {
             int l = 0, off = i;
             while (off >= cnt[l]) { off -= cnt[l]; ++l; }
             const WonEnt we = won[(size_t)l * a.wonCap + off];
@@ -223,8 +223,8 @@ __device__ void parent_part(const SolveArgs& a, int p, uint32_t rnd, const WonEn
     __syncthreads();
     const int n = pre[kCBlocks];
     const int stride = C->estride[p];
-    const int nw = gridDim.x * kSolveWaves;
-    for (int g = blockIdx.x * kSolveWaves + wid; g < n; g += nw) {
+    const int nw = nvb * kSolveWaves;
+    for (int g = vb * kSolveWaves + wid; g < n; g += nw) {
         const int sgi = seg_of(pre, kCBlocks, g);
         const ExpEnt ex = rows[(size_t)sgi * stride + (g - pre[sgi])];
         const int qq = (int)((ex.entry >> 32) & 0xFFFFu);
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             lUb[q] = kind == Q_SSSD ? __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]) : 0.0;
         }
     }
-    if (a.doParent) parent_part(a, a.c ^ 1, a.round - 1, a.wonPrev, a.expPrev, pre);
+    if (a.doParent) parent_part(a, a.c ^ 1, a.round - 1, a.wonPrev, a.expPrev, pre, blockIdx.x, gridDim.x);
     // a push list that overflowed (flagged for the host) is read up to its capacity
     for (int i = threadIdx.x; i < kSegs; i += blockDim.x)
         cnts[i] = i < kCBlocks ? C->dcnt[a.c][i]
@@ -319,6 +319,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             q = (int)((e >> 32) & 0xFFFFu);
             const int v = (int)(e & 0xFFFFFFFFu);
             const size_t o = (size_t)q * a.V + v;
+            const int tlv = a.tail[v];       // loaded with the entry's other reads
             double thr, ub = 0.0;
             int done, kind, gen;
             if (table) {
@@ -338,7 +339,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 double key = dv;
                 bool prune = false;
                 if (kind == Q_SSSD) {
-                    key = dv + a.hdist[o];
+                    key = dv + (double)a.hdist[o];
                     prune = key * (1.0 - 1e-12) > ub;
                 }
                 if (prune) {
@@ -351,7 +352,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 } else {
                     act = 2;
                     a.inq[o] = 0;
-                    rowLen = a.tail[v];
+                    rowLen = tlv;
                 }
             }
         }
@@ -368,7 +369,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             if (lane == 0) base = atomicAdd(&sE, __popcll(em));
             base = __shfl(base, 0);
             if (act == 2) {
-                ExpEnt x; x.entry = e; x.dist = db;
+                // the row length replaces the generation tag (rows are checked)
+                ExpEnt x; x.entry = (e & 0x0000FFFFFFFFFFFFull) | ((uint64_t)(uint32_t)rowLen << 48); x.dist = db;
                 a.exp[outBase + base + __popcll(em & lanes_below(lane))] = x;
             }
         }
@@ -443,7 +445,7 @@ __device__ __forceinline__ int won_pos(const SolveArgs& a, WonEnt* list, int* co
 // stale-high (distances only fall), so no improvement is missed.
 constexpr int kSteps = 8;
 
-__global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
+__global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
 {
     __shared__ int pre[kCBlocks + 1];
     __shared__ uint64_t pbuf[kSolveWaves][kPBuf];
@@ -498,7 +500,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
         const int wantParent = __builtin_amdgcn_readfirstlane(sl.z);
         const int gen = __builtin_amdgcn_readfirstlane(sl.w);
         const size_t sb = (size_t)qq * a.V;
-        const int tl = __builtin_amdgcn_readfirstlane(a.tail[vv]);
+        const int tl = __builtin_amdgcn_readfirstlane((int)(e >> 48));
         const size_t row = (size_t)vv * a.K;
         double ub = DBL_MAX;
         if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
@@ -509,16 +511,22 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
             // arcs: loads without branches (an index past the row re-reads
             // its last arc and is masked), so all of them are in flight at once
             double ws[kSteps];
+            float hs[kSteps];
 #pragma unroll
             for (int st = 0; st < kSteps; ++st) {
                 const int i = min(b + st * 64 + lane, tl - 1);
                 xs[st] = a.adj[row + i];
                 ws[st] = a.w[row + i];
             }
+            // target distances, and for A* the targets' bounds with them
 #pragma unroll
             for (int st = 0; st < kSteps; ++st) {
                 nb[st] = (unsigned long long)__double_as_longlong(dvv + ws[st]);
                 val[st] = a.dist[sb + xs[st]];
+            }
+            if (kind == Q_SSSD) {
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st) hs[st] = a.hdist[sb + xs[st]];
             }
             // candidates (with parents kept, an exact tie is one too)
             unsigned cand = 0;
@@ -529,15 +537,12 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
             // A*: drop a candidate that even the straight line to the target
             // cannot bring under the current answer (SVG.cu:399-405).  The
             // smallest key among the candidates feeds the bucket look-ahead.
-            if (kind == Q_SSSD && __ballot(cand != 0)) {
-#pragma unroll
-                for (int st = 0; st < kSteps; ++st)
-                    if ((cand >> st) & 1u) val[st] = (unsigned long long)__double_as_longlong(a.hdist[sb + xs[st]]);
+            if (kind == Q_SSSD) {
 #pragma unroll
                 for (int st = 0; st < kSteps; ++st)
                     if ((cand >> st) & 1u) {
                         const double nd = __longlong_as_double((long long)nb[st]);
-                        const double h = __longlong_as_double((long long)val[st]);
+                        const double h = (double)hs[st];
                         if ((nd + h * (1.0 - 1e-12)) > ub) {
                             cand &= ~(1u << st);
                         } else {
@@ -619,7 +624,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
 __global__ void __launch_bounds__(kSolveBlock) k_parent_fix(SolveArgs a)
 {
     __shared__ int pre[kCBlocks + 1];
-    parent_part(a, a.c, a.round, a.won, a.exp, pre);
+    parent_part(a, a.c, a.round, a.won, a.exp, pre, blockIdx.x, gridDim.x);
 }
 
 // Bookkeeping of this round as a pass of its own (one block): after the last
@@ -720,7 +725,7 @@ size_t solver_bytes_per_slot(int V)
     // dist, parent, stamp, inq, srcid, hdist; deferred lists (2) and row
     // lists (2, 16 B) at two entries per vertex; push lists (2 x kPLists) at
     // one; won lists (2 x kPLists) at a quarter
-    return (size_t)V * (8 + 4 + 4 + 4 + 4 + 8 + 2 * 2 * 8 + 2 * 2 * 16 + 2 * kPLists * 8 +
+    return (size_t)V * (8 + 4 + 4 + 4 + 4 + 4 + 2 * 2 * 8 + 2 * 2 * 16 + 2 * kPLists * 8 +
                         2 * kPLists * sizeof(WonEnt) / 4);
 }
 
@@ -728,6 +733,7 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
 {
     (void)m;
     if (slots < 1 || slots > 4096) return "slots must be in [1, 4096]";
+    if (g.K > 0xFFFF) return "K above 65535: a row's length must fit 16 bits of its list entry";
     s.V = g.V; s.K = g.K; s.S = slots; s.delta = delta; s.deltaMsad = delta;
     const size_t sv = (size_t)slots * (size_t)g.V;
     // live entries are at most one per (slot, vertex); entries of a slot's
@@ -740,7 +746,7 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
     HTRY(hipMalloc(&s.stamp, sizeof(uint32_t) * sv));
     HTRY(hipMalloc(&s.inq, sizeof(uint32_t) * sv));
     HTRY(hipMalloc(&s.srcid, sizeof(int32_t) * sv));
-    HTRY(hipMalloc(&s.hdist, sizeof(double) * sv));
+    HTRY(hipMalloc(&s.hdist, sizeof(float) * sv));
     for (int i = 0; i < 2; ++i) {
         HTRY(hipMalloc(&s.dl[i], sizeof(uint64_t) * s.frontCap));
         HTRY(hipMalloc(&s.pl[i], sizeof(uint64_t) * s.pCap * kPLists));
@@ -777,6 +783,9 @@ std::string solver_create(const DevGraph& g, const DevMesh& m, int slots, double
     s.cur = 0;
     s.round = 1;
     s.gens.assign((size_t)slots, 0);
+    // the set-up copies and fills above ran on the null stream; s.stream may
+    // be a non-blocking stream that does not wait for them
+    HTRY(hipDeviceSynchronize());
     return std::string();
 }
 
@@ -869,7 +878,7 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const size_t intBytes = ((ints.size() * sizeof(int) + 15) / 16) * 16;
     const size_t bytes = intBytes + nseeds * sizeof(double);
     if (bytes > s.stageCap[si]) {
-        HTRY(hipStreamSynchronize(0));              // the old buffer may still be read
+        HTRY(hipStreamSynchronize(s.stream));       // the old buffer may still be read
         hipFree(s.stage[si]);
         s.stageCap[si] = std::max(bytes, (size_t)1 << 20);
         HTRY(hipMalloc(&s.stage[si], s.stageCap[si]));
@@ -879,8 +888,8 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     std::memcpy(s.stageHost[si], ints.data(), ints.size() * sizeof(int));
     double* wh = (double*)((char*)s.stageHost[si] + intBytes);
     for (const Admission& a : adm) for (double w : a.w) *wh++ = w;
-    HTRY(hipMemcpyAsync(s.stage[si], s.stageHost[si], bytes, hipMemcpyHostToDevice, 0));
-    HTRY(hipEventRecord(s.stageEv[si], 0));
+    HTRY(hipMemcpyAsync(s.stage[si], s.stageHost[si], bytes, hipMemcpyHostToDevice, s.stream));
+    HTRY(hipEventRecord(s.stageEv[si], s.stream));
     const int* d = (const int*)s.stage[si];
     const int* dSlots = d;
     const int* dKinds = d + n;
@@ -891,11 +900,11 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const int* dSeeds = d + 6 * n + 1;
     const double* dW = (const double*)((const char*)s.stage[si] + intBytes);
     const size_t total = (size_t)n * s.V;
-    hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, 0,
+    hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, s.stream,
                        s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, n, s.V);
     DevGraph dummy;
     SolveArgs args = make_args(s, dummy, m);
-    hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
+    hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, s.stream, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
                        dSeeds, dW, s.delta, s.deltaMsad);
     s.launches += 2;
     for (const Admission& a : adm) {
@@ -931,12 +940,15 @@ std::string solver_enqueue_rounds(SolverState& s, const DevGraph& g, const DevMe
         SolveArgs a = make_args(s, g, m);
         a.doBook = inClassify && r > 0;
         a.doParent = a.doBook && s.anyParent;
-        hipLaunchKernelGGL(k_classify, dim3(kCBlocks), dim3(kSolveBlock), 0, 0, a);
-        hipLaunchKernelGGL(k_expand, dim3(kExpandBlocks), dim3(kSolveBlock), 0, 0, a);
+        hipLaunchKernelGGL(k_classify, dim3(kCBlocks), dim3(kSolveBlock), 0, s.stream, a);
+        hipLaunchKernelGGL(k_expand, dim3(s.expandBlocks), dim3(kSolveBlock), 0, s.stream, a);
         s.launches += 2;
         if (!inClassify || r == nRounds - 1) {
-            if (s.anyParent) { hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, 0, a); s.launches++; }
-            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, 0, a);
+            if (s.anyParent) {
+                hipLaunchKernelGGL(k_parent_fix, dim3(1024), dim3(kSolveBlock), 0, s.stream, a);
+                s.launches++;
+            }
+            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, s.stream, a);
             s.launches++;
         }
         s.cur ^= 1;
@@ -948,7 +960,7 @@ std::string solver_enqueue_rounds(SolverState& s, const DevGraph& g, const DevMe
 
 std::string solver_enqueue_status(SolverState& s, SlotParam* hostDst)
 {
-    HTRY(hipMemcpyAsync(hostDst, s.sp, sizeof(SlotParam) * (size_t)(s.S + 1), hipMemcpyDeviceToHost, 0));
+    HTRY(hipMemcpyAsync(hostDst, s.sp, sizeof(SlotParam) * (size_t)(s.S + 1), hipMemcpyDeviceToHost, s.stream));
     return std::string();
 }
 
@@ -957,7 +969,7 @@ std::string solver_rounds(SolverState& s, const DevGraph& g, const DevMesh& m, i
     std::string e = solver_enqueue_rounds(s, g, m, nRounds);
     if (e.empty()) e = solver_enqueue_status(s, s.spHost);
     if (!e.empty()) return e;
-    const hipError_t he = hipStreamSynchronize(0);
+    const hipError_t he = hipStreamSynchronize(s.stream);
     if (he != hipSuccess) return std::string("solver round: ") + hipGetErrorString(he);
     if (s.spHost[s.S].kind) return "solver frontier overflow";
     running = 0;
@@ -986,6 +998,7 @@ std::string solver_fetch(SolverState& s, int slot, double* distOut, double* dist
                          int32_t* parentOut)
 {
     const size_t sb = (size_t)slot * s.V;
+    HTRY(hipStreamSynchronize(s.stream));
     if (distT) {
         unsigned long long b;
         HTRY(hipMemcpy(&b, s.dist + sb + s.spHost[slot].target, sizeof(b), hipMemcpyDeviceToHost));
@@ -993,8 +1006,9 @@ std::string solver_fetch(SolverState& s, int slot, double* distOut, double* dist
     }
     if (distOut) HTRY(hipMemcpy(distOut, s.dist + sb, sizeof(double) * (size_t)s.V, hipMemcpyDeviceToHost));
     if (srcidOut) {
-        hipLaunchKernelGGL(k_resolve_src, dim3((s.V + 255) / 256), dim3(256), 0, 0, s.parent, s.srcid, s.dist,
+        hipLaunchKernelGGL(k_resolve_src, dim3((s.V + 255) / 256), dim3(256), 0, s.stream, s.parent, s.srcid, s.dist,
                            s.V, slot);
+        HTRY(hipStreamSynchronize(s.stream));
         HTRY(hipMemcpy(srcidOut, s.srcid + sb, sizeof(int32_t) * (size_t)s.V, hipMemcpyDeviceToHost));
     }
     if (parentOut) HTRY(hipMemcpy(parentOut, s.parent + sb, sizeof(int32_t) * (size_t)s.V, hipMemcpyDeviceToHost));
