@@ -81,12 +81,13 @@ def make_requests(V, n, qps, seed=21, msad_frac=0.3):
     return arrival, reqs
 
 
-def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msad_delta_scale=0.0, streams=0):
+def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msad_delta_scale=0.0, streams=0,
+                 chain_rows=0):
     import ctypes as C
     from svg_amd import _lib as L
     arrival, reqs = make_requests(V, n, qps)
     lib = L.lib()
-    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync, msad_delta_scale, streams)
+    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync, msad_delta_scale, streams, chain_rows)
     srv = C.c_void_p()
     L.check(lib.svg_server_start(g.g, C.byref(cfg), C.byref(srv)))
     arr = (L.Request * n)()
@@ -127,7 +128,7 @@ def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msa
         "msad_fraction": n_msad / n, "inflight_slots": inflight, "launches": int(st.launches),
         "max_concurrent": int(st.max_batch_seen), "delta_scale": delta_scale,
         "rounds_per_sync": rounds_per_sync, "msad_delta_scale": msad_delta_scale or delta_scale,
-        "streams": streams or 4,
+        "streams": streams or 4, "chain_rows": chain_rows or 4,
         "msad_returns": "distances and nearest-source ids",
         "latency_from": "scheduled arrival to host completion",
     }
@@ -173,12 +174,15 @@ def main():
     ap.add_argument("--inflight", type=int, default=64)
     ap.add_argument("--delta-scale", default="1.0",
                     help="bucket width / mean arc; a comma list runs one replay per value")
-    ap.add_argument("--msad-delta-scale", default="4",
+    ap.add_argument("--msad-delta-scale", default="1",
                     help="MSAD bucket width / mean arc (0 = --delta-scale); a comma list runs one replay per value")
     ap.add_argument("--rounds-per-sync", default="4",
                     help="frontier rounds between host checks; a comma list runs one replay per value")
     ap.add_argument("--streams", default="4",
                     help="slot groups on concurrent HIP streams; a comma list runs one replay per value")
+    ap.add_argument("--chain-rows", default="4",
+                    help="same-bucket rows a wave expands itself per round (-1 = none); a comma list runs one "
+                         "replay per value")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (byte-counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -279,15 +283,18 @@ def main():
             for mds in [float(x) for x in args.msad_delta_scale.split(",")]:
                 for rps in [int(x) for x in args.rounds_per_sync.split(",")]:
                     for nst in [int(x) for x in args.streams.split(",")]:
-                        sv, _ = serve_replay(g_keep, V, args.requests, args.qps, args.inflight, ds, rps, mds, nst)
-                        runs.append(sv)
-                        log(f"serving (delta x{ds}, msad delta x{mds or ds}, {rps} rounds/sync, {nst} streams): "
-                            f"p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms qps {sv['achieved_qps']:.1f}; "
-                            f"sssd {sv['sssd']}; msad {sv['msad']}")
+                        for ch in [int(x) for x in args.chain_rows.split(",")]:
+                            sv, _ = serve_replay(g_keep, V, args.requests, args.qps, args.inflight, ds, rps, mds,
+                                                 nst, ch)
+                            runs.append(sv)
+                            log(f"serving (delta x{ds}, msad delta x{mds or ds}, {rps} rounds/sync, {nst} streams, "
+                                f"chain {ch}): p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms "
+                                f"qps {sv['achieved_qps']:.1f}; sssd {sv['sssd']}; msad {sv['msad']}")
         res["serving"] = min(runs, key=lambda r: r["p99_ms"])
         if len(runs) > 1:
             res["serving_sweep"] = [{k: r[k] for k in ("delta_scale", "msad_delta_scale", "rounds_per_sync", "streams",
-                                                       "p50_ms", "p99_ms", "achieved_qps")} for r in runs]
+                                                       "chain_rows", "p50_ms", "p99_ms", "achieved_qps")}
+                                    for r in runs]
 
     # ---- CPU baseline: the oracle on a bounded sample of the same build
     if not args.no_cpu:

@@ -137,6 +137,8 @@ typedef struct {
                                (0 = delta_scale) */
     int streams;            /* slot groups, each with its own HIP stream and worker, whose
                                rounds run concurrently (0 = 4) */
+    int chain_rows;         /* same-bucket rows one wave expands itself within a round,
+                               up to 64 (0 = 4, negative = none) */
 } svg_server_cfg;
 
 typedef struct {

@@ -308,7 +308,8 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
         if (cfg && cfg->rounds_per_sync > 0) G.st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
         // the groups' expand passes share the device: together about two
         // machine-fills of waves
-        G.st.expandBlocks = std::max(256, std::min(1024, 2048 / nGroups));
+        G.st.expandBlocks = std::max(192, std::min(768, 1536 / nGroups));
+        if (cfg && cfg->chain_rows != 0) G.st.chainBudget = std::max(0, std::min(cfg->chain_rows, 64));
         if (cfg && cfg->msad_delta_scale > 0.0) G.st.deltaMsad = delta / scale * cfg->msad_delta_scale;
         bool ok = true;
         for (int i = 0; i < 2 && ok; ++i) {

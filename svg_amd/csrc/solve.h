@@ -89,7 +89,8 @@ struct SolverState {
     double deltaMsad = 0.0;               // bucket width of MSAD queries
     int roundsPerSync = 4;                // rounds queued between two host checks
     hipStream_t stream = 0;               // every launch and copy of this state (set before create)
-    int expandBlocks = 1024;              // expand grid: 1024 = 4 waves per SIMD, the pass's occupancy;
+    int chainBudget = 4;                  // same-bucket rows a wave expands itself per expand pass
+    int expandBlocks = 768;               // expand grid: 768 = 3 waves per SIMD, the pass's occupancy;
                                           // states sharing the device concurrently take fewer
     unsigned long long* dist = nullptr;   // S*V bits of double
     int32_t* parent = nullptr;            // S*V

@@ -66,6 +66,7 @@ struct SolveArgs {
     const WonEnt* wonPrev;              // the previous round's
     size_t wonCap;                      // per won list
     int* ovFlag;                        // set when a list would overflow
+    int chainBudget;                    // rows a wave may chain per expand pass
 };
 
 __device__ __forceinline__ double heur(const double* __restrict__ pos, int v, int t)
@@ -443,13 +444,26 @@ __device__ __forceinline__ int won_pos(const SolveArgs& a, WonEnt* list, int* co
 // A* bound's loads, the atomicMins, the frontier-flag exchanges.  The plain
 // read of a target's distance filters the relaxations first; it can only be
 // stale-high (distances only fall), so no improvement is missed.
+//
+// Chaining: a target this wave improves, whose frontier flag it takes and
+// whose key is still inside the slot's bucket, would be expanded by the next
+// round anyway; up to a budget of such rows per wave are expanded by the
+// wave itself before the pass ends, so one round advances the frontier by
+// several hops.  Any relaxation order reaches the same least fixed point,
+// and a chained row reads its distance after clearing its flag (both where
+// the atomics are performed), so an improvement that lands later pushes the
+// vertex again.
 constexpr int kSteps = 8;
+constexpr int kChainCap = 64;           // per-wave queue of chained rows (LDS)
 
-__global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
+__global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
 {
     __shared__ int pre[kCBlocks + 1];
     __shared__ uint64_t pbuf[kSolveWaves][kPBuf];
     __shared__ int4 lSlot[kLdsSlots];           // kind, target, wantParent, gen
+    __shared__ double lThr[kLdsSlots];          // bucket bound of this round
+    __shared__ int lDone[kLdsSlots];
+    __shared__ int2 cq[kSolveWaves][kChainCap]; // chained rows: slot, vertex
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     SolveCounts* C = a.cnt;
@@ -458,6 +472,8 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
         for (int q = threadIdx.x; q < a.S; q += blockDim.x) {
             const SlotParam& sp = a.sp[q];
             lSlot[q] = make_int4(sp.kind, sp.target, sp.wantParent, sp.gen);
+            lThr[q] = a.thr[a.c * a.S + q];
+            lDone[q] = a.doneP[a.c * a.S + q];
         }
     // state the classify pass has consumed is cleared for the next round
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kAccCopies * a.S; i += gridDim.x * blockDim.x) {
@@ -480,17 +496,19 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
     WonEnt* wonL = a.won + (size_t)list * a.wonCap;
     int* wonC = &C->wcnt[a.c][list * kCntPad];
     int wBase = 0, wUsed = kChunk;
-    for (int g = wave; g < n; g += nw) {
-        // the row's parameters are wave-uniform: keep them in scalar registers
-        const int sgi = seg_of(pre, kCBlocks, g);
-        const ExpEnt ex = a.exp[(size_t)sgi * stride + (g - pre[sgi])];
-        const uint64_t e = ex.entry;
-        const int qq = __builtin_amdgcn_readfirstlane((int)((e >> 32) & 0xFFFFu));
-        const int vv = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFFFFFu));
-        const double dvv = __longlong_as_double((long long)ex.dist);
+    SlotAcc* acc = a.acc + (size_t)(blockIdx.x % kAccCopies) * a.S;
+    const int chainMax = table ? min(a.chainBudget, kChainCap) : 0;
+    int nChain = 0;
+
+    // relaxes the arcs of row vv of slot qq from distance dvv
+    auto relax_row = [&](int qq, int vv, double dvv, int tl) {
         int4 sl;
+        double thrq = 0.0;
+        int doneq = 1;
         if (table) {
             sl = lSlot[qq];
+            thrq = lThr[qq];
+            doneq = lDone[qq];
         } else {
             const SlotParam& sp = a.sp[qq];
             sl = make_int4(sp.kind, sp.target, sp.wantParent, sp.gen);
@@ -499,12 +517,12 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
         const int t = __builtin_amdgcn_readfirstlane(sl.y);
         const int wantParent = __builtin_amdgcn_readfirstlane(sl.z);
         const int gen = __builtin_amdgcn_readfirstlane(sl.w);
+        const bool chainSlot = chainMax > 0 && !doneq;
         const size_t sb = (size_t)qq * a.V;
-        const int tl = __builtin_amdgcn_readfirstlane((int)(e >> 48));
         const size_t row = (size_t)vv * a.K;
         double ub = DBL_MAX;
         if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
-        unsigned long long rowMin = ~0ull;        // smallest candidate key of the row
+        unsigned long long rowMin = ~0ull;        // smallest key left to a later round
         for (int b = 0; b < tl; b += 64 * kSteps) {
             int xs[kSteps];
             unsigned long long nb[kSteps], val[kSteps];
@@ -527,6 +545,9 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
             if (kind == Q_SSSD) {
 #pragma unroll
                 for (int st = 0; st < kSteps; ++st) hs[st] = a.hdist[sb + xs[st]];
+            } else {
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st) hs[st] = 0.0f;
             }
             // candidates (with parents kept, an exact tie is one too)
             unsigned cand = 0;
@@ -535,25 +556,14 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
                 if (b + st * 64 + lane < tl && (wantParent ? nb[st] <= val[st] : nb[st] < val[st]))
                     cand |= 1u << st;
             // A*: drop a candidate that even the straight line to the target
-            // cannot bring under the current answer (SVG.cu:399-405).  The
-            // smallest key among the candidates feeds the bucket look-ahead.
+            // cannot bring under the current answer (SVG.cu:399-405)
             if (kind == Q_SSSD) {
 #pragma unroll
                 for (int st = 0; st < kSteps; ++st)
                     if ((cand >> st) & 1u) {
                         const double nd = __longlong_as_double((long long)nb[st]);
-                        const double h = (double)hs[st];
-                        if ((nd + h * (1.0 - 1e-12)) > ub) {
-                            cand &= ~(1u << st);
-                        } else {
-                            const unsigned long long kb = (unsigned long long)__double_as_longlong(nd + h);
-                            rowMin = kb < rowMin ? kb : rowMin;
-                        }
+                        if ((nd + (double)hs[st] * (1.0 - 1e-12)) > ub) cand &= ~(1u << st);
                     }
-            } else {
-#pragma unroll
-                for (int st = 0; st < kSteps; ++st)
-                    if ((cand >> st) & 1u) rowMin = nb[st] < rowMin ? nb[st] : rowMin;
             }
             if (!__ballot(cand != 0)) continue;
             // atomicMins; val becomes the previous distance
@@ -568,8 +578,14 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
                     else if (nb[st] == val[st] && wantParent) win |= 1u << st;
                 }
             if (wantParent) win |= imp;
+            // val becomes the key (A*: distance + bound)
+#pragma unroll
+            for (int st = 0; st < kSteps; ++st)
+                val[st] = kind == Q_SSSD ? (unsigned long long)__double_as_longlong(
+                                               __longlong_as_double((long long)nb[st]) + (double)hs[st])
+                                         : nb[st];
             // frontier flags of the improved targets
-            unsigned push = 0;
+            unsigned own = 0;
 #pragma unroll
             for (int st = 0; st < kSteps; ++st)
                 if ((imp >> st) & 1u) {
@@ -582,12 +598,31 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
                     // flag (an acquire/release here would write back and
                     // invalidate the L2 at every exchange)
                     if (__hip_atomic_exchange(&a.inq[o], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-                        push |= 1u << st;
+                        own |= 1u << st;
                 }
-            // pushes go through the wave's LDS buffer
+            // a target whose flag this wave took is chained (same bucket,
+            // budget left) or pushed through the wave's LDS buffer; the keys
+            // of improvements left to a later round feed the bucket look-ahead
 #pragma unroll
             for (int st = 0; st < kSteps; ++st) {
-                const bool pu = (push >> st) & 1u;
+                const bool im = (imp >> st) & 1u;
+                const unsigned long long key = val[st];
+                bool pu = (own >> st) & 1u;
+                if (chainSlot) {
+                    const bool ch = pu && __longlong_as_double((long long)key) < thrq;
+                    const unsigned long long cm = __ballot(ch);
+                    if (cm) {
+                        const int room = chainMax - nChain;
+                        const int rank = __popcll(cm & lanes_below(lane));
+                        if (ch && rank < room) {
+                            cq[wid][nChain + rank] = make_int2(qq, xs[st]);
+                            pu = false;
+                        }
+                        nChain += min(room, __popcll(cm));
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+                if (im && (pu || !((own >> st) & 1u))) rowMin = key < rowMin ? key : rowMin;
                 const unsigned long long pm = __ballot(pu);
                 if (pm) {
                     if (nBuf + 64 > kPBuf) flush_pushes(a, buf, nBuf, list, lane);
@@ -611,8 +646,49 @@ __global__ void __launch_bounds__(kSolveBlock, 4) k_expand(SolveArgs a)
             const unsigned long long o = __shfl_xor(rowMin, off);
             rowMin = o < rowMin ? o : rowMin;
         }
-        if (lane == 0 && rowMin != ~0ull)
-            atomicMin(&a.acc[(size_t)(blockIdx.x % kAccCopies) * a.S + qq].minImp, rowMin);
+        if (lane == 0 && rowMin != ~0ull) atomicMin(&acc[qq].minImp, rowMin);
+    };
+
+    // the wave's rows, then the rows it chained (the queue grows while it is
+    // worked through)
+    int g = wave, h = 0;
+    for (;;) {
+        int qq, vv, tl;
+        double dvv;
+        if (g < n) {
+            // the row's parameters are wave-uniform: keep them in scalar registers
+            const int sgi = seg_of(pre, kCBlocks, g);
+            const ExpEnt ex = a.exp[(size_t)sgi * stride + (g - pre[sgi])];
+            g += nw;
+            const uint64_t e = ex.entry;
+            qq = __builtin_amdgcn_readfirstlane((int)((e >> 32) & 0xFFFFu));
+            vv = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFFFFFu));
+            tl = __builtin_amdgcn_readfirstlane((int)(e >> 48));
+            dvv = __longlong_as_double((long long)ex.dist);
+        } else if (h < nChain) {
+            const int2 ce = cq[wid][h++];
+            qq = __builtin_amdgcn_readfirstlane(ce.x);
+            vv = __builtin_amdgcn_readfirstlane(ce.y);
+            const size_t o = (size_t)qq * a.V + vv;
+            unsigned long long d = 0;
+            int t0 = 0;
+            if (lane == 0) {
+                // the flag is cleared before the distance is read (the read
+                // waits on the exchange's result): an improvement after the
+                // clear pushes the vertex again, one before it is in the value
+                if (__hip_atomic_exchange(&a.inq[o], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 1u)
+                    d = atomicMin(&a.dist[o], kInfBits);
+                t0 = a.tail[vv];
+                atomicOr(&acc[qq].expanded, 1);
+                atomicAdd(&acc[qq].settled, 1ull);
+                atomicAdd(&acc[qq].relaxed, (unsigned long long)t0);
+            }
+            dvv = __longlong_as_double((long long)__shfl(d, 0));
+            tl = __builtin_amdgcn_readfirstlane(__shfl(t0, 0));
+        } else {
+            break;
+        }
+        relax_row(qq, vv, dvv, tl);
     }
     flush_pushes(a, buf, nBuf, list, lane);
     for (int j = wUsed + lane; j < kChunk; j += 64)
@@ -711,6 +787,7 @@ static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
     a.won = s.won[c]; a.wonPrev = s.won[c ^ 1]; a.wonCap = s.wonCap;
     a.doBook = 0; a.doParent = 0;
     a.ovFlag = s.ovDev;
+    a.chainBudget = s.chainBudget;
     return a;
 }
 
