@@ -854,9 +854,13 @@ struct BArgs {
     int32_t* cnt;               // chunk * kNumBatchedCounters event counters
     int hopCap;
     const int32_t* order;       // sources in processing order (null: by id)
+    int pseudoShare;            // pseudo events run when nP * pseudoShare >= the wave's stepping lanes
 };
 
 constexpr int kNumBatchedCounters = 13;
+// A wave runs its pseudo-source events once at least 1/kPseudoShare of its
+// stepping lanes wait for one.
+constexpr int kPseudoShare = 4;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
@@ -895,22 +899,50 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
     p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
     p.ph = a.pheap + (size_t)lane * (kHeapCap + 1);
+    using I = Ich<BatchedPolicy>;
+    I ich(p, 0, a.K);
+    Win top;
+    Pseudo ptop;
+    int slot = -1, ev = I::EV_DONE;
+    bool have = false;
+    // Each iteration every lane of the wave takes one step of its source.  A
+    // pseudo-source event (a fan sweep, the longer path) waits until enough
+    // of the wave's lanes want one, so the wave runs that path for many
+    // lanes at once instead of in nearly every iteration; a source's own
+    // order of operations is unchanged.
     for (;;) {
-        const int slot = atomicAdd(a.work, 1);
-        if (slot >= a.nsrc) break;
-        const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
-        policy_tables(p, a, slot, src);
-        p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
-        p.nrecIdx = -1;
-        p.ptail = 1;
-        Ich<BatchedPolicy> ich(p, src, a.K);
-        ich.execute();
-        a.status[slot] = p.flags;
-        a.vcount[slot] = p.vcount;
-        int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
-        c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
-        c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
-        c[10] = 0; c[11] = p.vcount; c[12] = p.scount;
+        if (!have) {
+            slot = atomicAdd(a.work, 1);
+            if (slot >= a.nsrc) break;
+            const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
+            policy_tables(p, a, slot, src);
+            p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
+            p.nrecIdx = -1;
+            p.ptail = 1;
+            ich.reset(src);
+            ich.initialize();
+            ev = ich.next_event(top, ptop);
+            have = true;
+        }
+        if (ev >= I::EV_STALE_W && ev <= I::EV_PSEUDO) {
+            const bool wantP = ev == I::EV_PSEUDO;
+            const int nP = __popcll(__ballot(wantP));
+            const int nAll = __popcll(__ballot(true));
+            if (!wantP || nP * a.pseudoShare >= nAll) {
+                if (ich.do_event(ev, top, ptop)) ev = ich.next_event(top, ptop);
+                else ev = I::EV_DONE;
+            }
+        }
+        if (ev == I::EV_DONE || ev == I::EV_STUCK) {
+            if (ev == I::EV_STUCK) p.count(CNT_ERRORS, 1);
+            a.status[slot] = p.flags;
+            a.vcount[slot] = p.vcount;
+            int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
+            c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
+            c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
+            c[10] = 0; c[11] = p.vcount; c[12] = p.scount;
+            have = false;
+        }
     }
 }
 
@@ -1075,6 +1107,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     std::memset(&a, 0, sizeof(a));
     a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
     a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
+    a.pseudoShare = kPseudoShare;
     std::string err;
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||

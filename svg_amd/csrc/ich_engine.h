@@ -480,6 +480,62 @@ struct Ich {
         }
     }
 
+    // Stepped form of execute() for lanes that share a wave.  next_event()
+    // names the next heap operation of this source without changing state
+    // and do_event() performs it, so a source runs exactly the operations of
+    // execute(), in its order (stale pops are interleaved between the two
+    // heaps, which are independent and do not change births), while the
+    // caller chooses which lanes of a wave step together.
+    enum : int { EV_DONE = 0, EV_STALE_W = 1, EV_STALE_P = 2, EV_WIN = 3, EV_PSEUDO = 4, EV_STUCK = 5 };
+    SVG_HD void reset(int src)
+    {
+        source = src;
+        total = 0;
+        pathPassVert = false;
+    }
+    SVG_HD int next_event(Win& top, Pseudo& ptop)
+    {
+        bool haveTop = false, havePtop = false;
+        if (!p.wempty()) {
+            p.wtop(top);
+            if (top.birth != p.birth_of(top.pseudoSrc)) return EV_STALE_W;
+            haveTop = true;
+        }
+        if (!p.pempty()) {
+            p.ptop(ptop);
+            if (ptop.birth != p.birth_of(ptop.vert)) return EV_STALE_P;
+            havePtop = true;
+        }
+        if (haveTop && (!havePtop || top.minDist < ptop.dist)) return EV_WIN;
+        if (havePtop && (!haveTop || top.minDist >= ptop.dist)) return EV_PSEUDO;
+        // unordered keys (NaN): execute() would spin on the same tops
+        return haveTop || havePtop ? EV_STUCK : EV_DONE;
+    }
+    // Performs `ev`; false once the source is finished (K reached or stopped).
+    SVG_HD bool do_event(int ev, const Win& top, const Pseudo& ptop)
+    {
+        const int nf = p.nf();
+        switch (ev) {
+            case EV_STALE_W: p.wpop(); p.count(CNT_WIN_POP, 1); return true;
+            case EV_STALE_P: p.ppop(); p.count(CNT_PSEUDO_POP, 1); return true;
+            case EV_WIN:
+                p.wpop();
+                p.count(CNT_WIN_POP, 1);
+                if (top.level > nf) return true;
+                propagate(top);
+                break;
+            case EV_PSEUDO:
+                p.ppop();
+                p.count(CNT_PSEUDO_POP, 1);
+                if (ptop.level >= nf) return true;
+                gen_sub_wins(ptop);
+                break;
+            default: return false;
+        }
+        if (K != -1 && total >= K) return false;
+        return !p.stop();
+    }
+
     // One unfolding step (ICHDevice.cu:420-462).
     SVG_HD bool unfold(Unfold& u)
     {
