@@ -750,18 +750,21 @@ struct BatchedPolicy {
             }
         };
         pick(1);
-        bool movedRoot = false;
+        // The new root is known from the first level: its record is loaded
+        // now and arrives while the sift continues (the sift writes only
+        // heap entries, never records).
+        const bool rootMoves = mc != -1 && mk < top.key;
+        rootKey = rootMoves ? mk : top.key;
+        rootIdx = rootMoves ? mi : top.idx;
+        if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
+        else nrecIdx = -1;
         while (mc != -1 && mk < top.key) {
             BHeapEnt m; m.key = mk; m.idx = mi; m.pad = 0;
             hset(cur, m);
-            if (cur == 1) { rootKey = mk; rootIdx = mi; movedRoot = true; }
             cur = mc;
             pick(cur);
         }
         hset(cur, top);
-        if (!movedRoot) { rootKey = top.key; rootIdx = top.idx; }
-        if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
-        else nrecIdx = -1;
     }
     __device__ void wpush(const Win& w)
     {

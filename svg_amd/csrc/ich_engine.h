@@ -256,22 +256,15 @@ struct Ich {
         Win lw, rw;
         bool hasL = true, hasR = true;
         double dOp = vo.dist;
-        // child on e1 (left): edge e1 runs b -> op; its next is e2 (op -> a)
-        // child on e2 (right): edge e2 runs op -> a; its next is e0 (a -> b)
-        if (interX <= leftx) {
+        // the ray from the source through the far vertex passes left of the
+        // window (only the right child), right of it (only the left child),
+        // or through it (both, after the split and vertex updates)
+        const bool onlyR = interX <= leftx;
+        const bool onlyL = !onlyR && interX >= rightx;
+        if (onlyR) {
             hasL = false;
-            double t0 = intersect(sx, sy, leftx, 0.0, v2x, v2y, v1x, v1y);
-            double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
-            build_window(win, t.e2, l2, t0, t1, v2x, v2y, v1x, v1y, rw);
-            p.count(CNT_CHILD_BUILT, 1);
-            if (!is_valid(rw, false, dOp, va.dist, vb.dist, l2, l0, l1)) hasR = false;
-        } else if (interX >= rightx) {
+        } else if (onlyL) {
             hasR = false;
-            double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
-            double t1 = intersect(sx, sy, rightx, 0.0, v0x, v0y, v2x, v2y);
-            build_window(win, t.e1, l1, t0, t1, v0x, v0y, v2x, v2y, lw);
-            p.count(CNT_CHILD_BUILT, 1);
-            if (!is_valid(lw, true, vb.dist, dOp, va.dist, l1, l2, l0)) hasL = false;
         } else {
             const double directDist = len2(v2x - sx, v2y - sy);
             const double cand = directDist + win.ps;
@@ -306,18 +299,25 @@ struct Ich {
                     }
                 }
             }
-            if (hasL) {
-                double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
-                build_window(win, t.e1, l1, t0, 0.0, v0x, v0y, v2x, v2y, lw);
-                p.count(CNT_CHILD_BUILT, 1);
-                if (!is_valid(lw, true, vb.dist, dOp, va.dist, l1, l2, l0)) hasL = false;
-            }
-            if (hasR) {
-                double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
-                build_window(win, t.e2, l2, 1.0, t1, v2x, v2y, v1x, v1y, rw);
-                p.count(CNT_CHILD_BUILT, 1);
-                if (!is_valid(rw, false, dOp, va.dist, vb.dist, l2, l0, l1)) hasR = false;
-            }
+        }
+        // Each child is built by one code path for all three cases (the
+        // cases differ only in the clipping parameters), so lanes of a wave
+        // in different cases build their children together.
+        // child on e1 (left): edge e1 runs b -> op; its next is e2 (op -> a)
+        if (hasL) {
+            const double t0 = intersect(sx, sy, leftx, 0.0, v0x, v0y, v2x, v2y);
+            const double t1 = onlyL ? intersect(sx, sy, rightx, 0.0, v0x, v0y, v2x, v2y) : 0.0;
+            build_window(win, t.e1, l1, t0, t1, v0x, v0y, v2x, v2y, lw);
+            p.count(CNT_CHILD_BUILT, 1);
+            if (!is_valid(lw, true, vb.dist, dOp, va.dist, l1, l2, l0)) hasL = false;
+        }
+        // child on e2 (right): edge e2 runs op -> a; its next is e0 (a -> b)
+        if (hasR) {
+            const double t0 = onlyR ? intersect(sx, sy, leftx, 0.0, v2x, v2y, v1x, v1y) : 1.0;
+            const double t1 = intersect(sx, sy, rightx, 0.0, v2x, v2y, v1x, v1y);
+            build_window(win, t.e2, l2, t0, t1, v2x, v2y, v1x, v1y, rw);
+            p.count(CNT_CHILD_BUILT, 1);
+            if (!is_valid(rw, false, dOp, va.dist, vb.dist, l2, l0, l1)) hasR = false;
         }
         if (hasL) { lw.twin = t.tw1; push_win(lw); }
         if (hasR) { rw.twin = t.tw2; push_win(rw); }
