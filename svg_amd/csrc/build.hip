@@ -727,6 +727,64 @@ struct BatchedPolicy {
         w.birth = (int)(int8_t)(r.lvlb & 0xFF);
         w.level = r.lvlb >> 8;
     }
+    // One 3-level block of the global heap part (slots 1..7 of its 8), read
+    // with seven independent 16-B loads.
+    __device__ __forceinline__ void load_block(int blk, double (&k)[8], int (&ix)[8]) const
+    {
+        const uint4* q = reinterpret_cast<const uint4*>(wh + (size_t)blk * 8);
+#pragma unroll
+        for (int s = 1; s < 8; ++s) {
+            const uint4 v = q[s];
+            k[s] = __hiloint2double((int)v.y, (int)v.x);
+            ix[s] = (int)v.z;
+        }
+    }
+    // Sift-down through the three global levels below `cur`, whose children
+    // are the roots of two adjacent blocks: both blocks are read at once, so
+    // the three levels cost one round trip to memory instead of three.
+    // Returns true when the sift continues below the third level.
+    __device__ __forceinline__ bool sift_blocks(int& cur, const BHeapEnt& top, int blkOfLeft)
+    {
+        double ka[8], kb[8];
+        int ia[8], ib[8];
+        load_block(blkOfLeft, ka, ia);
+        load_block(blkOfLeft + 1, kb, ib);
+        // level 1 of the blocks: the two roots
+        int l = 2 * cur, r = l + 1;
+        if (l >= wtail) return false;
+        bool right = r < wtail && !(ka[1] < kb[1]);
+        double ck = right ? kb[1] : ka[1];
+        int ci = right ? ib[1] : ia[1];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
+        cur = right ? r : l;
+        // the chosen block's entries, by select (no dynamic register index)
+        double x[8];
+        int xi[8];
+#pragma unroll
+        for (int s = 2; s < 8; ++s) { x[s] = right ? kb[s] : ka[s]; xi[s] = right ? ib[s] : ia[s]; }
+        // level 2: slots 2, 3
+        l = 2 * cur; r = l + 1;
+        if (l >= wtail) return false;
+        right = r < wtail && !(x[2] < x[3]);
+        ck = right ? x[3] : x[2];
+        ci = right ? xi[3] : xi[2];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
+        cur = right ? r : l;
+        // level 3: slots 4, 5 (under slot 2) or 6, 7 (under slot 3)
+        const double y0 = right ? x[6] : x[4], y1 = right ? x[7] : x[5];
+        const int j0 = right ? xi[6] : xi[4], j1 = right ? xi[7] : xi[5];
+        l = 2 * cur; r = l + 1;
+        if (l >= wtail) return false;
+        right = r < wtail && !(y0 < y1);
+        ck = right ? y1 : y0;
+        ci = right ? j1 : j0;
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
+        cur = right ? r : l;
+        return true;
+    }
     __device__ void wpop()
     {
         wrelease(rootIdx);
@@ -758,12 +816,18 @@ struct BatchedPolicy {
         rootIdx = rootMoves ? mi : top.idx;
         if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
         else nrecIdx = -1;
+        // levels in LDS (children of nodes 1..7)
+        bool deeper = false;
         while (mc != -1 && mk < top.key) {
             BHeapEnt m; m.key = mk; m.idx = mi; m.pad = 0;
             hset(cur, m);
             cur = mc;
+            if (cur >= 8) { deeper = true; break; }
             pick(cur);
         }
+        // global levels: children of nodes 8..15 root blocks 0..15, children
+        // of nodes 64..127 root blocks 16..143
+        if (deeper && sift_blocks(cur, top, 2 * cur - 16)) sift_blocks(cur, top, 16 + (2 * cur - 128));
         hset(cur, top);
     }
     __device__ void wpush(const Win& w)

@@ -512,25 +512,23 @@ struct Ich {
         return haveTop || havePtop ? EV_STUCK : EV_DONE;
     }
     // Performs `ev`; false once the source is finished (K reached or stopped).
+    // A stale pop and a main pop of the same heap share one pop call site,
+    // so lanes of a wave doing either run it together.
     SVG_HD bool do_event(int ev, const Win& top, const Pseudo& ptop)
     {
         const int nf = p.nf();
-        switch (ev) {
-            case EV_STALE_W: p.wpop(); p.count(CNT_WIN_POP, 1); return true;
-            case EV_STALE_P: p.ppop(); p.count(CNT_PSEUDO_POP, 1); return true;
-            case EV_WIN:
-                p.wpop();
-                p.count(CNT_WIN_POP, 1);
-                if (top.level > nf) return true;
-                propagate(top);
-                break;
-            case EV_PSEUDO:
-                p.ppop();
-                p.count(CNT_PSEUDO_POP, 1);
-                if (ptop.level >= nf) return true;
-                gen_sub_wins(ptop);
-                break;
-            default: return false;
+        if (ev == EV_STALE_W || ev == EV_WIN) {
+            p.wpop();
+            p.count(CNT_WIN_POP, 1);
+            if (ev == EV_STALE_W || top.level > nf) return true;
+            propagate(top);
+        } else if (ev == EV_STALE_P || ev == EV_PSEUDO) {
+            p.ppop();
+            p.count(CNT_PSEUDO_POP, 1);
+            if (ev == EV_STALE_P || ptop.level >= nf) return true;
+            gen_sub_wins(ptop);
+        } else {
+            return false;
         }
         if (K != -1 && total >= K) return false;
         return !p.stop();
