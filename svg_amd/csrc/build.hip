@@ -238,6 +238,7 @@ struct PlainPolicy {
         vt[k].srcId = (unsigned)srcVert;
     }
     __device__ void vtouch(int v) { unsigned k = vkey((unsigned)v); vt[k].index = v; }
+    __device__ void push_hint() {}
     __device__ int birth_of(int v) { return (int)vget(v).birth; }
     __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& s)
     {
@@ -541,6 +542,28 @@ struct BatchedPolicy {
     {
         if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
         else wh[bslot(i)] = e;
+        if (i == pfI0) pfE0 = e;
+        if (i == pfI1) pfE1 = e;
+    }
+    // Parents of the next two insertion slots, read ahead (at the start of a
+    // propagation, while its face and table reads are in flight) so the first
+    // comparison of each of the step's pushes finds its entry in registers;
+    // hset keeps the copies current.
+    int pfI0 = -1, pfI1 = -1;
+    BHeapEnt pfE0, pfE1;
+    __device__ void push_hint()
+    {
+        const int a = wtail / 2, b = (wtail + 1) / 2;
+        pfI0 = a >= kHeapLds ? a : -1;
+        pfI1 = b >= kHeapLds && b != a ? b : -1;
+        if (pfI0 >= 0) pfE0 = wh[bslot(a)];
+        if (pfI1 >= 0) pfE1 = wh[bslot(b)];
+    }
+    __device__ BHeapEnt hget_push(int i) const
+    {
+        if (i == pfI0) return pfE0;
+        if (i == pfI1) return pfE1;
+        return hget(i);
     }
     // pseudo heap
     BPsEnt* ph; int ptail;
@@ -843,7 +866,7 @@ struct BatchedPolicy {
         int cur = wtail - 1, p = cur / 2;
         const double key = w.minDist;
         while (p > 0) {
-            const BHeapEnt pe = hget(p);
+            const BHeapEnt pe = hget_push(p);
             if (!(pe.key > key)) break;
             hset(cur, pe);
             cur = p; p = cur / 2;

@@ -241,6 +241,7 @@ struct Ich {
         const int e0 = win.twin;
         if (e0 == -1) return;
         p.count(CNT_PROPAGATE, 1);
+        p.push_hint();
         Tri t;
         p.face3(e0, t);
         VInfo va, vb, vo;
