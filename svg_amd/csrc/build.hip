@@ -883,22 +883,44 @@ struct BatchedPolicy {
         q.dist = e.key; q.vert = e.vert; q.pseudoSrc = e.pseudoSrc;
         q.birth = (int)(int8_t)(e.lvlb & 0xFF); q.level = e.lvlb >> 8;
     }
+    // Sift-down two levels per round trip: a node's children and
+    // grandchildren (contiguous in the array) are read together.
     __device__ void ppop()
     {
         --ptail;
         const BPsEnt top = ph[ptail];
-        int cur = 1, l = 2, r = 3, mc = -1;
-        if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
-        else if (l < ptail) mc = l;
+        int cur = 1;
         bool movedRoot = false;
-        while (mc != -1 && ph[mc].key < top.key) {
-            const BPsEnt c = ph[mc];
+        auto sel = [](bool c, const BPsEnt& a, const BPsEnt& b) {
+            BPsEnt r;
+            r.key = c ? a.key : b.key; r.vert = c ? a.vert : b.vert;
+            r.pseudoSrc = c ? a.pseudoSrc : b.pseudoSrc; r.lvlb = c ? a.lvlb : b.lvlb; r.pad = 0;
+            return r;
+        };
+        for (;;) {
+            const int l = 2 * cur, gl = 2 * l;
+            if (l >= ptail) break;
+            BPsEnt c0 = ph[l], c1 = c0, g0 = c0, g1 = c0, g2 = c0, g3 = c0;
+            if (l + 1 < ptail) c1 = ph[l + 1];
+            if (gl < ptail) g0 = ph[gl];
+            if (gl + 1 < ptail) g1 = ph[gl + 1];
+            if (gl + 2 < ptail) g2 = ph[gl + 2];
+            if (gl + 3 < ptail) g3 = ph[gl + 3];
+            // the left child only if it is strictly smaller (PriorityQueue.cuh:85-88)
+            const bool right = l + 1 < ptail && !(c0.key < c1.key);
+            const BPsEnt c = sel(right, c1, c0);
+            if (!(c.key < top.key)) break;
             ph[cur] = c;
             if (cur == 1) { pRoot = c; movedRoot = true; }
-            cur = mc; l = cur * 2; r = cur * 2 + 1;
-            if (r < ptail) mc = ph[l].key < ph[r].key ? l : r;
-            else if (l < ptail) mc = l;
-            else mc = -1;
+            cur = l + (right ? 1 : 0);
+            const int l2 = 2 * cur;
+            if (l2 >= ptail) break;
+            const BPsEnt d0 = sel(right, g2, g0), d1 = sel(right, g3, g1);
+            const bool r2 = l2 + 1 < ptail && !(d0.key < d1.key);
+            const BPsEnt d = sel(r2, d1, d0);
+            if (!(d.key < top.key)) break;
+            ph[cur] = d;
+            cur = l2 + (r2 ? 1 : 0);
         }
         ph[cur] = top;
         if (!movedRoot) pRoot = top;
@@ -999,7 +1021,8 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     // pseudo-source event (a fan sweep, the longer path) waits until enough
     // of the wave's lanes want one, so the wave runs that path for many
     // lanes at once instead of in nearly every iteration; a source's own
-    // order of operations is unchanged.
+    // order of operations is unchanged.  (A stale pseudo pop does not wait:
+    // it stands before every later step of its source.)
     for (;;) {
         if (!have) {
             slot = atomicAdd(a.work, 1);
