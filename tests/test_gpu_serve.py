@@ -22,13 +22,11 @@ def graph():
     return m, g, len(xyz)
 
 
-def test_mixed_stream_equals_alone(graph):
-    m, g, V = graph
-    rng = np.random.default_rng(3)
-    n = 120
+def _replay_matches_alone(g, V, cfg, n=120, seed=3):
+    rng = np.random.default_rng(seed)
     lib = L.lib()
     srv = C.c_void_p()
-    L.check(lib.svg_server_start(g.g, C.byref(L.ServerCfg(16, 0, 8)), C.byref(srv)))
+    L.check(lib.svg_server_start(g.g, C.byref(cfg), C.byref(srv)))
     reqs = (L.Request * n)()
     keep, expect = [], []
     for i in range(n):
@@ -63,13 +61,28 @@ def test_mixed_stream_equals_alone(graph):
     L.check(lib.svg_server_stats_get(srv, C.byref(st)))
     L.check(lib.svg_server_stop(srv))
     assert st.completed == n and np.all(lat >= 0)
-    assert st.max_batch_seen > 1, "requests overlapped on the device"
     for kind, out, sout, ref, sref in expect:
         if kind == "sssd":
             assert out[0] == ref
         else:
             assert np.array_equal(out, ref)
             assert np.array_equal(sout, sref)
+    return st
+
+
+def test_mixed_stream_equals_alone(graph):
+    m, g, V = graph
+    st = _replay_matches_alone(g, V, L.ServerCfg(16, 0, 8))
+    assert st.max_batch_seen > 1, "requests overlapped on the device"
+
+
+@pytest.mark.parametrize("streams,chain,mds", [(1, -1, 0.0), (1, 64, 8.0), (4, -1, 2.0), (3, 16, 0.5), (8, 4, 1.0)])
+def test_server_settings_do_not_change_results(graph, streams, chain, mds):
+    # slot groups, chaining budget and MSAD bucket width change the order of
+    # relaxations only: every answer stays bit-identical to the query alone
+    m, g, V = graph
+    cfg = L.ServerCfg(16, 0, 16, 1.0, 4, mds, streams, chain)
+    _replay_matches_alone(g, V, cfg, n=60, seed=11 + streams)
 
 
 def test_queue_full_is_refused_not_lost(graph):
