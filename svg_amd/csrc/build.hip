@@ -655,7 +655,21 @@ struct BatchedPolicy {
     }
     // The source keeps birth time 0 from Initialize on: no relaxation can
     // improve a distance of 0.
-    __device__ int birth_of(int v) const { return v == src ? 0 : (int)vget(v).birth; }
+    // The births of the last two vertices asked for are kept in registers
+    // (the next event's checks ask for the pseudo top's vertex and the
+    // window top's pseudo-source, which change rarely); vput keeps them
+    // current, the table reset clears them.
+    int bcV0 = -1, bcB0 = 0, bcV1 = -1, bcB1 = 0;
+    __device__ int birth_of(int v)
+    {
+        if (v == src) return 0;
+        if (v == bcV0) return bcB0;
+        if (v == bcV1) return bcB1;
+        const int b = (int)vget(v).birth;
+        bcV1 = bcV0; bcB1 = bcB0;
+        bcV0 = v; bcB0 = b;
+        return b;
+    }
     __device__ SInfo sresolve(unsigned h, int e) const
     {
         for (;;) {
@@ -697,6 +711,8 @@ struct BatchedPolicy {
         }
         vt[h].eb = ((uint32_t)x.enter & kEnterNone) | ((uint32_t)(uint8_t)x.birth << 24);
         vt[h].dist = x.dist;
+        if (v == bcV0) bcB0 = (int)x.birth;
+        if (v == bcV1) bcB1 = (int)x.birth;
     }
     __device__ void vtouch(int) {}
     // --- split table
@@ -992,6 +1008,7 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
     p.vlimit = (int)(a.vcap - (a.vcap >> 2));      // keep the load factor <= 3/4
     p.slimit = (int)(a.scap - (a.scap >> 2));
     p.vcount = 0; p.scount = 0;
+    p.bcV0 = -1; p.bcV1 = -1;
     p.flags = 0; p.dead = false;
     p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
     p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;

@@ -236,14 +236,17 @@ struct Ich {
     // records of the far triangle and its split record are read together up
     // front; the only one the step can change before IsValidWindow reads it
     // again is the opposite vertex, whose new distance is then `cand`.
-    SVG_HD void propagate(const Win& win)
+    SVG_HD void propagate(const Win& win) { Tri t; propagate(win, false, t); }
+    // faceRead: `face` already holds the far face of the window's twin edge.
+    SVG_HD void propagate(const Win& win, bool faceRead, const Tri& face)
     {
         const int e0 = win.twin;
         if (e0 == -1) return;
         p.count(CNT_PROPAGATE, 1);
         p.push_hint();
         Tri t;
-        p.face3(e0, t);
+        if (faceRead) t = face;
+        else p.face3(e0, t);
         VInfo va, vb, vo;
         SInfo sp;
         p.lookup_face(t, va, vb, vo, sp);
@@ -519,10 +522,15 @@ struct Ich {
     {
         const int nf = p.nf();
         if (ev == EV_STALE_W || ev == EV_WIN) {
+            // the far face of a window that will propagate is read before the
+            // pop, so the read overlaps the heap's sift-down
+            const bool prop = ev == EV_WIN && !(top.level > nf) && top.twin != -1;
+            Tri t;
+            if (prop) p.face3(top.twin, t);
             p.wpop();
             p.count(CNT_WIN_POP, 1);
             if (ev == EV_STALE_W || top.level > nf) return true;
-            propagate(top);
+            propagate(top, prop, t);
         } else if (ev == EV_STALE_P || ev == EV_PSEUDO) {
             p.ppop();
             p.count(CNT_PSEUDO_POP, 1);
