@@ -44,7 +44,9 @@ int svg_device_name(char *buf, int cap);
  * angle sums and firstEdge with the reference rules; open meshes use
  * decisions D1/D2/D8/D9 (boundary twin -1). */
 int svg_mesh_from_arrays(const double *xyz, int nv, const int32_t *tri, int nf, svg_mesh_t *out);
-/* Mesh::LoadFromFile (Mesh.cu:24-126): 'v x y z' and 'f a b c' lines. */
+/* Mesh::LoadFromFile (Mesh.cu:24-126): 'v x y z' and 'f a b c' lines as
+ * there; also 'f' corners written i/t/n, i//n or as negative indices, and
+ * polygons (split into fans).  Indices are checked by svg_mesh_from_arrays. */
 int svg_mesh_load_obj(const char *path, svg_mesh_t *out);
 /* Mesh::copyToGPU (Mesh.cu:128-142).  Called implicitly by svg_build. */
 int svg_mesh_upload(svg_mesh_t m);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 #include <unordered_map>
@@ -98,28 +99,54 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
     return std::string();
 }
 
-// OBJ reader with the reference's rules (Mesh.cu:41-60): "v x y z" lines and
-// "f a b c" lines with plain 1-based indices; other lines are skipped.
+// OBJ reader.  The reference (Mesh.cu:41-60) reads "v x y z" lines and the
+// first three plain 1-based indices of "f" lines, skipping everything else;
+// files of that form load identically here.  Beyond it, as real meshes need:
+// a face corner may be written "i", "i/t", "i//n" or "i/t/n" (only the vertex
+// index is used), a negative index counts back from the last vertex read, a
+// face with more than three corners is split into the fan (c0, ci, ci+1) (the
+// reference would keep its first triangle only), and "#" starts a comment.
+// Texture and normal lines ("vt", "vn") are skipped, as there.
 std::string load_obj(const char* path, std::vector<double>& xyz, std::vector<int32_t>& tri)
 {
     std::ifstream in(path);
     if (!in) return std::string("cannot open file ") + path;
     std::string line;
     xyz.clear(); tri.clear();
+    long lineNo = 0;
+    std::vector<long> corner;
     while (std::getline(in, line)) {
+        ++lineNo;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
         if (line.size() < 2) continue;
         std::istringstream ss(line);
-        char t;
-        if (line[0] == 'v' && (line[1] == ' ' || line[1] == '\t')) {
+        std::string tag;
+        ss >> tag;
+        if (tag == "v") {
             double x, y, z;
-            ss >> t >> x >> y >> z;
-            if (!ss) return "bad vertex line: " + line;
+            ss >> x >> y >> z;
+            if (!ss) return "line " + std::to_string(lineNo) + ": bad vertex line: " + line;
             xyz.push_back(x); xyz.push_back(y); xyz.push_back(z);
-        } else if (line[0] == 'f') {
-            long a, b, c;
-            ss >> t >> a >> b >> c;
-            if (!ss) return "bad face line (only 'f a b c' is accepted): " + line;
-            tri.push_back((int32_t)(a - 1)); tri.push_back((int32_t)(b - 1)); tri.push_back((int32_t)(c - 1));
+        } else if (tag == "f") {
+            corner.clear();
+            std::string tok;
+            const long nv = (long)(xyz.size() / 3);
+            while (ss >> tok) {
+                const size_t slash = tok.find('/');
+                const std::string head = tok.substr(0, slash);
+                char* end = nullptr;
+                const long idx = std::strtol(head.c_str(), &end, 10);
+                if (head.empty() || *end != '\0' || idx == 0)
+                    return "line " + std::to_string(lineNo) + ": bad face corner '" + tok + "'";
+                corner.push_back(idx > 0 ? idx - 1 : nv + idx);
+            }
+            if (corner.size() < 3) return "line " + std::to_string(lineNo) + ": face with fewer than 3 corners";
+            for (size_t i = 1; i + 1 < corner.size(); ++i) {
+                tri.push_back((int32_t)corner[0]);
+                tri.push_back((int32_t)corner[i]);
+                tri.push_back((int32_t)corner[i + 1]);
+            }
         }
     }
     return std::string();

@@ -47,3 +47,35 @@ def test_obj_roundtrip(tmp_path):
     for k in a:
         assert np.array_equal(a[k], b[k])
     assert not S.Mesh().LoadFromFile(str(tmp_path / "missing.obj"))
+
+
+def _write(p, text):
+    with open(p, "w") as f:
+        f.write(text)
+
+
+def test_obj_corner_forms_polygons_and_comments(tmp_path):
+    # one unit square split two ways: plain triangles (the reference's form)
+    # against a quad with i/t/n, i//n and negative corners, comments, vt/vn
+    plain = tmp_path / "plain.obj"
+    _write(plain, "v 0 0 0\nv 1 0 0\nv 1 1 0.1\nv 0 1 0\nf 1 2 3\nf 1 3 4\n")
+    fancy = tmp_path / "fancy.obj"
+    _write(fancy, "# a quad\nv 0 0 0\nv 1 0 0\nvt 0 0\nvn 0 0 1\nv 1 1 0.1  # third\nv 0 1 0\n"
+                  "f 1/1/1 2//1 -2/3 -1\n")
+    a = S.Mesh()
+    assert a.LoadFromFile(str(plain))
+    b = S.Mesh()
+    assert b.LoadFromFile(str(fancy))
+    ea, eb = a.export(), b.export()
+    for k in ea:
+        assert np.array_equal(ea[k], eb[k]), k
+    assert (a.vertNum, a.faceNum) == (4, 2)
+
+
+def test_obj_bad_corner_is_an_error(tmp_path):
+    bad = tmp_path / "bad.obj"
+    _write(bad, "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 x 3\n")
+    assert not S.Mesh().LoadFromFile(str(bad))
+    short = tmp_path / "short.obj"
+    _write(short, "v 0 0 0\nv 1 0 0\nf 1 2\n")
+    assert not S.Mesh().LoadFromFile(str(short))
