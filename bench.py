@@ -82,12 +82,13 @@ def make_requests(V, n, qps, seed=21, msad_frac=0.3):
 
 
 def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msad_delta_scale=0.0, streams=0,
-                 chain_rows=0):
+                 chain_rows=0, queue_cap=0):
     import ctypes as C
     from svg_amd import _lib as L
     arrival, reqs = make_requests(V, n, qps)
     lib = L.lib()
-    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync, msad_delta_scale, streams, chain_rows)
+    cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync, msad_delta_scale, streams, chain_rows,
+                      queue_cap)
     srv = C.c_void_p()
     L.check(lib.svg_server_start(g.g, C.byref(cfg), C.byref(srv)))
     arr = (L.Request * n)()
@@ -122,7 +123,8 @@ def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msa
     ok = lat >= 0
     out = {
         "requests": n, "completed": int(ok.sum()), "refused": int((~ok).sum()),
-        "offered_qps": qps, "achieved_qps": float(ok.sum() / wall),
+        "offered_qps": None if queue_cap else qps, "achieved_qps": float(ok.sum() / wall),
+        "arrivals": "all at once (batch)" if queue_cap else "Poisson",
         "p50_ms": float(np.percentile(lat[ok], 50)), "p99_ms": float(np.percentile(lat[ok], 99)),
         "mean_ms": float(lat[ok].mean()), "max_ms": float(lat[ok].max()),
         "msad_fraction": n_msad / n, "inflight_slots": inflight, "launches": int(st.launches),
@@ -183,6 +185,8 @@ def main():
     ap.add_argument("--chain-rows", default="4",
                     help="same-bucket rows a wave expands itself per round (-1 = none); a comma list runs one "
                          "replay per value")
+    ap.add_argument("--batch", action="store_true",
+                    help="submit every request at once (queue sized to hold them): throughput, not latency")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (byte-counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -284,8 +288,9 @@ def main():
                 for rps in [int(x) for x in args.rounds_per_sync.split(",")]:
                     for nst in [int(x) for x in args.streams.split(",")]:
                         for ch in [int(x) for x in args.chain_rows.split(",")]:
-                            sv, _ = serve_replay(g_keep, V, args.requests, args.qps, args.inflight, ds, rps, mds,
-                                                 nst, ch)
+                            sv, _ = serve_replay(g_keep, V, args.requests, 1e9 if args.batch else args.qps,
+                                                 args.inflight, ds, rps, mds, nst, ch,
+                                                 args.requests if args.batch else 0)
                             runs.append(sv)
                             log(f"serving (delta x{ds}, msad delta x{mds or ds}, {rps} rounds/sync, {nst} streams, "
                                 f"chain {ch}): p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms "

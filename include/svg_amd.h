@@ -141,6 +141,8 @@ typedef struct {
                                rounds run concurrently (0 = 4) */
     int chain_rows;         /* same-bucket rows one wave expands itself within a round,
                                up to 64 (0 = 4, negative = none) */
+    long long queue_cap;    /* requests that may wait for a slot; more are refused with
+                               SVG_E_BUSY (0 = 64 per slot, at least 1024) */
 } svg_server_cfg;
 
 typedef struct {

@@ -41,7 +41,8 @@ class BuildStats(C.Structure):
 class ServerCfg(C.Structure):
     _fields_ = [("max_inflight", C.c_int), ("mem_budget", C.c_longlong), ("max_batch", C.c_int),
                 ("delta_scale", C.c_double), ("rounds_per_sync", C.c_int),
-                ("msad_delta_scale", C.c_double), ("streams", C.c_int), ("chain_rows", C.c_int)]
+                ("msad_delta_scale", C.c_double), ("streams", C.c_int), ("chain_rows", C.c_int),
+                ("queue_cap", C.c_longlong)]
 
 
 class Request(C.Structure):

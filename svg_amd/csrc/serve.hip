@@ -293,7 +293,7 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     const int nGroups = std::max(1, std::min(cfg && cfg->streams > 0 ? cfg->streams : 4, std::min(slots, 8)));
     S->slots = slots;
     S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
-    S->queueCap = (size_t)std::max(1024, 64 * slots);
+    S->queueCap = cfg && cfg->queue_cap > 0 ? (size_t)cfg->queue_cap : (size_t)std::max(1024, 64 * slots);
     const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
     const double delta = graph_mean_arc(g->d) * scale;
     for (int gi = 0; gi < nGroups; ++gi) {
