@@ -983,12 +983,18 @@ struct BArgs {
     int hopCap;
     const int32_t* order;       // sources in processing order (null: by id)
     int pseudoShare;            // pseudo events run when nP * pseudoShare >= the wave's stepping lanes
+    int lanesPerWave;           // source lanes per 64-wide wave
 };
 
 constexpr int kNumBatchedCounters = 13;
 // A wave runs its pseudo-source events once at least 1/kPseudoShare of its
 // stepping lanes wait for one.
 constexpr int kPseudoShare = 4;
+// Source lanes per wave.  With one source per lane the chip holds about one
+// wave per SIMD, so a wave's memory waits go unhidden; half-filled waves
+// give two waves per SIMD that overlap each other's waits, at the price of
+// issuing each instruction for fewer sources.
+constexpr int kLanesPerWave = 32;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
@@ -1017,7 +1023,10 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
 __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
 {
     extern __shared__ double ldsHeap[];       // (kHeapLds-1) keys then idx per thread
-    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    // the first lanesPerWave lanes of each wave run sources
+    const int wl = threadIdx.x & 63;
+    if (wl >= a.lanesPerWave) return;
+    const int lane = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * a.lanesPerWave + wl;
     if (lane >= a.lanes) return;
     BatchedPolicy p;
     policy_mesh(p, a);
@@ -1238,6 +1247,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
     a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
     a.pseudoShare = kPseudoShare;
+    a.lanesPerWave = kLanesPerWave;
     std::string err;
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
@@ -1280,7 +1290,9 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         const int pl = std::min(lanes, n);
         a.lanes = pl;
         const size_t ldsBytes = (size_t)(kHeapLds - 1) * block * (sizeof(double) + sizeof(int));
-        hipLaunchKernelGGL(batched_prop_kernel, dim3((pl + block - 1) / block), dim3(block), ldsBytes, 0, a);
+        const size_t threads = (size_t)((pl + a.lanesPerWave - 1) / a.lanesPerWave) * 64;
+        hipLaunchKernelGGL(batched_prop_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), ldsBytes,
+                           0, a);
         hipEventRecord(e1, 0);
         const int eblocks = std::min(n, 16384);
         if (vcap <= 1024)
