@@ -530,14 +530,6 @@ struct BatchedPolicy {
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
 
-    __device__ BHeapEnt hget(int i) const
-    {
-        if (i < kHeapLds) {
-            BHeapEnt e; e.key = lk[(i - 1) * lstride]; e.idx = li[(i - 1) * lstride]; e.pad = 0;
-            return e;
-        }
-        return wh[bslot(i)];
-    }
     __device__ void hset(int i, const BHeapEnt& e)
     {
         if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
@@ -558,12 +550,6 @@ struct BatchedPolicy {
         pfI1 = b >= kHeapLds && b != a ? b : -1;
         if (pfI0 >= 0) pfE0 = wh[bslot(a)];
         if (pfI1 >= 0) pfE1 = wh[bslot(b)];
-    }
-    __device__ BHeapEnt hget_push(int i) const
-    {
-        if (i == pfI0) return pfE0;
-        if (i == pfI1) return pfE1;
-        return hget(i);
     }
     // pseudo heap
     BPsEnt* ph; int ptail;
@@ -824,29 +810,43 @@ struct BatchedPolicy {
         cur = right ? r : l;
         return true;
     }
+    // LDS-only and global-only reads of heap entries: with one accessor
+    // for both, the compiler merges the two paths into generic (flat)
+    // loads, whose waits also wait for every global load in flight.
+    __device__ __forceinline__ double lkey(int i) const { return lk[(i - 1) * lstride]; }
+    __device__ __forceinline__ int lidx(int i) const { return li[(i - 1) * lstride]; }
+    __device__ __forceinline__ BHeapEnt gget(int i) const { return wh[bslot(i)]; }
     __device__ void wpop()
     {
         wrelease(rootIdx);
         --wtail;
-        const BHeapEnt top = hget(wtail);
-        // Children are compared field by field (no select between two
-        // aggregates, which would put them on the stack).
-        int cur = 1, mc = -1;
-        double mk = 0.0; int mi = 0;
-        auto pick = [&](int c) {
-            const int l = 2 * c, r = l + 1;
+        // the last entry: read from both parts, kept from the one it is in
+        double tk;
+        int ti;
+        {
+            const int il = wtail < kHeapLds ? wtail : kHeapLds - 1;
+            const BHeapEnt g = gget(wtail < kHeapLds ? kHeapLds : wtail);
+            const double lkk = lkey(il);
+            const int lii = lidx(il);
+            tk = wtail < kHeapLds ? lkk : g.key;
+            ti = wtail < kHeapLds ? lii : g.idx;
+        }
+        BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
+        // levels in LDS: children of nodes 1..7 are entries 2..15
+        int cur = 1;
+        int mc = -1;
+        double mk = 0.0;
+        int mi = 0;
+        {
+            const int l = 2, r = 3;
             if (r < wtail) {
-                const BHeapEnt cl = hget(l), cr = hget(r);
-                const bool left = cl.key < cr.key;
-                mc = left ? l : r; mk = left ? cl.key : cr.key; mi = left ? cl.idx : cr.idx;
+                const double kl = lkey(l), kr = lkey(r);
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? lidx(l) : lidx(r);
             } else if (l < wtail) {
-                const BHeapEnt cl = hget(l);
-                mc = l; mk = cl.key; mi = cl.idx;
-            } else {
-                mc = -1;
+                mc = l; mk = lkey(l); mi = lidx(l);
             }
-        };
-        pick(1);
+        }
         // The new root is known from the first level: its record is loaded
         // now and arrives while the sift continues (the sift writes only
         // heap entries, never records).
@@ -855,14 +855,22 @@ struct BatchedPolicy {
         rootIdx = rootMoves ? mi : top.idx;
         if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
         else nrecIdx = -1;
-        // levels in LDS (children of nodes 1..7)
         bool deeper = false;
         while (mc != -1 && mk < top.key) {
-            BHeapEnt m; m.key = mk; m.idx = mi; m.pad = 0;
-            hset(cur, m);
+            lk[(cur - 1) * lstride] = mk;
+            li[(cur - 1) * lstride] = mi;
             cur = mc;
             if (cur >= 8) { deeper = true; break; }
-            pick(cur);
+            const int l = 2 * cur, r = l + 1;
+            if (r < wtail) {
+                const double kl = lkey(l), kr = lkey(r);
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? lidx(l) : lidx(r);
+            } else if (l < wtail) {
+                mc = l; mk = lkey(l); mi = lidx(l);
+            } else {
+                mc = -1;
+            }
         }
         // global levels: children of nodes 8..15 root blocks 0..15, children
         // of nodes 64..127 root blocks 16..143
@@ -881,13 +889,24 @@ struct BatchedPolicy {
         ++wtail;
         int cur = wtail - 1, p = cur / 2;
         const double key = w.minDist;
-        while (p > 0) {
-            const BHeapEnt pe = hget_push(p);
-            if (!(pe.key > key)) break;
+        BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
+        // global levels of the path (parents with index >= 16), then LDS ones
+        bool done = false;
+        while (p >= kHeapLds) {
+            const BHeapEnt pe = p == pfI0 ? pfE0 : (p == pfI1 ? pfE1 : gget(p));
+            if (!(pe.key > key)) { done = true; break; }
             hset(cur, pe);
             cur = p; p = cur / 2;
         }
-        BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
+        if (!done) {
+            while (p > 0) {
+                const double pk = lkey(p);
+                if (!(pk > key)) break;
+                BHeapEnt pe; pe.key = pk; pe.idx = lidx(p); pe.pad = 0;
+                hset(cur, pe);
+                cur = p; p = cur / 2;
+            }
+        }
         hset(cur, ne);
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
