@@ -526,7 +526,7 @@ struct BatchedPolicy {
     // window heap
     BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
     double* lk; int* li; int lstride;   // LDS part of the heap (logical 1..15), lane-major
-    int wtail, wftop, wfresh, wcache;
+    int wtail, wftop, wfresh, wcache, wcache2;
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
 
@@ -723,12 +723,18 @@ struct BatchedPolicy {
     }
 
     // --- window heap: PriorityQueue.cuh:51-105 semantics on (key, record) entries
+    // Free record slots form a stack whose top two entries are kept in
+    // registers (wcache, then wcache2, then wfree[0..wftop)); the refill
+    // read after an allocation is needed only two allocations later, so
+    // the two pushes of one propagation never wait on it.  A slot below
+    // the register part exists only while both registers are filled.
     __device__ int walloc()
     {
         int idx;
         if (wcache >= 0) {
             idx = wcache;
-            wcache = wftop > 0 ? (int)wfree[--wftop] : -1;
+            wcache = wcache2;
+            wcache2 = wftop > 0 ? (int)wfree[--wftop] : -1;
         } else {
             idx = wfresh++;
         }
@@ -736,7 +742,8 @@ struct BatchedPolicy {
     }
     __device__ void wrelease(int idx)
     {
-        if (wcache >= 0) wfree[wftop++] = (uint16_t)wcache;
+        if (wcache2 >= 0) wfree[wftop++] = (uint16_t)wcache2;
+        wcache2 = wcache;
         wcache = idx;
     }
     __device__ bool wempty() const { return wtail == 1; }
@@ -1074,7 +1081,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             if (slot >= a.nsrc) break;
             const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
             policy_tables(p, a, slot, src);
-            p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.rootKey = 0.0; p.rootIdx = 0;
+            p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.wcache2 = -1; p.rootKey = 0.0; p.rootIdx = 0;
             p.nrecIdx = -1;
             p.ptail = 1;
             ich.reset(src);
