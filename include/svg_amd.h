@@ -127,6 +127,22 @@ int svg_msad(svg_graph_t g, const int32_t *src, const double *w, int ns, double 
 /* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */
 int svg_sssd_batch(svg_graph_t g, const int32_t *st, int nq, double *out);
 
+/* ------------------------------------------------------- geodesic paths */
+/* The surface polyline of an SVG arc u -> v from the key points its row
+ * stores (SVGNode::nextToSrcEdge/X, nextToDstEdge, SVG.cuh:21-27, written by
+ * BuildGeodesicPathTo ICHDevice.cu:336-480): the straight line through the
+ * unfolded faces, traced as the reference's unused InitialValueGeodesic.cu:
+ * 54-256 sketches.  xyz[3*cap] receives u, every edge crossing and v; *n the
+ * point count (set, with SVG_E_ARG, also when cap is too small); the traced
+ * position on the last key point's edge goes to *dst_x_traced (may be NULL).
+ * Host-side; needs no device. */
+int svg_trace_arc(svg_mesh_t m, int u, int v, int32_t src_edge, double src_x, int32_t dst_edge,
+                  double *xyz, int cap, int *n, double *dst_x_traced);
+/* The surface polyline of a vertex path whose consecutive vertices are SVG
+ * arcs (such as the path svg_sssd returns, read from its end): where the
+ * reference prints the vertex sequence (kernel.cu:75-81). */
+int svg_geodesic_path(svg_graph_t g, const int32_t *verts, int nverts, double *xyz, int cap, int *n);
+
 /* ---------------------------------------------------------------- serving */
 typedef struct {
     int max_inflight;       /* request slots resident on the device */

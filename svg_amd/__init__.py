@@ -70,6 +70,20 @@ class Mesh:
     def edgeNum(self) -> int:
         return self._info()[2]
 
+    def trace_arc(self, u, v, src_edge, src_x, dst_edge):
+        """Polyline (n, 3) of the SVG arc u -> v from its stored key points, and
+        the traced position on the last key point's edge."""
+        n = C.c_int()
+        dx = C.c_double()
+        rc = lib().svg_trace_arc(self.h, int(u), int(v), int(src_edge), float(src_x), int(dst_edge),
+                                 np.empty(3, np.float64), 0, C.byref(n), C.byref(dx))
+        if rc < 0 and n.value <= 0:
+            check(rc)
+        out = np.empty((max(n.value, 1), 3), np.float64)
+        check(lib().svg_trace_arc(self.h, int(u), int(v), int(src_edge), float(src_x), int(dst_edge),
+                                  out.reshape(-1), n.value, C.byref(n), C.byref(dx)))
+        return out[: n.value], dx.value
+
     def export(self):
         nv, nf, ne = self._info()
         twin = np.empty(ne, np.int32)
@@ -206,6 +220,18 @@ class SVG:
         par = np.empty(V, np.int32) if want_parent else None
         check(lib().svg_msad(self.g, src, w, len(src), dist, _ptr(sid), _ptr(par)))
         return dist, sid, par
+
+    def GeodesicPath(self, verts):
+        """Surface polyline (n, 3) of a vertex path of SVG arcs, e.g. the
+        reversed path of SolveSSSD(s, t, path=True)."""
+        v = np.ascontiguousarray(verts, dtype=np.int32)
+        n = C.c_int()
+        rc = lib().svg_geodesic_path(self.g, v, len(v), np.empty(3, np.float64), 0, C.byref(n))
+        if rc < 0 and n.value <= 0:
+            check(rc)
+        out = np.empty((max(n.value, 1), 3), np.float64)
+        check(lib().svg_geodesic_path(self.g, v, len(v), out.reshape(-1), n.value, C.byref(n)))
+        return out[: n.value]
 
     def SolveSSSDBatch(self, pairs):
         st = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1))

@@ -66,6 +66,7 @@ COUNTER_NAMES = ["win_push", "win_pop", "pseudo_push", "pseudo_pop", "propagate"
 # Symbols the C ABI exports (include/svg_amd.h), used by the load check.
 EXPORTS = [
     "svg_init", "svg_last_error", "svg_device_name", "svg_mesh_from_arrays", "svg_mesh_load_obj",
+    "svg_trace_arc", "svg_geodesic_path",
     "svg_mesh_upload", "svg_mesh_info", "svg_mesh_export", "svg_mesh_free",
     "svg_build_params_default", "svg_build", "svg_build_stats_get", "svg_build_status",
     "svg_graph_info", "svg_graph_to_host", "svg_graph_save", "svg_graph_load", "svg_graph_free",
@@ -113,6 +114,9 @@ def lib():
                            C.POINTER(C.c_int)]
     L.svg_msad.argtypes = [_vp, _ip, _dp, C.c_int, _dp, _vp, _vp]
     L.svg_sssd_batch.argtypes = [_vp, _ip, C.c_int, _dp]
+    L.svg_trace_arc.argtypes = [_vp, C.c_int, C.c_int, C.c_int32, C.c_double, C.c_int32, _dp, C.c_int,
+                                C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.svg_geodesic_path.argtypes = [_vp, _ip, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)]
     if hasattr(L, "svg_server_start"):
         L.svg_server_start.argtypes = [_vp, C.POINTER(ServerCfg), C.POINTER(_vp)]
         L.svg_server_submit.argtypes = [_vp, C.POINTER(Request)]

@@ -254,6 +254,62 @@ extern "C" int svg_graph_to_host(svg_graph_t g, int32_t* tails, int32_t* adj, do
     return SVG_OK;
 }
 
+static int copy_points(const std::vector<double>& pts, double* xyz, int cap, int* n)
+{
+    const int np = (int)(pts.size() / 3);
+    if (n) *n = np;
+    if (np > cap) return svg_set_error(SVG_E_ARG, "output holds " + std::to_string(cap) + " points, " +
+                                                     std::to_string(np) + " needed");
+    if (xyz) std::memcpy(xyz, pts.data(), sizeof(double) * pts.size());
+    return SVG_OK;
+}
+
+extern "C" int svg_trace_arc(svg_mesh_t m, int u, int v, int32_t src_edge, double src_x, int32_t dst_edge,
+                             double* xyz, int cap, int* n, double* dst_x_traced)
+{
+    if (!m) return svg_set_error(SVG_E_ARG, "null mesh");
+    const svg::HostMesh& h = m->h;
+    if (u < 0 || u >= h.nv) return svg_set_error(SVG_E_ARG, "vertex out of range");
+    std::vector<double> pts(h.xyz.begin() + 3 * u, h.xyz.begin() + 3 * u + 3);
+    std::string e = svg::trace_arc(h, u, v, src_edge, src_x, dst_edge, h.nf, pts, dst_x_traced);
+    if (!e.empty()) return svg_set_error(SVG_E_ARG, "arc " + std::to_string(u) + "->" + std::to_string(v) + ": " + e);
+    return copy_points(pts, xyz, cap, n);
+}
+
+extern "C" int svg_geodesic_path(svg_graph_t g, const int32_t* verts, int nverts, double* xyz, int cap, int* n)
+{
+    if (!g || !verts || nverts < 1) return svg_set_error(SVG_E_ARG, "null graph or empty path");
+    int rc = svg_ensure_device();
+    if (rc) return rc;
+    const svg::HostMesh& h = g->mesh->h;
+    const int K = g->d.K;
+    for (int i = 0; i < nverts; ++i)
+        if (verts[i] < 0 || verts[i] >= h.nv) return svg_set_error(SVG_E_ARG, "vertex out of range");
+    std::vector<double> pts(h.xyz.begin() + 3 * verts[0], h.xyz.begin() + 3 * verts[0] + 3);
+    std::vector<int32_t> adj((size_t)K), se((size_t)K), de((size_t)K);
+    std::vector<double> sx((size_t)K);
+    for (int i = 0; i + 1 < nverts; ++i) {
+        const int u = verts[i], v = verts[i + 1];
+        int32_t tl = 0;
+        const size_t o = (size_t)u * (size_t)K;
+        hipError_t he = hipMemcpy(&tl, g->d.tail + u, sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (he == hipSuccess && tl > 0) he = hipMemcpy(adj.data(), g->d.adj + o, sizeof(int32_t) * (size_t)tl, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("row read: ") + hipGetErrorString(he));
+        int j = -1;
+        for (int k = 0; k < tl; ++k) if (adj[(size_t)k] == v) { j = k; break; }
+        if (j < 0) return svg_set_error(SVG_E_ARG, "no arc " + std::to_string(u) + "->" + std::to_string(v));
+        int32_t es = -1, ed = -1;
+        double xs = 0.0;
+        he = hipMemcpy(&es, g->d.srcEdge + o + j, sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (he == hipSuccess) he = hipMemcpy(&ed, g->d.dstEdge + o + j, sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (he == hipSuccess) he = hipMemcpy(&xs, g->d.srcX + o + j, sizeof(double), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("row read: ") + hipGetErrorString(he));
+        std::string e = svg::trace_arc(h, u, v, es, xs, ed, h.nf, pts, nullptr);
+        if (!e.empty()) return svg_set_error(SVG_E_ARG, "arc " + std::to_string(u) + "->" + std::to_string(v) + ": " + e);
+    }
+    return copy_points(pts, xyz, cap, n);
+}
+
 // SVGNode file record (SVG.cuh:21-27, 40 bytes with 4 bytes of padding).
 struct FileNode {
     int32_t adjNode;

@@ -62,6 +62,11 @@ struct HostMesh {
 // Builds the half-edge structure; returns an empty string or an error text.
 std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int nf, HostMesh& m);
 std::string load_obj(const char* path, std::vector<double>& xyz, std::vector<int32_t>& tri);
+// Polyline of the SVG arc u -> v from its two stored key points (geodesic.cpp):
+// appends the crossings' xyz and v's (not u's) to pts; dstXTraced (optional)
+// receives the traced position on the last key point's edge.
+std::string trace_arc(const HostMesh& m, int u, int v, int srcEdge, double srcX, int dstEdge, int hopCap,
+                      std::vector<double>& pts, double* dstXTraced);
 
 // Device images of one mesh (all device pointers).
 struct DevMesh {

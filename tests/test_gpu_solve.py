@@ -47,6 +47,19 @@ def test_sssd_path_reproduces_distance(small_graph):
     assert tot == d
 
 
+def test_geodesic_path_polyline(small_graph):
+    # the surface polyline of an A* path: from s to t, as long as the distance
+    m, g, og, h = small_graph
+    xyz = og.xyz.reshape(-1, 3)
+    for s, t in ((5, 400), (17, 611), (300, 2)):
+        d, path = g.SolveSSSD(s, t, path=True)
+        pts = g.GeodesicPath(path[::-1])
+        assert np.array_equal(pts[0], xyz[s]) and np.array_equal(pts[-1], xyz[t])
+        length = np.linalg.norm(np.diff(pts, axis=0), axis=1).sum()
+        assert abs(length - d) <= 1e-9 * d, (s, t, length, d)
+        assert len(pts) >= len(path)
+
+
 @pytest.mark.parametrize("ns", [1, 3, 17])
 def test_msad_matches_oracle_dijkstra(small_graph, ns):
     m, g, og, h = small_graph
