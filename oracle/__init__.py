@@ -71,6 +71,10 @@ def lib():
     L.oracle_sssd_many.restype = C.c_double
     L.oracle_sssd_many.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, _dp, _ip, C.c_int,
                                    C.c_int, _dp]
+    L.oracle_ich_point.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_uint, C.c_uint, _dp,
+                                   _ip, _lp]
+    L.oracle_sssd_point.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_uint, C.c_int, _ip, _dp,
+                                    _ip, C.c_int, _dp, C.c_int, _dp, _dp, _ip, _dp, _ip, _lp]
     L.oracle_selftest_heap.argtypes = [_dp, C.c_int, _ip]
     L.oracle_selftest_hash.argtypes = [C.c_uint, _up, C.c_int, _ip]
     L.oracle_acos_many.argtypes = [_dp, _dp, C.c_int]
@@ -133,6 +137,44 @@ def ich_single(mesh: Mesh, src: int, K: int = -1, split_coef: int = 10, vert_coe
     cnt = np.zeros(len(COUNTERS), np.int64)
     lib().oracle_ich_single(mesh.h, src, K, split_coef, vert_coef, dist, enter, cnt)
     return dist, enter, dict(zip(COUNTERS, cnt.tolist()))
+
+
+def ich_point(mesh: Mesh, face: int, p, K: int = -1, split_coef: int = 10, vert_coef: int = 2):
+    """Distances / enter edges of one ICH run from point p of `face`."""
+    dist = np.empty(mesh.nv, np.float64)
+    enter = np.empty(mesh.nv, np.int32)
+    cnt = np.zeros(len(COUNTERS), np.int64)
+    pp = np.ascontiguousarray(p, dtype=np.float64).ravel()
+    rc = lib().oracle_ich_point(mesh.h, face, pp, K, split_coef, vert_coef, dist, enter, cnt)
+    if rc:
+        raise ValueError("bad face")
+    return dist, enter, dict(zip(COUNTERS, cnt.tolist()))
+
+
+def sssd_point(mesh: Mesh, rows, tails, K: int, f0: int, p0, f1: int, p1,
+               split_coef: int = 10, vert_coef: int = 2):
+    """SVG::SolveSSSD(f0, p0, f1, p1) on the graph (rows, tails) built with K.
+
+    Returns a dict: dist, src_edge, src_x, dst_edge, dst_x, last_vert, route
+    (0 direct, 1 through the graph, -1 unreachable), gdist, gparent."""
+    V = mesh.nv
+    adj = np.ascontiguousarray(rows["adj"], dtype=np.int32).ravel()
+    w = np.ascontiguousarray(rows["geodDist"], dtype=np.float64).ravel()
+    tl = np.ascontiguousarray(tails, dtype=np.int32)
+    dout = np.zeros(3, np.float64)
+    iout = np.zeros(4, np.int32)
+    gdist = np.empty(V, np.float64)
+    gpar = np.empty(V, np.int32)
+    stat = np.zeros(3, np.int64)
+    rc = lib().oracle_sssd_point(mesh.h, K, split_coef, vert_coef, rows.shape[1], adj, w, tl,
+                                 f0, np.ascontiguousarray(p0, dtype=np.float64).ravel(), f1,
+                                 np.ascontiguousarray(p1, dtype=np.float64).ravel(), dout, iout,
+                                 gdist, gpar, stat)
+    if rc:
+        raise ValueError("bad argument")
+    return dict(dist=float(dout[0]), src_x=float(dout[1]), dst_x=float(dout[2]),
+                src_edge=int(iout[0]), dst_edge=int(iout[1]), last_vert=int(iout[2]),
+                route=int(iout[3]), gdist=gdist, gparent=gpar)
 
 
 class Graph:

@@ -60,6 +60,7 @@
 #define O_WINPQ_SIZE 1024                  /* SVG.cuh:13 */
 #define O_PSEUDOWINPQ_SIZE 1024            /* SVG.cuh:14 */
 #define O_NONE 0xFFFFFFFFu
+#define O_STORED_WIN 100                   /* STORED_WIN_BUF_SIZE (SVG.cuh:15), Q2 */
 
 /* ------------------------------------------------------------------------ */
 /* acos, fdlibm method (D4)                                                  */
@@ -409,6 +410,12 @@ typedef struct {
     int pathPassVert;
     long long *cnt;
     long hopCap;
+    /* surface-point source (ICHDevice.cuh:148-151) */
+    int srcFace;                      /* sourcePointFace, -1: vertex source */
+    double spos[3];                   /* sourcePointPos */
+    int keptFace;                     /* keptFaces[0], -1: none */
+    OWindow stored[O_STORED_WIN];     /* storedWindows */
+    int nStored;
 } OIch;
 
 static void ich_init(OIch *s, const OMesh *m, int K, unsigned splitCoef, unsigned vertCoef)
@@ -453,6 +460,9 @@ static void ich_clear(OIch *s)
     }
     s->split.overflowed = 0; s->vert.overflowed = 0;
     s->sourceVert = O_NONE;
+    s->srcFace = -1;
+    s->keptFace = -1;
+    s->nStored = 0;
     s->totalCalcVertNum = 0;
 }
 
@@ -990,7 +1000,15 @@ static void build_path_to(OIch *s, unsigned vertId, OPath *out)
             double lpy = -sqrt(fabs(l1 * l1 - lpx * lpx));
             double cpx = l0 - gpos, cpy = 0.0;
             unsigned ps = split_r(&s->split, (unsigned)enter)->pseudoSrcId;
-            while (ps < (unsigned)V && opVert != ps) {
+            if (ps >= (unsigned)V && s->srcFace < 0) {
+                /* surface-point pseudo-source in a vertex-source run */
+                s->pathPassVert = 1; s->cnt[C_ERRORS]++;
+                return;
+            }
+            /* unfold towards the pseudo-source vertex, or (surface-point
+             * source, ICHDevice.cu:415-418) until the face behind the key
+             * point is the source's face */
+            while (ps < (unsigned)V ? opVert != ps : m->e[tw].face != s->srcFace) {
                 if (++hops > s->hopCap) { s->pathPassVert = 1; s->cnt[C_ERRORS]++; return; }
                 s->cnt[C_BT_HOPS]++;
                 int err = 0;
@@ -1002,10 +1020,8 @@ static void build_path_to(OIch *s, unsigned vertId, OPath *out)
                 opVert = (unsigned)m->e[m->e[tw].next].v1;
             }
             if (ps >= (unsigned)V) {
-                /* surface-point pseudo-source: never reached for vertex sources */
-                curVert = ps;
-                s->pathPassVert = 1; s->cnt[C_ERRORS]++;
-                out->srcId = curVert;
+                /* reached the source point's face (ICHDevice.cu:465-468) */
+                out->srcId = ps;
                 return;
             }
             if (!vert_r(&s->vert, opVert)->isSource) {
@@ -1016,6 +1032,198 @@ static void build_path_to(OIch *s, unsigned vertId, OPath *out)
         }
     }
     out->srcId = curVert;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Surface-point sources (ICHDevice.cu:56-65, 67-132, 143-334, 583-615).     */
+/* ------------------------------------------------------------------------ */
+static double o_vdist(const OMesh *m, const double *p, unsigned v)
+{
+    return o_len3(p[0], p[1], p[2], m->v[v].x, m->v[v].y, m->v[v].z);
+}
+
+/* ICHDevice::Initialize, surface-point branch (ICHDevice.cu:583-615): one
+ * window over each edge of the source face, pseudo-source id V (never stale),
+ * and the face's vertices at their straight-line distances. */
+static void ich_initialize_point(OIch *s)
+{
+    const OMesh *m = s->mesh;
+    const unsigned V = (unsigned)m->nv;
+    const double *p = s->spos;
+    for (int j = 0; j < 3; ++j) {
+        unsigned opEdge = (unsigned)(3 * s->srcFace + j);      /* faces[f].edges[j] */
+        OWindow w;
+        memset(&w, 0, sizeof(w));
+        w.edgeID = opEdge;
+        w.b0 = 0.0; w.b1 = m->e[opEdge].len;
+        w.d0 = o_vdist(m, p, (unsigned)m->e[opEdge].v0);
+        w.d1 = o_vdist(m, p, (unsigned)m->e[opEdge].v1);
+        w.pseudoSrcDist = 0.0; win_calc_min(&w);
+        w.srcID = V; w.pseudoSrcId = V;
+        w.pseudoSrcBirthTime = 0; w.level = 0;
+        push_win(s, &w);
+
+        unsigned opVert = (unsigned)m->e[opEdge].v0;
+        if (o_vdist(m, p, opVert) > vert_w(&s->vert, opVert)->dist) continue;
+        OVInfo *vi = vert_w(&s->vert, opVert);
+        vi->birthTime = 0;
+        vi->dist = o_vdist(m, p, opVert);
+        vi->enterEdge = -1;
+        vi->srcId = V; vi->pseudoSrcId = V;
+        if (m->v[opVert].not_saddle) continue;
+        OPseudo q;
+        q.vertID = opVert;
+        q.dist = o_len3(m->v[opVert].x, m->v[opVert].y, m->v[opVert].z, p[0], p[1], p[2]);
+        q.srcId = V; q.pseudoSrcId = V;
+        q.pseudoBirthTime = (unsigned)(int)vert_r(&s->vert, opVert)->birthTime;
+        q.level = 0;
+        push_pseudo(s, &q);
+    }
+}
+
+/* ICHDevice::Execute (ICHDevice.cu:67-132) for a surface-point source, with
+ * the windows that cross into the kept face stored as they are popped
+ * (ICHDevice.cu:93-107). */
+static void ich_execute_point(OIch *s, int K)
+{
+    const OMesh *m = s->mesh;
+    const unsigned V = (unsigned)m->nv;
+    ich_initialize_point(s);
+    while (s->winQ.tail != 1 || s->pseudoQ.tail != 1) {
+        s->cnt[C_ITER]++;
+        while (s->winQ.tail != 1 &&
+               s->winQ.a[1].item.pseudoSrcId < V &&
+               s->winQ.a[1].item.pseudoSrcBirthTime !=
+                   (int)vert_r(&s->vert, s->winQ.a[1].item.pseudoSrcId)->birthTime) {
+            OWinQ_pop(&s->winQ);
+            s->cnt[C_WIN_POP]++; s->cnt[C_WIN_STALE]++;
+        }
+        /* Q3: the stale test of the pseudo-source queue is guarded by the
+         * window queue's top (ICHDevice.cu:84), i.e. slot 1 of its array,
+         * which keeps the last window popped once the queue is empty.  While
+         * that window descends from the point (pseudo-source id V) stale
+         * pseudo-sources are not skipped and are expanded. */
+        while (s->pseudoQ.tail != 1 &&
+               s->winQ.a[1].item.pseudoSrcId < V &&
+               s->pseudoQ.a[1].item.pseudoBirthTime !=
+                   (unsigned)(int)vert_r(&s->vert, s->pseudoQ.a[1].item.vertID)->birthTime) {
+            OPseudoQ_pop(&s->pseudoQ);
+            s->cnt[C_PSEUDO_POP]++; s->cnt[C_PSEUDO_STALE]++;
+        }
+        int wEmpty = s->winQ.tail == 1, pEmpty = s->pseudoQ.tail == 1;
+        if (!wEmpty && (pEmpty || s->winQ.a[1].item.minDist < s->pseudoQ.a[1].item.dist)) {
+            OWindow win = s->winQ.a[1].item;
+            OWinQ_pop(&s->winQ);
+            s->cnt[C_WIN_POP]++;
+            if (win.level > m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
+            int tw = m->e[win.edgeID].twin;
+            if (tw != -1 && s->keptFace >= 0 && m->e[tw].face == s->keptFace &&
+                s->nStored < O_STORED_WIN)
+                s->stored[s->nStored++] = win;
+            propagate_window(s, &win);
+        } else if (!pEmpty && (wEmpty || s->winQ.a[1].item.minDist >= s->pseudoQ.a[1].item.dist)) {
+            OPseudo pw = s->pseudoQ.a[1].item;
+            OPseudoQ_pop(&s->pseudoQ);
+            s->cnt[C_PSEUDO_POP]++;
+            if (pw.level >= (unsigned)m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
+            gen_sub_wins(s, &pw);
+        }
+        if (K != -1 && s->totalCalcVertNum >= K) break;
+    }
+}
+
+/* ICHDevice::BuildGeodesicPathTo(faceId, pos, ...) (ICHDevice.cu:143-334):
+ * the distance to a point of the kept face through a stored window or one of
+ * the face's vertices, and the key points of that path.  out's X fields keep
+ * what the caller put there unless a key point is written. */
+static double face_path_to(OIch *s, int f, const double *pos, OPath *out)
+{
+    const OMesh *m = s->mesh;
+    const unsigned V = (unsigned)m->nv;
+    s->pathPassVert = 0;
+    out->srcEdge = -1; out->dstEdge = -1;
+    double minDist = DBL_MAX, xInter = 0.0, posx = 0.0, posy = 0.0;
+    OWindow minWin;
+    memset(&minWin, 0, sizeof(minWin));
+    int dstVert = -1, throughWin = 1;
+    for (int i = 0; i < s->nStored; ++i) {
+        const OWindow *w = &s->stored[i];
+        int tw = m->e[w->edgeID].twin;
+        if (tw == -1) continue;
+        if (m->e[tw].face != f) continue;
+        int e0 = tw, e1 = m->e[e0].next, e2 = m->e[e1].next;
+        double l0 = m->e[e0].len, l1 = m->e[e1].len, l2 = m->e[e2].len;
+        unsigned v0 = (unsigned)m->e[e0].v1, v1 = (unsigned)m->e[e0].v0, v2 = (unsigned)m->e[e1].v1;
+        double p0x = 0.0, p0y = 0.0, p1x = l0, p1y = 0.0;
+        double p2x = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0);
+        double p2y = -sqrt(fabs(l1 * l1 - p2x * p2x));
+        double sx, sy;
+        win_flat_src(w, &sx, &sy);
+        /* the point's barycentric weights from Heron's formula */
+        double a = o_vdist(m, pos, v0), b = o_vdist(m, pos, v1), c = o_vdist(m, pos, v2);
+        double s0 = (b + c + l2) / 2.0;
+        double s1 = (a + c + l1) / 2.0;
+        double s2 = (a + b + l0) / 2.0;
+        s0 = sqrt(fabs(s0 * (s0 - b) * (s0 - c) * (s0 - l2)));
+        s1 = sqrt(fabs(s1 * (s1 - a) * (s1 - c) * (s1 - l1)));
+        s2 = sqrt(fabs(s2 * (s2 - a) * (s2 - b) * (s2 - l0)));
+        double w0 = s0 / (s0 + s1 + s2);
+        double w1 = s1 / (s0 + s1 + s2);
+        double w2 = s2 / (s0 + s1 + s2);
+        double cx = w0 * p0x + w1 * p1x + w2 * p2x;
+        double cy = w0 * p0y + w1 * p1y + w2 * p2y;
+        double xi = sx - (cx - sx) / (cy - sy) * sy;
+        if (xi <= 0.0 || xi >= l0) continue;
+        double cur;
+        if (xi > w->b0 && xi < w->b1)
+            cur = o_len2(cx - sx, cy - sy) + w->pseudoSrcDist;
+        else if (xi <= w->b0)
+            cur = o_len2(cx - w->b0, cy - 0.0) + w->d0 + w->pseudoSrcDist;
+        else
+            cur = o_len2(cx - w->b1, cy - 0.0) + w->d1 + w->pseudoSrcDist;
+        if (cur < minDist) { minDist = cur; minWin = *w; xInter = xi; posx = cx; posy = cy; }
+    }
+    for (int i = 0; i < 3; ++i) {
+        unsigned ov = (unsigned)m->e[3 * f + i].v0;
+        double cd = o_vdist(m, pos, ov) + vert_r(&s->vert, ov)->dist;
+        if (cd < minDist) { throughWin = 0; dstVert = (int)ov; minDist = cd; }
+    }
+    if (minDist == DBL_MAX) return minDist;
+    if (!throughWin) {
+        build_path_to(s, (unsigned)dstVert, out);
+        s->pathPassVert = 1;
+        out->dstEdge = m->v[dstVert].firstEdge; out->dstX = 0.0;
+        return minDist;
+    }
+    int gid = m->e[minWin.edgeID].twin;
+    double gpos = m->e[gid].len - xInter;
+    out->dstEdge = gid; out->dstX = gpos;
+    int tw = m->e[gid].twin;
+    unsigned opVert = (unsigned)m->e[m->e[tw].next].v1;
+    double lpx = posx, lpy = posy, cpx = m->e[gid].len - gpos, cpy = 0.0;
+    unsigned ps = minWin.pseudoSrcId;
+    long hops = 0;
+    while (ps < V ? opVert != ps : m->e[tw].face != s->srcFace) {
+        if (++hops > s->hopCap) { s->pathPassVert = 1; s->cnt[C_ERRORS]++; return minDist; }
+        s->cnt[C_BT_HOPS]++;
+        int err = 0;
+        unfold_step(m, &gid, &gpos, &lpx, &lpy, &cpx, &cpy, &err);
+        if (err) { s->pathPassVert = 1; s->cnt[C_ERRORS]++; return minDist; }
+        out->srcEdge = gid; out->srcX = gpos;
+        tw = m->e[gid].twin;
+        if (tw == -1) { s->pathPassVert = 1; s->cnt[C_ERRORS]++; return minDist; }
+        opVert = (unsigned)m->e[m->e[tw].next].v1;
+    }
+    if (ps < V && !vert_r(&s->vert, opVert)->isSource) {
+        /* the path bends at the pseudo-source: the source-side key point is
+         * the one next to the source on that vertex's own path */
+        s->pathPassVert = 1;
+        OPath sub;
+        build_path_to(s, opVert, &sub);
+        out->srcEdge = sub.srcEdge; out->srcX = sub.srcX;
+    }
+    out->srcId = minWin.srcID;
+    return minDist;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1284,6 +1492,154 @@ int oracle_msad(int V, int K, const int *adj, const double *w, const int *tails,
     }
     o_astar(&g, -1, 0, dist, parent, srcId, idx, &q, stat);
     free(q.a); free(idx);
+    return 0;
+}
+
+/* Single ICH run from a point of face f (K = -1: to the end): distances
+ * (DBL_MAX when unreached) and enter edges of every vertex. */
+int oracle_ich_point(const OMesh *m, int f, const double *p, int K, unsigned splitCoef,
+                     unsigned vertCoef, double *dist, int *enterEdge, long long *counters)
+{
+    if (f < 0 || f >= m->nf) return -1;
+    OIch *s = (OIch *)malloc(sizeof(OIch));
+    if (!s) return -1;
+    int tabK = K > 0 ? K : m->nv;
+    ich_init(s, m, tabK, splitCoef, vertCoef);
+    long long local[C_NCOUNTERS];
+    s->cnt = counters ? counters : local;
+    memset(s->cnt, 0, sizeof(long long) * C_NCOUNTERS);
+    ich_clear(s);
+    s->srcFace = f;
+    memcpy(s->spos, p, sizeof(s->spos));
+    ich_execute_point(s, K);
+    s->cnt[C_TOTAL_CALC] = s->totalCalcVertNum;
+    s->cnt[C_MAX_WINQ] = s->winQ.maxLen;
+    s->cnt[C_MAX_PSEUDOQ] = s->pseudoQ.maxLen;
+    for (int j = 0; j < m->nv; ++j) {
+        const OVInfo *vi = vert_r(&s->vert, (unsigned)j);
+        dist[j] = vi->dist;
+        enterEdge[j] = vi->enterEdge;
+    }
+    ich_destroy(s);
+    free(s);
+    return 0;
+}
+
+/*
+ * SVG::SolveSSSD(f0, p0, f1, p1) (SVG.cu:253-352): geodesic distance between
+ * two surface points.  An ICH from p0 keeps the windows crossing into f1; if
+ * the path through them (or through a vertex of f1) records a source-side
+ * key point, that distance is the answer (route 0).  Otherwise a Dijkstra on
+ * the graph seeded with every vertex the p0 ICH reached, at its ICH distance,
+ * is joined with an ICH from p1 at the vertex minimising the sum (route 1).
+ *
+ * dout[3] = {geodDist, nextToSrcX, nextToDstX};
+ * iout[4] = {nextToSrcEdge, nextToDstEdge, saddleVertNearDst, route};
+ * gdist / gparent (V, may be NULL) receive the graph stage's distances and
+ * parents (route 1).  route -1: no vertex joins the two (distance DBL_MAX).
+ *
+ * Decisions where the reference is undefined: SVGNode X fields start at 0.0;
+ * a key-point edge without a twin is kept as it is when the reference would
+ * take its twin; no join vertex gives route -1 (the reference backtraces
+ * vertex 0xFFFFFFFF).
+ */
+int oracle_sssd_point(const OMesh *m, int K, unsigned splitCoef, unsigned vertCoef,
+                      int Kg, const int *adj, const double *w, const int *tails,
+                      int f0, const double *p0, int f1, const double *p1,
+                      double *dout, int *iout, double *gdist, int *gparent, long long *stat)
+{
+    const int V = m->nv;
+    if (f0 < 0 || f0 >= m->nf || f1 < 0 || f1 >= m->nf || K <= 0) return -1;
+    OIch *a = (OIch *)malloc(sizeof(OIch));
+    OIch *b = (OIch *)malloc(sizeof(OIch));
+    double *gd = gdist ? gdist : (double *)malloc(sizeof(double) * (size_t)V);
+    int *gp = gparent ? gparent : (int *)malloc(sizeof(int) * (size_t)V);
+    int *sid = (int *)malloc(sizeof(int) * (size_t)V);
+    OHeap q; int *idx = NULL;
+    if (!a || !b || !gd || !gp || !sid || o_solver_alloc(V, &q, &idx)) return -1;
+    long long cnt[C_NCOUNTERS];
+    ich_init(a, m, K, splitCoef, vertCoef);
+    ich_init(b, m, K, splitCoef, vertCoef);
+    a->cnt = cnt; b->cnt = cnt;
+    memset(cnt, 0, sizeof(cnt));
+
+    double resD = 0.0, resSX = 0.0, resDX = 0.0;
+    int resSE = -1, resDE = -1, last = -1, route = 0;
+
+    ich_clear(a);
+    a->srcFace = f0; memcpy(a->spos, p0, sizeof(a->spos));
+    a->keptFace = f1;
+    ich_execute_point(a, K);
+    OPath fp;
+    fp.srcX = resSX; fp.dstX = resDX; fp.srcEdge = fp.dstEdge = -1; fp.srcId = 0;
+    resD = face_path_to(a, f1, p1, &fp);
+    resSE = fp.srcEdge; resSX = fp.srcX; resDE = fp.dstEdge; resDX = fp.dstX;
+
+    for (int i = 0; i < V; ++i) { gd[i] = DBL_MAX; gp[i] = -1; sid[i] = -1; idx[i] = -1; }
+    if (resSE == -1) {
+        route = 1;
+        /* seeds in the p0 table's slot order (SVG.cu:283-291) */
+        for (unsigned i = 0; i < a->vert.size; ++i) {
+            int vi = a->vert.data[i].index;
+            if (a->vert.data[i].item.dist == DBL_MAX) continue;
+            gd[vi] = a->vert.data[i].item.dist;
+            gp[vi] = -1;
+            sid[vi] = vi;
+            oh_push(&q, vi, &idx[vi], a->vert.data[i].item.dist);
+        }
+        OGraph g = {V, Kg, adj, w, tails, NULL};
+        o_astar(&g, -1, 0, gd, gp, sid, idx, &q, stat);
+
+        ich_clear(b);
+        b->srcFace = f1; memcpy(b->spos, p1, sizeof(b->spos));
+        ich_execute_point(b, K);
+        double minDist = DBL_MAX;
+        for (unsigned i = 0; i < b->vert.size; ++i) {
+            int vi = b->vert.data[i].index;
+            if (b->vert.data[i].item.dist == DBL_MAX) continue;
+            if (gd[vi] + b->vert.data[i].item.dist >= minDist) continue;
+            minDist = gd[vi] + b->vert.data[i].item.dist;
+            last = vi;
+        }
+        resD = minDist;
+        if (last < 0) {
+            route = -1;
+            resSE = resDE = -1; resSX = resDX = 0.0;
+        } else {
+            /* the p1 side: its source-side key point becomes the result's
+             * destination-side one (SVG.cu:309-318) */
+            OPath pb;
+            build_path_to(b, (unsigned)last, &pb);
+            int nDE = pb.srcEdge;
+            double nDX = pb.srcX;
+            resSE = pb.dstEdge; resSX = pb.dstX;
+            if (nDE == -1) { nDE = resSE; nDX = resDX; }   /* reads nextToDstX, as SVG.cu:314 does */
+            if (nDE != -1 && m->e[nDE].twin != -1) {
+                nDE = m->e[nDE].twin;
+                nDX = m->e[nDE].len - nDX;
+            }
+            /* the p0 side from the first vertex of the graph path (SVG.cu:320-351);
+             * the p0 run is the one above (a second run is identical) */
+            int first = last;
+            while (gp[first] != -1) first = gp[first];
+            if (vert_r(&a->vert, (unsigned)first)->dist != DBL_MAX) {
+                OPath pa;
+                build_path_to(a, (unsigned)first, &pa);
+                int nSE = pa.srcEdge;
+                double nSX = pa.srcX;
+                resDE = pa.dstEdge; resDX = pa.dstX;
+                if (nSE == -1) { nSE = resDE; nSX = resDX; }
+                resSE = nSE; resSX = nSX;
+                resDE = nDE; resDX = nDX;
+            }
+        }
+    }
+    dout[0] = resD; dout[1] = resSX; dout[2] = resDX;
+    iout[0] = resSE; iout[1] = resDE; iout[2] = last; iout[3] = route;
+    ich_destroy(a); ich_destroy(b); free(a); free(b);
+    if (!gdist) free(gd);
+    if (!gparent) free(gp);
+    free(sid); free(q.a); free(idx);
     return 0;
 }
 
