@@ -124,6 +124,20 @@ int svg_sssd(svg_graph_t g, int s, int t, double *dist_t, int32_t *path, int pat
  * weights; dist[V], src_id[V], parent[V] (src_id / parent may be NULL). */
 int svg_msad(svg_graph_t g, const int32_t *src, const double *w, int ns, double *dist,
              int32_t *src_id, int32_t *parent);
+/* SVG::SolveSSSD(f0, p0, f1, p1) (SVG.cu:253-352): geodesic distance
+ * between point p0[3] on face f0 and point p1[3] on face f1 (coordinates in
+ * the mesh's frame).  Bounded ICH runs from both points (K of the graph,
+ * tables K*10 / K*2 as kernel.cu:188-191) answer directly through f1's
+ * windows when that path records a source-side key point; otherwise the
+ * graph joins them at the vertex minimising graph distance + p1 distance.
+ * key_edge[2] / key_x[2] (may be NULL) receive nextToSrcEdge/X and
+ * nextToDstEdge/X; *last_vert the last graph vertex (saddleVertNearDst, -1
+ * when direct); *route 0 direct, 1 through the graph, -1 nothing joins the
+ * points (*dist = DBL_MAX).  path (optional, cap entries) receives the graph
+ * vertices from last_vert back to the first. */
+int svg_sssd_point(svg_graph_t g, int f0, const double *p0, int f1, const double *p1, double *dist,
+                   int32_t *key_edge, double *key_x, int32_t *last_vert, int *route, int32_t *path,
+                   int path_cap, int *path_len);
 /* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */
 int svg_sssd_batch(svg_graph_t g, const int32_t *st, int nq, double *out);
 

@@ -8,13 +8,26 @@ unchanged; every method runs the HIP library through the C ABI
 from __future__ import annotations
 
 import ctypes as C
+from typing import NamedTuple, Optional
 
 import numpy as np
 
 from ._lib import (SVG_MODE_BATCHED, SVG_MODE_PLAIN, BuildParams, BuildStats, COUNTER_NAMES,
                    SvgError, _ptr, check, lib)
 
-__all__ = ["Mesh", "SVG", "SvgError", "SVG_MODE_PLAIN", "SVG_MODE_BATCHED", "device_name"]
+__all__ = ["Mesh", "SVG", "SvgError", "SVG_MODE_PLAIN", "SVG_MODE_BATCHED", "PointResult", "device_name"]
+
+
+class PointResult(NamedTuple):
+    """SVGNode fields of a surface-point query (SVG.cu:253-352)."""
+    dist: float
+    src_edge: int
+    src_x: float
+    dst_edge: int
+    dst_x: float
+    last_vert: int
+    route: int
+    path: Optional[np.ndarray]
 
 
 def device_name() -> str:
@@ -200,7 +213,13 @@ class SVG:
         self.g = g
 
     # --------------------------------------------------------------- queries
-    def SolveSSSD(self, s: int, t: int, path: bool = False):  # SVG.cu:241-251
+    def SolveSSSD(self, *args, path: bool = False):
+        """SolveSSSD(s, t): A* between vertices (SVG.cu:241-251); with
+        path=True also the vertex path t -> s.  SolveSSSD(f0, p0, f1, p1):
+        between surface points (SVG.cu:253-352), see SolveSSSDPoints."""
+        if len(args) == 4:
+            return self.SolveSSSDPoints(*args, path=path)
+        s, t = args
         d = C.c_double()
         if path:
             V, _, _ = self.info()
@@ -210,6 +229,29 @@ class SVG:
             return d.value, buf[: n.value].copy()
         check(lib().svg_sssd(self.g, int(s), int(t), C.byref(d), None, 0, None))
         return d.value
+
+    def SolveSSSDPoints(self, f0: int, p0, f1: int, p1, path: bool = False):   # SVG.cu:253-352
+        """Geodesic distance between point p0 of face f0 and point p1 of face f1.
+
+        Returns a PointResult: dist, the key points next to each end
+        (src_edge, src_x, dst_edge, dst_x: half-edge and distance from its
+        verts[0]), last_vert (saddleVertNearDst, -1 when answered directly),
+        route (0 direct, 1 through the graph, -1 nothing joins the points) and,
+        with path=True, the graph vertices from last_vert back to the first."""
+        a = np.ascontiguousarray(p0, dtype=np.float64).reshape(3)
+        b = np.ascontiguousarray(p1, dtype=np.float64).reshape(3)
+        d = C.c_double()
+        ke = np.empty(2, np.int32)
+        kx = np.empty(2, np.float64)
+        lv = C.c_int32()
+        rt = C.c_int()
+        n = C.c_int()
+        V, _, _ = self.info()
+        buf = np.empty(V if path else 1, np.int32)
+        check(lib().svg_sssd_point(self.g, int(f0), a, int(f1), b, C.byref(d), ke, kx, C.byref(lv),
+                                   C.byref(rt), _ptr(buf), V if path else 0, C.byref(n)))
+        return PointResult(d.value, int(ke[0]), float(kx[0]), int(ke[1]), float(kx[1]), lv.value, rt.value,
+                           buf[: n.value].copy() if path else None)
 
     def SolveMSAD(self, sources, weights, want_parent: bool = True):   # SVG.cu:354-367
         V, _, _ = self.info()

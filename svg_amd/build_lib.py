@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsvg_amd.so")
 
-SOURCES = ["svg_mesh.cpp", "geodesic.cpp", "capi.cpp", "build.hip", "solve.hip", "serve.hip"]
+SOURCES = ["svg_mesh.cpp", "geodesic.cpp", "capi.cpp", "build.hip", "solve.hip", "serve.hip", "point.hip"]
 ARCH = os.environ.get("SVG_OFFLOAD_ARCH", "gfx950")
 
 

@@ -123,8 +123,9 @@ struct Ich {
     int total;              // totalCalcVertNum
     int K;
     bool pathPassVert;
+    int srcFace;            // sourcePointFace: -1 for a vertex source
 
-    SVG_HD Ich(P& pol, int src, int K_) : p(pol), source(src), total(0), K(K_), pathPassVert(false) {}
+    SVG_HD Ich(P& pol, int src, int K_) : p(pol), source(src), total(0), K(K_), pathPassVert(false), srcFace(-1) {}
 
     SVG_HD void push_win(const Win& w) { p.count(CNT_WIN_PUSH, 1); p.wpush(w); }
     SVG_HD void push_pseudo(const Pseudo& q) { p.count(CNT_PSEUDO_PUSH, 1); p.ppush(q); }
@@ -626,7 +627,7 @@ struct Ich {
                 u.gpos = sp.x;
                 if (dstEdge == -1) { dstEdge = u.gid; dstX = u.gpos; }
                 else { srcEdge = u.gid; srcX = u.gpos; }
-                const int tw = p.twin(u.gid);
+                int tw = p.twin(u.gid);
                 if (tw == -1) { p.count(CNT_ERRORS, 1); return true; }
                 int opVert = p.v1(p.next(tw));
                 const double l0 = p.len(u.gid);
@@ -636,16 +637,21 @@ struct Ich {
                 u.lpy = -sqrt(fabs(l1 * l1 - u.lpx * u.lpx));
                 u.cpx = l0 - u.gpos; u.cpy = 0.0;
                 const int ps = sp.ps;
-                if (ps < 0 || ps >= p.nv()) { p.count(CNT_ERRORS, 1); return true; }
-                while (opVert != ps) {
+                // a surface-point source's windows carry pseudo-source id V
+                const bool pointPs = ps >= p.nv() && srcFace >= 0;
+                if (ps < 0 || (ps >= p.nv() && !pointPs)) { p.count(CNT_ERRORS, 1); return true; }
+                // unfold to the pseudo-source vertex, or (point source,
+                // ICHDevice.cu:415-418) until the face behind is the source's
+                while (pointPs ? tw / 3 != srcFace : opVert != ps) {
                     if (++hops > hopCap) { p.count(CNT_ERRORS, 1); return true; }
                     p.count(CNT_BT_HOPS, 1);
                     if (!unfold(u)) { p.count(CNT_ERRORS, 1); return true; }
                     srcEdge = u.gid; srcX = u.gpos;
-                    const int tw2 = p.twin(u.gid);
-                    if (tw2 == -1) { p.count(CNT_ERRORS, 1); return true; }
-                    opVert = p.v1(p.next(tw2));
+                    tw = p.twin(u.gid);
+                    if (tw == -1) { p.count(CNT_ERRORS, 1); return true; }
+                    opVert = p.v1(p.next(tw));
                 }
+                if (pointPs) return pass;                              // ICHDevice.cu:465-468
                 if (!p.vget(opVert).isSource) {
                     srcEdge = p.first_edge(opVert); srcX = 0.0;
                     pass = true;
@@ -655,6 +661,190 @@ struct Ich {
             cv = p.vget(curVert);
         }
         return pass;
+    }
+
+    // ---------------------------------------------------------------------
+    // Surface-point sources.  Half-edge 3f+j is edge j of face f (Mesh.cu:
+    // 69-77), so face(e) = e / 3.
+    // ---------------------------------------------------------------------
+
+    // ICHDevice::Initialize, surface-point branch (ICHDevice.cu:583-615):
+    // a window over each edge of face f with pseudo-source id V (never stale)
+    // and the face's vertices at their straight-line distances d[j] (from the
+    // point to verts[0] of half-edge 3f+j).
+    SVG_HD void initialize_point(int f, const double d[3])
+    {
+        const int nv = p.nv();
+        for (int j = 0; j < 3; ++j) {
+            const int opEdge = 3 * f + j;
+            Win w;
+            w.edge = opEdge;
+            w.b0 = 0.0; w.b1 = p.len(opEdge);
+            w.d0 = d[j];
+            w.d1 = d[j == 2 ? 0 : j + 1];
+            w.ps = 0.0;
+            win_finish(w);
+            w.pseudoSrc = nv; w.birth = 0; w.level = 0;
+            w.twin = p.twin(opEdge);
+            push_win(w);
+            const int opVert = p.v0(opEdge);
+            p.vtouch(opVert);
+            if (d[j] > p.vget(opVert).dist) continue;
+            VInfo vi = p.vget(opVert);
+            vi.birth = 0; vi.dist = d[j]; vi.enter = -1;
+            p.vput(opVert, vi);
+            if (p.vflag(opVert) & 1) continue;                    // angle < 2*PI
+            Pseudo q;
+            q.vert = opVert; q.dist = d[j];
+            q.pseudoSrc = nv;
+            q.birth = p.vget(opVert).birth;
+            q.level = 0;
+            push_pseudo(q);
+        }
+    }
+
+    // ICHDevice::Execute (ICHDevice.cu:67-132) for a surface-point source.
+    // Windows that cross into keptFace are stored as they are popped
+    // (ICHDevice.cu:93-107), at most storeCap of them.  The pseudo-source
+    // queue's stale test is guarded by the window queue's top, slot 1 of its
+    // array, which keeps the last window popped once the queue is empty
+    // (ICHDevice.cu:84): while that window descends from the point, stale
+    // pseudo-sources are expanded rather than skipped.  guardPs tracks that
+    // slot: every pop is of a top read just before.
+    SVG_HD void execute_point(Win* stored, int storeCap, int& nStored, int keptFace)
+    {
+        const int nf = p.nf(), nv = p.nv();
+        int guardPs = nv;
+        while (!p.wempty() || !p.pempty()) {
+            Win top;
+            bool haveTop = false;
+            while (!p.wempty()) {
+                p.wtop(top);
+                guardPs = top.pseudoSrc;
+                if (top.pseudoSrc >= nv || top.birth == p.birth_of(top.pseudoSrc)) { haveTop = true; break; }
+                p.wpop();
+                p.count(CNT_WIN_POP, 1);
+            }
+            Pseudo ptop;
+            bool havePtop = false;
+            while (!p.pempty()) {
+                p.ptop(ptop);
+                if (guardPs >= nv || ptop.birth == p.birth_of(ptop.vert)) { havePtop = true; break; }
+                p.ppop();
+                p.count(CNT_PSEUDO_POP, 1);
+            }
+            if (haveTop && (!havePtop || top.minDist < ptop.dist)) {
+                p.wpop();
+                p.count(CNT_WIN_POP, 1);
+                if (top.level > nf) continue;
+                if (top.twin != -1 && keptFace >= 0 && top.twin / 3 == keptFace && nStored < storeCap)
+                    stored[nStored++] = top;
+                propagate(top);
+            } else if (havePtop && (!haveTop || top.minDist >= ptop.dist)) {
+                p.ppop();
+                p.count(CNT_PSEUDO_POP, 1);
+                if (ptop.level >= nf) continue;
+                gen_sub_wins(ptop);
+            } else if (haveTop || havePtop) {
+                p.count(CNT_ERRORS, 1);                  // unordered keys (NaN)
+                break;
+            }
+            if (K != -1 && total >= K) break;
+            if (p.stop()) break;
+        }
+    }
+
+    // ICHDevice::BuildGeodesicPathTo(faceId, pos, ...) (ICHDevice.cu:143-334):
+    // distance from the source to a point of face f through a stored window
+    // or one of f's vertices, and the key points of that path.  fv[j] is
+    // verts[0] of half-edge 3f+j and dP[j] the point's straight-line distance
+    // to it.  srcX / dstX keep the caller's values unless a key point is
+    // written.  Returns DBL_MAX (no key points) when nothing reaches f.
+    SVG_HD double face_path_to(int f, const int fv[3], const double dP[3], const Win* stored, int nStored,
+                               int& srcEdge, double& srcX, int& dstEdge, double& dstX, int hopCap)
+    {
+        const int nv = p.nv();
+        srcEdge = -1; dstEdge = -1;
+        double minDist = DBL_MAX, xInter = 0.0, posx = 0.0, posy = 0.0;
+        int mi = -1, dstVert = -1;
+        bool throughWin = true;
+        for (int i = 0; i < nStored; ++i) {
+            const Win& w = stored[i];
+            const int tw = w.twin;
+            if (tw == -1 || tw / 3 != f) continue;
+            Tri t;
+            p.face3(tw, t);
+            const double l0 = t.l0, l1 = t.l1, l2 = t.l2;
+            const double p0x = 0.0, p0y = 0.0, p1x = l0, p1y = 0.0;
+            const double p2x = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0);
+            const double p2y = -sqrt(fabs(l1 * l1 - p2x * p2x));
+            // v0 = verts[1](e0), v1 = verts[0](e0), v2 = verts[1](e1)
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                if (fv[j] == t.b) a = dP[j];
+                if (fv[j] == t.a) b = dP[j];
+                if (fv[j] == t.op) c = dP[j];
+            }
+            // the point's barycentric weights from Heron's formula
+            double s0 = (b + c + l2) / 2.0;
+            double s1 = (a + c + l1) / 2.0;
+            double s2 = (a + b + l0) / 2.0;
+            s0 = sqrt(fabs(s0 * (s0 - b) * (s0 - c) * (s0 - l2)));
+            s1 = sqrt(fabs(s1 * (s1 - a) * (s1 - c) * (s1 - l1)));
+            s2 = sqrt(fabs(s2 * (s2 - a) * (s2 - b) * (s2 - l0)));
+            const double w0 = s0 / (s0 + s1 + s2);
+            const double w1 = s1 / (s0 + s1 + s2);
+            const double w2 = s2 / (s0 + s1 + s2);
+            const double cx = w0 * p0x + w1 * p1x + w2 * p2x;
+            const double cy = w0 * p0y + w1 * p1y + w2 * p2y;
+            const double xi = w.sx - (cx - w.sx) / (cy - w.sy) * w.sy;
+            if (xi <= 0.0 || xi >= l0) continue;
+            double cur;
+            if (xi > w.b0 && xi < w.b1) cur = len2(cx - w.sx, cy - w.sy) + w.ps;
+            else if (xi <= w.b0) cur = len2(cx - w.b0, cy - 0.0) + w.d0 + w.ps;
+            else cur = len2(cx - w.b1, cy - 0.0) + w.d1 + w.ps;
+            if (cur < minDist) { minDist = cur; mi = i; xInter = xi; posx = cx; posy = cy; }
+        }
+        for (int j = 0; j < 3; ++j) {
+            const double cd = dP[j] + p.vget(fv[j]).dist;
+            if (cd < minDist) { throughWin = false; dstVert = fv[j]; minDist = cd; }
+        }
+        if (minDist == DBL_MAX) return minDist;
+        if (!throughWin) {
+            path_to(dstVert, srcEdge, srcX, dstEdge, dstX, hopCap);
+            pathPassVert = true;
+            dstEdge = p.first_edge(dstVert); dstX = 0.0;
+            return minDist;
+        }
+        const Win& mw = stored[mi];
+        Unfold u;
+        u.gid = mw.twin;
+        u.gpos = p.len(u.gid) - xInter;
+        dstEdge = u.gid; dstX = u.gpos;
+        int tw = p.twin(u.gid);
+        if (tw == -1) { p.count(CNT_ERRORS, 1); return minDist; }
+        int opVert = p.v1(p.next(tw));
+        u.lpx = posx; u.lpy = posy;
+        u.cpx = p.len(u.gid) - u.gpos; u.cpy = 0.0;
+        const int ps = mw.pseudoSrc;
+        int hops = 0;
+        while (ps >= nv ? tw / 3 != srcFace : opVert != ps) {
+            if (++hops > hopCap) { p.count(CNT_ERRORS, 1); return minDist; }
+            p.count(CNT_BT_HOPS, 1);
+            if (!unfold(u)) { p.count(CNT_ERRORS, 1); return minDist; }
+            srcEdge = u.gid; srcX = u.gpos;
+            tw = p.twin(u.gid);
+            if (tw == -1) { p.count(CNT_ERRORS, 1); return minDist; }
+            opVert = p.v1(p.next(tw));
+        }
+        if (ps < nv && !p.vget(opVert).isSource) {
+            // the path bends at the pseudo-source: the source-side key point
+            // is the one next to the source on that vertex's own path
+            pathPassVert = true;
+            int de; double dx;
+            path_to(opVert, srcEdge, srcX, de, dx, hopCap);
+        }
+        return minDist;
     }
 };
 

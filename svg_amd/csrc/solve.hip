@@ -14,9 +14,11 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
+#include "point.h"
 #include "solve.h"
 #include "svg_capi_internal.h"
 
@@ -1110,6 +1112,11 @@ void svg_solver_release(svg_graph_s* g)
         delete g->solver;
         g->solver = nullptr;
     }
+    if (g && g->point) {
+        point_free(*g->point);
+        delete g->point;
+        g->point = nullptr;
+    }
 }
 
 static int get_solver(svg_graph_s* g, int slots, SolverState** out)
@@ -1187,6 +1194,60 @@ extern "C" int svg_msad(svg_graph_t g, const int32_t* src, const double* w, int 
     e = solver_fetch(*st, 0, dist, nullptr, src_id, parent);
     solver_release_slot(*st, 0);
     if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+    return SVG_OK;
+}
+
+extern "C" int svg_sssd_point(svg_graph_t g, int f0, const double* p0, int f1, const double* p1, double* dist,
+                              int32_t* key_edge, double* key_x, int32_t* last_vert, int* route, int32_t* path,
+                              int path_cap, int* path_len)
+{
+    if (!g || !p0 || !p1 || !dist) return svg_set_error(SVG_E_ARG, "null argument");
+    const DevMesh& dm = g->mesh->d;
+    const HostMesh& hm = g->mesh->h;
+    if (f0 < 0 || f1 < 0 || f0 >= hm.nf || f1 >= hm.nf) return svg_set_error(SVG_E_ARG, "face out of range");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(p0[k]) || !std::isfinite(p1[k])) return svg_set_error(SVG_E_ARG, "point not finite");
+    if (path_cap > 0 && !path) return svg_set_error(SVG_E_ARG, "null path");
+    std::lock_guard<std::mutex> lk(g->mu);
+    int rc = svg_mesh_upload_impl(const_cast<svg_mesh_s*>(g->mesh), true);
+    if (rc) return rc;
+    if (!g->point) g->point = new PointWork();
+    std::string e = point_alloc(*g->point, g->d.K);
+    if (!e.empty()) return svg_set_error(SVG_E_MEM, e);
+    const int hopCap = 4 * (hm.nf + hm.nv) + 16;
+    SolverState* st = nullptr;
+    rc = get_solver(g, 1, &st);
+    if (rc) return rc;
+    PointRes r;
+    e = point_ich(*g->point, dm, f0, p0, f1, p1, hopCap, st->stream, r);
+    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+    int n = 0;
+    if (r.srcEdge == -1) {
+        // no source-side key point: join through the graph (SVG.cu:284-352)
+        std::vector<int> src;
+        std::vector<double> w;
+        e = point_seeds(*g->point, st->stream, src, w);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        e = solver_admit(*st, dm, 0, Q_MSAD, src.data(), w.data(), (int)src.size(), -1, true);
+        if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
+        rc = run_to_completion(g, st);
+        if (rc) return rc;
+        e = point_join(*g->point, dm, st->dist, st->parent, f0, f1, hopCap, st->stream, r);
+        if (!e.empty()) { solver_release_slot(*st, 0); return svg_set_error(SVG_E_HIP, e); }
+        if (path_cap > 0 && r.lastVert >= 0) {
+            std::vector<int32_t> par((size_t)g->d.V);
+            e = solver_fetch(*st, 0, nullptr, nullptr, nullptr, par.data());
+            if (!e.empty()) { solver_release_slot(*st, 0); return svg_set_error(SVG_E_HIP, e); }
+            for (int u = r.lastVert; u >= 0 && u < g->d.V && n < path_cap; u = par[u]) path[n++] = u;
+        }
+        solver_release_slot(*st, 0);
+    }
+    *dist = r.dist;
+    if (key_edge) { key_edge[0] = r.srcEdge; key_edge[1] = r.dstEdge; }
+    if (key_x) { key_x[0] = r.srcX; key_x[1] = r.dstX; }
+    if (last_vert) *last_vert = r.lastVert;
+    if (route) *route = r.route;
+    if (path_len) *path_len = n;
     return SVG_OK;
 }
 

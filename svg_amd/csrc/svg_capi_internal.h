@@ -9,7 +9,7 @@
 #include "svg_graph.h"
 #include "svg_mesh.h"
 
-namespace svg { struct SolverState; }   // solve.h
+namespace svg { struct SolverState; struct PointWork; }   // solve.h, point.h
 
 struct svg_mesh_s {
     svg::HostMesh h;
@@ -24,9 +24,11 @@ struct svg_graph_s {
     svg::BuildStats stats;
     long long arcs = 0;
     svg::SolverState* solver = nullptr;   // lazily created query workspace
+    svg::PointWork* point = nullptr;      // lazily created surface-point query buffers
     std::mutex mu;
 };
 
 int svg_set_error(int code, const std::string& msg);
 int svg_ensure_device();
 void svg_solver_release(svg_graph_s* g);
+int svg_mesh_upload_impl(svg_mesh_s* m, bool aos);

@@ -1,0 +1,107 @@
+"""GPU surface-point SSSD (SVG::SolveSSSD(f0, p0, f1, p1), SVG.cu:253-352)
+against the oracle's restatement on the same graph.
+
+Both ICH runs and the direct answer are the same arithmetic in the same order
+as the oracle (bit-equal distances); the graph stage's distances are bit-equal
+to the oracle's Dijkstra; the join picks the same vertex.  Parents can differ
+only where two graph paths tie exactly (the solver keeps the lowest parent id,
+the reference the first relaxation), which these meshes do not produce.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import svg_amd as S
+from svg_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+MESHES = {
+    "sphere642_K40": (lambda: mg.noisy_sphere(seed=9, splits4=3, centroid_split=False), 40),
+    "torus40x24_K80": (lambda: mg.bumpy_torus(nu=40, nv=24), 80),
+    "grid12_full": (lambda: mg.planar_grid(n=12, jitter=0.3, seed=7), 144),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(MESHES))
+def graph(request):
+    make, K = MESHES[request.param]
+    xyz, tri = make()
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(K)
+    g.ConstructSVG()
+    h = g.CopySVGToHost()
+    rows = np.zeros(h["adj"].shape, dtype=O.ROW_DTYPE)
+    rows["adj"] = h["adj"]
+    rows["geodDist"] = h["dist"]
+    om = O.Mesh(xyz, tri)
+    twin = om.export()["twin"]
+    return request.param, xyz, tri, K, g, om, rows, h["tails"], twin
+
+
+def _pairs(rng, tri, twin, n):
+    out = []
+    for i in range(n):
+        f0 = int(rng.integers(len(tri)))
+        if i % 3 == 2:
+            f1 = int(rng.integers(len(tri)))
+        else:                               # a short walk: both routes occur
+            f1 = f0
+            for _ in range(int(rng.integers(0, 6))):
+                t = int(twin[3 * f1 + int(rng.integers(3))])
+                if t >= 0:
+                    f1 = t // 3
+        w0, w1 = rng.dirichlet([2, 2, 2]), rng.dirichlet([2, 2, 2])
+        out.append((f0, w0, f1, w1))
+    return out
+
+
+def test_point_queries_match_oracle(graph):
+    name, xyz, tri, K, g, om, rows, tails, twin = graph
+    rng = np.random.default_rng(11)
+    routes = {}
+    for f0, w0, f1, w1 in _pairs(rng, tri, twin, 36):
+        p0, p1 = w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]
+        print(f"{name} point query f{f0} -> f{f1}", flush=True)
+        r = g.SolveSSSDPoints(f0, p0, f1, p1, path=True)
+        o = O.sssd_point(om, rows, tails, K, f0, p0, f1, p1)
+        routes[o["route"]] = routes.get(o["route"], 0) + 1
+        assert r.route == o["route"], (f0, f1, r, o["route"])
+        assert r.dist == o["dist"], (f0, f1, r.dist, o["dist"])
+        assert (r.src_edge, r.dst_edge, r.last_vert) == (o["src_edge"], o["dst_edge"], o["last_vert"])
+        assert r.src_x == o["src_x"] and r.dst_x == o["dst_x"]
+        if r.route == 1:
+            # the graph path runs from the join vertex back to a seed, and its
+            # first vertex's graph distance is its own ICH distance
+            assert r.path[0] == r.last_vert
+            assert o["gparent"][r.path[-1]] == -1
+            assert np.array_equal(r.path[1:], o["gparent"][r.path[:-1]])
+        else:
+            assert len(r.path) == 0
+    assert routes.get(0, 0) > 3 and routes.get(1, 0) > 3, routes
+
+
+def test_point_query_planar_direct_is_euclidean(graph):
+    name, xyz, tri, K, g, om, rows, tails, twin = graph
+    if not name.startswith("grid"):
+        pytest.skip("closed form on the flat mesh only")
+    rng = np.random.default_rng(4)
+    n = 0
+    for f0, w0, f1, w1 in _pairs(rng, tri, twin, 30):
+        p0, p1 = w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]
+        r = g.SolveSSSDPoints(f0, p0, f1, p1)
+        if r.route == 0:
+            eu = np.linalg.norm(p1 - p0)
+            assert abs(r.dist - eu) <= 1e-12 * eu
+            n += 1
+    assert n > 5
+
+
+def test_point_query_rejects_bad_faces(graph):
+    name, xyz, tri, K, g, *_ = graph
+    with pytest.raises(S.SvgError):
+        g.SolveSSSDPoints(len(tri), xyz[0], 0, xyz[1])
+    with pytest.raises(S.SvgError):
+        g.SolveSSSDPoints(0, [np.nan, 0, 0], 1, xyz[1])
