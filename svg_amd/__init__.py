@@ -253,6 +253,24 @@ class SVG:
         return PointResult(d.value, int(ke[0]), float(kx[0]), int(ke[1]), float(kx[1]), lv.value, rt.value,
                            buf[: n.value].copy() if path else None)
 
+    def SolveSSSDPointsBatch(self, faces, points):
+        """Many surface-point queries: faces (n, 2) = (f0, f1), points (n, 2, 3)
+        = (p0, p1).  Returns dict of arrays: dist, src_edge, src_x, dst_edge,
+        dst_x, last_vert, route (as SolveSSSDPoints)."""
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 2)
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2, 3)
+        n = f.shape[0]
+        if p.shape[0] != n:
+            raise ValueError("faces and points differ in length")
+        dist = np.empty(n, np.float64)
+        ke = np.empty(2 * n, np.int32)
+        kx = np.empty(2 * n, np.float64)
+        lv = np.empty(n, np.int32)
+        rt = np.empty(n, np.int32)
+        check(lib().svg_sssd_point_batch(self.g, n, f.reshape(-1), p.reshape(-1), dist, ke, kx, lv, rt))
+        return dict(dist=dist, src_edge=ke[0::2].copy(), src_x=kx[0::2].copy(), dst_edge=ke[1::2].copy(),
+                    dst_x=kx[1::2].copy(), last_vert=lv, route=rt)
+
     def SolveMSAD(self, sources, weights, want_parent: bool = True):   # SVG.cu:354-367
         V, _, _ = self.info()
         src = np.ascontiguousarray(sources, dtype=np.int32)

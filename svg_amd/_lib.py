@@ -70,7 +70,7 @@ EXPORTS = [
     "svg_mesh_upload", "svg_mesh_info", "svg_mesh_export", "svg_mesh_free",
     "svg_build_params_default", "svg_build", "svg_build_stats_get", "svg_build_status",
     "svg_graph_info", "svg_graph_to_host", "svg_graph_save", "svg_graph_load", "svg_graph_free",
-    "svg_sssd", "svg_msad", "svg_sssd_point", "svg_sssd_batch", "svg_server_start", "svg_server_submit",
+    "svg_sssd", "svg_msad", "svg_sssd_point", "svg_sssd_point_batch", "svg_sssd_batch", "svg_server_start", "svg_server_submit",
     "svg_server_wait", "svg_server_stats_get", "svg_server_stop", "svg_server_replay",
 ]
 
@@ -114,6 +114,7 @@ def lib():
                            C.POINTER(C.c_int)]
     L.svg_msad.argtypes = [_vp, _ip, _dp, C.c_int, _dp, _vp, _vp]
     L.svg_sssd_batch.argtypes = [_vp, _ip, C.c_int, _dp]
+    L.svg_sssd_point_batch.argtypes = [_vp, C.c_int, _ip, _dp, _dp, _ip, _dp, _ip, _ip]
     L.svg_sssd_point.argtypes = [_vp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_double), _ip, _dp,
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int), _vp, C.c_int, C.POINTER(C.c_int)]
     L.svg_trace_arc.argtypes = [_vp, C.c_int, C.c_int, C.c_int32, C.c_double, C.c_int32, _dp, C.c_int,

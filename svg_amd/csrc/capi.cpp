@@ -402,6 +402,7 @@ extern "C" int svg_graph_free(svg_graph_t g)
     if (!g) return SVG_OK;
     svg_ensure_device();
     svg_solver_release(g);
+    svg_point_release(g);
     free_graph(g->d);
     delete g;
     return SVG_OK;

@@ -27,33 +27,41 @@ struct PointRes {
 };
 
 struct PointWork {
-    int K = 0;
+    int K = 0, cap = 0;                  // K of the graph; queries the buffers hold
     unsigned ssz = 0, vsz = 0;
-    void* sp[2] = {nullptr, nullptr};    // split tables (K*10 + 1 records)
-    void* vt[2] = {nullptr, nullptr};    // vertex tables (K*2 + 1 records)
-    void* wq[2] = {nullptr, nullptr};    // window heaps
-    void* pq[2] = {nullptr, nullptr};    // pseudo-source heaps
-    void* stored = nullptr;              // windows kept for the destination face
-    PointRes* res = nullptr;             // device result
-    std::vector<uint8_t> hostVt;         // host copy of the p0 vertex table
+    void* sp = nullptr;                  // split tables (K*10 + 1 records), 2 runs per query
+    void* vt = nullptr;                  // vertex tables (K*2 + 1 records)
+    void* wq = nullptr;                  // window heaps
+    void* pq = nullptr;                  // pseudo-source heaps
+    void* stored = nullptr;              // windows kept for the destination face, per query
+    PointRes* res = nullptr;             // device results, per query
+    int32_t* fDev = nullptr;             // faces (f0, f1) per query
+    double* pDev = nullptr;              // points (p0, p1) per query
+    int ldsOk = -1;                      // a run's heaps and vertex table fit a block's LDS (-1: not checked)
+    std::vector<uint8_t> hostVt;         // host copy of one p0 vertex table
 };
 
-std::string point_alloc(PointWork& w, int K);
+// Device bytes one query of the batch holds.
+size_t point_bytes_per_query(int K);
+std::string point_alloc(PointWork& w, int K, int cap);
 void point_free(PointWork& w);
 
-// Both ICH runs (p0 keeping the windows into f1, and p1) and the direct
-// answer through f1; the result lands in `out`.  Points are xyz triples.
-std::string point_ich(PointWork& w, const DevMesh& m, int f0, const double* p0, int f1, const double* p1,
-                      int hopCap, hipStream_t stream, PointRes& out);
+// Both ICH runs of n queries (faces[2n] = f0, f1; pts[6n] = p0, p1) and each
+// query's direct answer through f1, copied to out[n].
+std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* faces, const double* pts, int hopCap,
+                      hipStream_t stream, PointRes* out);
 
-// Graph seeds from the p0 run: every vertex it reached, in its table's slot
-// order (SVG.cu:283-291).
-std::string point_seeds(PointWork& w, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist);
+// Graph seeds of query q: every vertex its p0 run reached, in its table's
+// slot order (SVG.cu:283-291).
+std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist);
 
-// Joins the graph stage (dist bits / parents of V vertices, device) with the
-// p1 run, then backtraces both ends.  `out` carries the direct attempt's
-// result in and the final one out.
-std::string point_join(PointWork& w, const DevMesh& m, const unsigned long long* gdist, const int32_t* gparent,
-                       int f0, int f1, int hopCap, hipStream_t stream, PointRes& out);
+// Joins query q's graph stage (dist bits / parents of V vertices, device)
+// with its p1 run and backtraces both ends, into its device result
+// (asynchronous on stream).
+std::string point_join(PointWork& w, const DevMesh& m, int q, const unsigned long long* gdist,
+                       const int32_t* gparent, int hopCap, hipStream_t stream);
+
+// Copies the first n device results to out.
+std::string point_results(PointWork& w, int n, hipStream_t stream, PointRes* out);
 
 }  // namespace svg

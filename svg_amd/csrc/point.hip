@@ -1,11 +1,13 @@
 // Geodesic distance between two surface points on gfx950 (SVG::SolveSSSD(f0,
 // p0, f1, p1), SVG.cu:253-352).
 //
-// k_point_ich runs the two bounded ICH propagations the query needs, one per
-// wave of a 128-thread block (wave 0: from p0, keeping the windows that cross
-// into f1; wave 1: from p1), in the reference's table layout
-// (plain_policy.h), then wave 0 answers directly through f1
-// (BuildGeodesicPathTo(face, pos), ICHDevice.cu:143-334).  The reference runs
+// Each query needs two bounded ICH propagations (from p0, keeping the windows
+// that cross into f1, and from p1), run in the reference's table layout
+// (plain_policy.h); the p0 run then answers directly through f1
+// (BuildGeodesicPathTo(face, pos), ICHDevice.cu:143-334).  A single query
+// runs each propagation in one block with its heaps in LDS
+// (k_point_ich_lds); many queries run one propagation per thread
+// (k_point_ich_many).  The reference runs
 // the p0 propagation twice more (SVG.cu:294-296, 330-332) only to rebuild
 // state these resident tables still hold.  When the direct answer has no
 // source-side key point (SVG.cu:284) the graph stage runs on the solver (a
@@ -41,6 +43,8 @@ namespace {
 
 constexpr unsigned kSplitCoef = 10, kVertCoef = 2;   // kernel.cu:188-191 (K*10, K*2 entries)
 
+// Run r of the batch is query r / 2 from its first point (r even: p0, keeping
+// the windows that cross into f1) or its second (r odd: p1).
 struct PointArgs {
     EdgeAoS* edges;
     VertAoS* verts;
@@ -48,14 +52,15 @@ struct PointArgs {
     double* cornerB;
     int nv, nf, K, hopCap;
     unsigned ssz, vsz;
-    PSplitItem* sp[2];
-    PVertItem* vt[2];
-    PWinItem* wq[2];
-    PPsItem* pq[2];
-    Win* stored;
-    PointRes* res;
-    int f[2];
-    double p[2][3];
+    PSplitItem* sp;          // per-run tables, run-major
+    PVertItem* vt;
+    PWinItem* wq;
+    PPsItem* pq;
+    Win* stored;             // kStoredWin per query
+    PointRes* res;           // per query
+    const int32_t* f;        // f0, f1 per query
+    const double* p;         // p0, p1 (xyz) per query
+    int nRuns;
 };
 
 __device__ void make_policy(const PointArgs& a, int run, PlainPolicy& p)
@@ -63,7 +68,10 @@ __device__ void make_policy(const PointArgs& a, int run, PlainPolicy& p)
     p.E = a.edges; p.Vt = a.verts; p.cA = a.cornerA; p.cB = a.cornerB;
     p.nv_ = a.nv; p.nf_ = a.nf;
     p.ssz = a.ssz; p.vsz = a.vsz;
-    p.sp = a.sp[run]; p.vt = a.vt[run]; p.wq = a.wq[run]; p.pq = a.pq[run];
+    p.sp = a.sp + (size_t)run * (a.ssz + 1);
+    p.vt = a.vt + (size_t)run * (a.vsz + 1);
+    p.wq = a.wq + (size_t)run * (kHeapCap + 1);
+    p.pq = a.pq + (size_t)run * (kHeapCap + 1);
     p.wtail = 1; p.ptail = 1; p.flags = 0;
     p.srcVert = a.nv;            // srcId of a surface point (ICHDevice.cu:596)
 }
@@ -80,47 +88,84 @@ __device__ void face_dists(const PointArgs& a, int f, const double* q, int fv[3]
     }
 }
 
-__global__ void __launch_bounds__(128) k_point_ich(PointArgs a)
+// One ICH run from its point; the p0 run then answers directly through f1.
+__device__ void point_run(const PointArgs& a, int run, PlainPolicy& p)
 {
-    if ((threadIdx.x & 63) != 0) return;
-    const int run = threadIdx.x >> 6;
-    PlainPolicy p;
-    make_policy(a, run, p);
+    const int q = run >> 1, side = run & 1;
+    const int f = a.f[2 * q + side];
     p.clear();
     Ich<PlainPolicy> ich(p, -1, a.K);
-    ich.srcFace = a.f[run];
+    ich.srcFace = f;
     int fv[3];
     double d[3];
-    face_dists(a, a.f[run], a.p[run], fv, d);
-    ich.initialize_point(a.f[run], d);
+    face_dists(a, f, a.p + 6 * q + 3 * side, fv, d);
+    ich.initialize_point(f, d);
     int nStored = 0;
-    ich.execute_point(a.stored, run == 0 ? kStoredWin : 0, nStored, run == 0 ? a.f[1] : -1);
-    if (run == 1) {
-        atomicOr(&a.res->status, p.flags);
-        return;
+    Win* stored = a.stored + (size_t)q * kStoredWin;
+    ich.execute_point(stored, side == 0 ? kStoredWin : 0, nStored, side == 0 ? a.f[2 * q + 1] : -1);
+    PointRes* res = a.res + q;
+    if (side == 0) {
+        PointRes r;
+        r.srcX = 0.0; r.dstX = 0.0;            // SVGNode X fields start at 0
+        face_dists(a, a.f[2 * q + 1], a.p + 6 * q + 3, fv, d);
+        r.dist = ich.face_path_to(a.f[2 * q + 1], fv, d, stored, nStored, r.srcEdge, r.srcX, r.dstEdge, r.dstX,
+                                  a.hopCap);
+        res->dist = r.dist; res->srcX = r.srcX; res->dstX = r.dstX;
+        res->srcEdge = r.srcEdge; res->dstEdge = r.dstEdge;
+        res->lastVert = -1; res->firstVert = -1; res->route = 0;
     }
-    PointRes r;
-    r.srcX = 0.0; r.dstX = 0.0;            // SVGNode X fields start at 0
-    face_dists(a, a.f[1], a.p[1], fv, d);
-    r.dist = ich.face_path_to(a.f[1], fv, d, a.stored, nStored, r.srcEdge, r.srcX, r.dstEdge, r.dstX, a.hopCap);
-    r.lastVert = -1; r.firstVert = -1;
-    r.route = 0;
-    atomicOr(&a.res->status, p.flags);
-    a.res->dist = r.dist; a.res->srcX = r.srcX; a.res->dstX = r.dstX;
-    a.res->srcEdge = r.srcEdge; a.res->dstEdge = r.dstEdge;
-    a.res->lastVert = r.lastVert; a.res->firstVert = r.firstVert; a.res->route = r.route;
+    atomicOr(&res->status, p.flags);
+}
+
+// LDS bytes of a run's window heap, pseudo-source heap and vertex table.
+inline size_t lds_bytes(unsigned vsz)
+{
+    return sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1) + sizeof(PVertItem) * (vsz + 1);
+}
+
+// One block per run, for a single query.  A run is one sequential
+// propagation, a chain of dependent heap and table accesses: its two heaps
+// and its vertex table live in the block's LDS (160 KB on gfx950; the split
+// table stays in global memory) and the vertex table is written back for the
+// join.
+__global__ void __launch_bounds__(64) k_point_ich_lds(PointArgs a)
+{
+    extern __shared__ double ldsBuf[];
+    const int run = blockIdx.x;
+    PlainPolicy p;
+    make_policy(a, run, p);
+    PVertItem* dst = p.vt;
+    unsigned char* b = (unsigned char*)ldsBuf;
+    p.wq = (PWinItem*)b;
+    p.pq = (PPsItem*)(b + sizeof(PWinItem) * (kHeapCap + 1));
+    p.vt = (PVertItem*)(b + sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1));
+    if (threadIdx.x == 0) point_run(a, run, p);
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < a.vsz + 1; i += blockDim.x) dst[i] = p.vt[i];
+}
+
+// Many queries: one run per thread, tables in global memory (the plain build
+// kernel's layout), so the device holds thousands of runs at once.
+__global__ void __launch_bounds__(64) k_point_ich_many(PointArgs a)
+{
+    const int total = gridDim.x * blockDim.x;
+    for (int run = blockIdx.x * blockDim.x + threadIdx.x; run < a.nRuns; run += total) {
+        PlainPolicy p;
+        make_policy(a, run, p);
+        point_run(a, run, p);
+    }
 }
 
 constexpr int kJoinThreads = 256;
 
-__global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const unsigned long long* gdist,
+__global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, int q, const unsigned long long* gdist,
                                                               const int32_t* gparent)
 {
     // min over the p1 table's slots of graph + ICH distance; ties keep the
     // lowest slot (the reference's strict-< scan in slot order)
     __shared__ double sBest[kJoinThreads];
     __shared__ int sSlot[kJoinThreads];
-    const PVertItem* vt = a.vt[1];
+    const PVertItem* vt = a.vt + (size_t)(2 * q + 1) * (a.vsz + 1);
     double best = DBL_MAX;
     int slot = INT_MAX;
     for (int i = threadIdx.x; i < (int)a.vsz; i += kJoinThreads) {
@@ -144,7 +189,7 @@ __global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const 
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    PointRes r = *a.res;             // the direct attempt
+    PointRes r = a.res[q];           // the direct attempt
     r.route = 1;
     r.dist = sBest[0];
     if (sSlot[0] == INT_MAX || !(sBest[0] < DBL_MAX)) {
@@ -153,7 +198,7 @@ __global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const 
         r.route = -1; r.dist = DBL_MAX;
         r.srcEdge = r.dstEdge = -1; r.srcX = r.dstX = 0.0;
         r.lastVert = r.firstVert = -1;
-        *a.res = r;
+        a.res[q] = r;
         return;
     }
     const int last = vt[sSlot[0]].index;
@@ -161,12 +206,11 @@ __global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const 
     // p1 side: its source-side key point becomes the destination-side one
     // (SVG.cu:309-318), converted to the twin half-edge
     PlainPolicy p1;
-    make_policy(a, 1, p1);
+    make_policy(a, 2 * q + 1, p1);
     Ich<PlainPolicy> ich1(p1, -1, a.K);
-    ich1.srcFace = a.f[1];
+    ich1.srcFace = a.f[2 * q + 1];
     int nDE, bSE;
     double nDX, bSX;
-    nDX = 0.0;
     ich1.path_to(last, nDE, nDX, bSE, bSX, a.hopCap);
     r.srcEdge = bSE; r.srcX = bSX;
     if (nDE == -1) { nDE = r.srcEdge; nDX = r.dstX; }      // reads nextToDstX, as SVG.cu:314 does
@@ -180,11 +224,11 @@ __global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const 
         first = gparent[first];
     r.firstVert = first;
     // p0 side from the first vertex (SVG.cu:320-351); the p0 tables are the
-    // run of k_point_ich (the reference's re-run rebuilds the same state)
+    // run above (the reference's re-run rebuilds the same state)
     PlainPolicy p0;
-    make_policy(a, 0, p0);
+    make_policy(a, 2 * q, p0);
     Ich<PlainPolicy> ich0(p0, -1, a.K);
-    ich0.srcFace = a.f[0];
+    ich0.srcFace = a.f[2 * q];
     if (p0.vget(first).dist != DBL_MAX) {
         int nSE, aDE;
         double nSX, aDX;
@@ -194,80 +238,100 @@ __global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const 
         r.srcEdge = nSE; r.srcX = nSX;
         r.dstEdge = nDE; r.dstX = nDX;
     }
-    *a.res = r;
+    a.res[q] = r;
 }
 
-void fill_args(PointWork& w, const DevMesh& m, int f0, const double* p0, int f1, const double* p1, int hopCap,
-               PointArgs& a)
+PointArgs make_args(PointWork& w, const DevMesh& m, int hopCap, int nRuns)
 {
+    PointArgs a;
     a.edges = m.edges; a.verts = m.verts; a.cornerA = m.cornerA; a.cornerB = m.cornerB;
     a.nv = m.nv; a.nf = m.nf; a.K = w.K; a.hopCap = hopCap;
     a.ssz = w.ssz; a.vsz = w.vsz;
-    for (int r = 0; r < 2; ++r) {
-        a.sp[r] = (PSplitItem*)w.sp[r]; a.vt[r] = (PVertItem*)w.vt[r];
-        a.wq[r] = (PWinItem*)w.wq[r]; a.pq[r] = (PPsItem*)w.pq[r];
-    }
+    a.sp = (PSplitItem*)w.sp; a.vt = (PVertItem*)w.vt; a.wq = (PWinItem*)w.wq; a.pq = (PPsItem*)w.pq;
     a.stored = (Win*)w.stored;
     a.res = w.res;
-    a.f[0] = f0; a.f[1] = f1;
-    for (int k = 0; k < 3; ++k) {
-        a.p[0][k] = p0 ? p0[k] : 0.0;
-        a.p[1][k] = p1 ? p1[k] : 0.0;
-    }
+    a.f = w.fDev; a.p = w.pDev;
+    a.nRuns = nRuns;
+    return a;
 }
 
 }  // namespace
 
-std::string point_alloc(PointWork& w, int K)
+size_t point_bytes_per_query(int K)
 {
-    if (w.K == K && w.res) return std::string();
+    const size_t ssz = (size_t)K * kSplitCoef, vsz = (size_t)K * kVertCoef;
+    return 2 * (sizeof(PSplitItem) * (ssz + 1) + sizeof(PVertItem) * (vsz + 1) +
+                sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1)) +
+           sizeof(Win) * kStoredWin + sizeof(PointRes) + 2 * sizeof(int32_t) + 6 * sizeof(double);
+}
+
+std::string point_alloc(PointWork& w, int K, int cap)
+{
+    if (w.K == K && w.cap >= cap && w.res) return std::string();
     point_free(w);
     w.K = K;
+    w.cap = cap;
     w.ssz = (unsigned)K * kSplitCoef;
     w.vsz = (unsigned)K * kVertCoef;
-    for (int r = 0; r < 2; ++r) {
-        PT_TRY(hipMalloc(&w.sp[r], sizeof(PSplitItem) * (w.ssz + 1)));
-        PT_TRY(hipMalloc(&w.vt[r], sizeof(PVertItem) * (w.vsz + 1)));
-        PT_TRY(hipMalloc(&w.wq[r], sizeof(PWinItem) * (kHeapCap + 1)));
-        PT_TRY(hipMalloc(&w.pq[r], sizeof(PPsItem) * (kHeapCap + 1)));
-    }
-    PT_TRY(hipMalloc(&w.stored, sizeof(Win) * kStoredWin));
-    PT_TRY(hipMalloc(&w.res, sizeof(PointRes)));
+    const size_t runs = 2 * (size_t)cap;
+    PT_TRY(hipMalloc(&w.sp, sizeof(PSplitItem) * (w.ssz + 1) * runs));
+    PT_TRY(hipMalloc(&w.vt, sizeof(PVertItem) * (w.vsz + 1) * runs));
+    PT_TRY(hipMalloc(&w.wq, sizeof(PWinItem) * (kHeapCap + 1) * runs));
+    PT_TRY(hipMalloc(&w.pq, sizeof(PPsItem) * (kHeapCap + 1) * runs));
+    PT_TRY(hipMalloc(&w.stored, sizeof(Win) * kStoredWin * (size_t)cap));
+    PT_TRY(hipMalloc(&w.res, sizeof(PointRes) * (size_t)cap));
+    PT_TRY(hipMalloc(&w.fDev, sizeof(int32_t) * 2 * (size_t)cap));
+    PT_TRY(hipMalloc(&w.pDev, sizeof(double) * 6 * (size_t)cap));
     w.hostVt.resize(sizeof(PVertItem) * (w.vsz + 1));
     return std::string();
 }
 
 void point_free(PointWork& w)
 {
-    for (int r = 0; r < 2; ++r) {
-        hipFree(w.sp[r]); hipFree(w.vt[r]); hipFree(w.wq[r]); hipFree(w.pq[r]);
-        w.sp[r] = w.vt[r] = w.wq[r] = w.pq[r] = nullptr;
-    }
-    hipFree(w.stored); hipFree(w.res);
-    w.stored = nullptr; w.res = nullptr;
-    w.K = 0;
+    hipFree(w.sp); hipFree(w.vt); hipFree(w.wq); hipFree(w.pq);
+    hipFree(w.stored); hipFree(w.res); hipFree(w.fDev); hipFree(w.pDev);
+    w.sp = w.vt = w.wq = w.pq = w.stored = nullptr;
+    w.res = nullptr; w.fDev = nullptr; w.pDev = nullptr;
+    w.K = 0; w.cap = 0;
 }
 
-std::string point_ich(PointWork& w, const DevMesh& m, int f0, const double* p0, int f1, const double* p1,
-                      int hopCap, hipStream_t stream, PointRes& out)
+std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* faces, const double* pts, int hopCap,
+                      hipStream_t stream, PointRes* out)
 {
     if (!m.edges || !m.verts) return "surface-point queries need the reference-layout mesh";
-    if (f0 < 0 || f1 < 0 || f0 >= m.nf || f1 >= m.nf) return "face out of range";
-    PointArgs a;
-    fill_args(w, m, f0, p0, f1, p1, hopCap, a);
-    PointRes zero;
-    memset(&zero, 0, sizeof(zero));
-    PT_TRY(hipMemcpyAsync(w.res, &zero, sizeof(PointRes), hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_point_ich, dim3(1), dim3(128), 0, stream, a);
+    if (n < 1 || n > w.cap) return "query count out of range";
+    for (int i = 0; i < 2 * n; ++i)
+        if (faces[i] < 0 || faces[i] >= m.nf) return "face out of range";
+    PT_TRY(hipMemcpyAsync(w.fDev, faces, sizeof(int32_t) * 2 * (size_t)n, hipMemcpyHostToDevice, stream));
+    PT_TRY(hipMemcpyAsync(w.pDev, pts, sizeof(double) * 6 * (size_t)n, hipMemcpyHostToDevice, stream));
+    PT_TRY(hipMemsetAsync(w.res, 0, sizeof(PointRes) * (size_t)n, stream));
+    PointArgs a = make_args(w, m, hopCap, 2 * n);
+    if (n == 1 && w.ldsOk < 0) {
+        // LDS-resident runs when the heaps and the vertex table fit a block
+        int dev = 0, maxLds = 0;
+        PT_TRY(hipGetDevice(&dev));
+        PT_TRY(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        w.ldsOk = lds_bytes(w.vsz) <= (size_t)maxLds ? 1 : 0;
+        if (w.ldsOk)
+            PT_TRY(hipFuncSetAttribute((const void*)k_point_ich_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes(w.vsz)));
+    }
+    if (n == 1 && w.ldsOk == 1) {
+        hipLaunchKernelGGL(k_point_ich_lds, dim3(2), dim3(64), lds_bytes(w.vsz), stream, a);
+    } else {
+        const int blocks = (2 * n + 63) / 64;
+        hipLaunchKernelGGL(k_point_ich_many, dim3(blocks), dim3(64), 0, stream, a);
+    }
     PT_TRY(hipGetLastError());
-    PT_TRY(hipMemcpyAsync(&out, w.res, sizeof(PointRes), hipMemcpyDeviceToHost, stream));
+    PT_TRY(hipMemcpyAsync(out, w.res, sizeof(PointRes) * (size_t)n, hipMemcpyDeviceToHost, stream));
     PT_TRY(hipStreamSynchronize(stream));
     return std::string();
 }
 
-std::string point_seeds(PointWork& w, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist)
+std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist)
 {
-    PT_TRY(hipMemcpyAsync(w.hostVt.data(), w.vt[0], w.hostVt.size(), hipMemcpyDeviceToHost, stream));
+    const PVertItem* dvt = (const PVertItem*)w.vt + (size_t)(2 * q) * (w.vsz + 1);
+    PT_TRY(hipMemcpyAsync(w.hostVt.data(), dvt, w.hostVt.size(), hipMemcpyDeviceToHost, stream));
     PT_TRY(hipStreamSynchronize(stream));
     const PVertItem* vt = (const PVertItem*)w.hostVt.data();
     src.clear(); dist.clear();
@@ -279,14 +343,18 @@ std::string point_seeds(PointWork& w, hipStream_t stream, std::vector<int>& src,
     return std::string();
 }
 
-std::string point_join(PointWork& w, const DevMesh& m, const unsigned long long* gdist, const int32_t* gparent,
-                       int f0, int f1, int hopCap, hipStream_t stream, PointRes& out)
+std::string point_join(PointWork& w, const DevMesh& m, int q, const unsigned long long* gdist,
+                       const int32_t* gparent, int hopCap, hipStream_t stream)
 {
-    PointArgs a;
-    fill_args(w, m, f0, nullptr, f1, nullptr, hopCap, a);
-    hipLaunchKernelGGL(k_point_join, dim3(1), dim3(kJoinThreads), 0, stream, a, gdist, gparent);
+    PointArgs a = make_args(w, m, hopCap, 0);
+    hipLaunchKernelGGL(k_point_join, dim3(1), dim3(kJoinThreads), 0, stream, a, q, gdist, gparent);
     PT_TRY(hipGetLastError());
-    PT_TRY(hipMemcpyAsync(&out, w.res, sizeof(PointRes), hipMemcpyDeviceToHost, stream));
+    return std::string();
+}
+
+std::string point_results(PointWork& w, int n, hipStream_t stream, PointRes* out)
+{
+    PT_TRY(hipMemcpyAsync(out, w.res, sizeof(PointRes) * (size_t)n, hipMemcpyDeviceToHost, stream));
     PT_TRY(hipStreamSynchronize(stream));
     return std::string();
 }

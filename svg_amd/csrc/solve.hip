@@ -1112,6 +1112,10 @@ void svg_solver_release(svg_graph_s* g)
         delete g->solver;
         g->solver = nullptr;
     }
+}
+
+void svg_point_release(svg_graph_s* g)
+{
     if (g && g->point) {
         point_free(*g->point);
         delete g->point;
@@ -1197,57 +1201,130 @@ extern "C" int svg_msad(svg_graph_t g, const int32_t* src, const double* w, int 
     return SVG_OK;
 }
 
+// Surface-point queries (SVG.cu:253-352) on the graph: the ICH runs and
+// direct answers of up to `chunk` queries at once, then the graph stage of
+// those without a source-side key point through the solver's slots (SVG.cu:
+// 284-352), each joined on the device as its slot finishes.  For one query,
+// path (cap entries) receives the graph vertices from last_vert back.
+static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const double* pts, PointRes* out,
+                         int32_t* path, int path_cap, int* path_len)
+{
+    const DevMesh& dm = g->mesh->d;
+    const HostMesh& hm = g->mesh->h;
+    for (int i = 0; i < 2 * nq; ++i)
+        if (faces[i] < 0 || faces[i] >= hm.nf) return svg_set_error(SVG_E_ARG, "face out of range");
+    for (int i = 0; i < 6 * nq; ++i)
+        if (!std::isfinite(pts[i])) return svg_set_error(SVG_E_ARG, "point not finite");
+    int rc = svg_mesh_upload_impl(const_cast<svg_mesh_s*>(g->mesh), true);
+    if (rc) return rc;
+    // a chunk's tables: about 570 KB per query at K = 500
+    const int chunk = std::max(1, std::min(nq, 2048));
+    if (!g->point) g->point = new PointWork();
+    std::string e = point_alloc(*g->point, g->d.K, chunk);
+    if (!e.empty()) return svg_set_error(SVG_E_MEM, e);
+    PointWork& w = *g->point;
+    const int hopCap = 4 * (hm.nf + hm.nv) + 16;
+    const int S = std::min(64, std::max(1, nq));
+    SolverState* st = nullptr;
+    rc = get_solver(g, S, &st);
+    if (rc) return rc;
+    if (path_len) *path_len = 0;
+    std::vector<int> src;
+    std::vector<double> wts;
+    for (int c0 = 0; c0 < nq; c0 += chunk) {
+        const int n = std::min(chunk, nq - c0);
+        e = point_ich(w, dm, n, faces + 2 * (size_t)c0, pts + 6 * (size_t)c0, hopCap, st->stream, out + c0);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        std::vector<int> todo;
+        for (int q = 0; q < n; ++q)
+            if (out[c0 + q].srcEdge == -1) todo.push_back(q);
+        if (todo.empty()) continue;
+        std::vector<int> slotQuery((size_t)st->S, -1);
+        size_t next = 0, done = 0;
+        while (done < todo.size()) {
+            std::vector<Admission> adm;
+            for (int sl = 0; sl < st->S && next < todo.size(); ++sl) {
+                if (slotQuery[sl] != -1) continue;
+                const int q = todo[next++];
+                e = point_seeds(w, q, st->stream, src, wts);
+                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                Admission a;
+                a.slot = sl; a.kind = Q_MSAD; a.target = -1; a.wantParent = true;
+                a.src = src; a.w = wts;
+                adm.push_back(std::move(a));
+                slotQuery[sl] = q;
+            }
+            if (!adm.empty()) {
+                e = solver_admit_many(*st, dm, adm);
+                if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
+            }
+            int running = 0;
+            e = solver_round(*st, g->d, dm, running);
+            if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+            for (int sl = 0; sl < st->S; ++sl) {
+                if (slotQuery[sl] == -1 || !st->spHost[sl].done) continue;
+                const size_t sb = (size_t)sl * st->V;
+                e = point_join(w, dm, slotQuery[sl], st->dist + sb, st->parent + sb, hopCap, st->stream);
+                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                if (nq == 1 && path_cap > 0) {
+                    PointRes r1;
+                    e = point_results(w, 1, st->stream, &r1);
+                    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                    std::vector<int32_t> par((size_t)st->V);
+                    e = solver_fetch(*st, sl, nullptr, nullptr, nullptr, par.data());
+                    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                    int k = 0;
+                    for (int u = r1.lastVert; u >= 0 && u < st->V && k < path_cap; u = par[u]) path[k++] = u;
+                    if (path_len) *path_len = k;
+                }
+                solver_release_slot(*st, sl);
+                slotQuery[sl] = -1;
+                ++done;
+            }
+        }
+        e = point_results(w, n, st->stream, out + c0);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+    }
+    return SVG_OK;
+}
+
 extern "C" int svg_sssd_point(svg_graph_t g, int f0, const double* p0, int f1, const double* p1, double* dist,
                               int32_t* key_edge, double* key_x, int32_t* last_vert, int* route, int32_t* path,
                               int path_cap, int* path_len)
 {
     if (!g || !p0 || !p1 || !dist) return svg_set_error(SVG_E_ARG, "null argument");
-    const DevMesh& dm = g->mesh->d;
-    const HostMesh& hm = g->mesh->h;
-    if (f0 < 0 || f1 < 0 || f0 >= hm.nf || f1 >= hm.nf) return svg_set_error(SVG_E_ARG, "face out of range");
-    for (int k = 0; k < 3; ++k)
-        if (!std::isfinite(p0[k]) || !std::isfinite(p1[k])) return svg_set_error(SVG_E_ARG, "point not finite");
     if (path_cap > 0 && !path) return svg_set_error(SVG_E_ARG, "null path");
     std::lock_guard<std::mutex> lk(g->mu);
-    int rc = svg_mesh_upload_impl(const_cast<svg_mesh_s*>(g->mesh), true);
-    if (rc) return rc;
-    if (!g->point) g->point = new PointWork();
-    std::string e = point_alloc(*g->point, g->d.K);
-    if (!e.empty()) return svg_set_error(SVG_E_MEM, e);
-    const int hopCap = 4 * (hm.nf + hm.nv) + 16;
-    SolverState* st = nullptr;
-    rc = get_solver(g, 1, &st);
-    if (rc) return rc;
+    const int32_t faces[2] = {f0, f1};
+    const double pts[6] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2]};
     PointRes r;
-    e = point_ich(*g->point, dm, f0, p0, f1, p1, hopCap, st->stream, r);
-    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-    int n = 0;
-    if (r.srcEdge == -1) {
-        // no source-side key point: join through the graph (SVG.cu:284-352)
-        std::vector<int> src;
-        std::vector<double> w;
-        e = point_seeds(*g->point, st->stream, src, w);
-        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-        e = solver_admit(*st, dm, 0, Q_MSAD, src.data(), w.data(), (int)src.size(), -1, true);
-        if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
-        rc = run_to_completion(g, st);
-        if (rc) return rc;
-        e = point_join(*g->point, dm, st->dist, st->parent, f0, f1, hopCap, st->stream, r);
-        if (!e.empty()) { solver_release_slot(*st, 0); return svg_set_error(SVG_E_HIP, e); }
-        if (path_cap > 0 && r.lastVert >= 0) {
-            std::vector<int32_t> par((size_t)g->d.V);
-            e = solver_fetch(*st, 0, nullptr, nullptr, nullptr, par.data());
-            if (!e.empty()) { solver_release_slot(*st, 0); return svg_set_error(SVG_E_HIP, e); }
-            for (int u = r.lastVert; u >= 0 && u < g->d.V && n < path_cap; u = par[u]) path[n++] = u;
-        }
-        solver_release_slot(*st, 0);
-    }
+    int rc = point_queries(g, 1, faces, pts, &r, path, path_cap, path_len);
+    if (rc) return rc;
     *dist = r.dist;
     if (key_edge) { key_edge[0] = r.srcEdge; key_edge[1] = r.dstEdge; }
     if (key_x) { key_x[0] = r.srcX; key_x[1] = r.dstX; }
     if (last_vert) *last_vert = r.lastVert;
     if (route) *route = r.route;
-    if (path_len) *path_len = n;
+    return SVG_OK;
+}
+
+extern "C" int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t* faces, const double* points,
+                                    double* dist, int32_t* key_edge, double* key_x, int32_t* last_vert,
+                                    int32_t* route)
+{
+    if (!g || !faces || !points || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
+    if (nq == 0) return SVG_OK;
+    std::lock_guard<std::mutex> lk(g->mu);
+    std::vector<PointRes> r((size_t)nq);
+    int rc = point_queries(g, nq, faces, points, r.data(), nullptr, 0, nullptr);
+    if (rc) return rc;
+    for (int q = 0; q < nq; ++q) {
+        dist[q] = r[q].dist;
+        if (key_edge) { key_edge[2 * q] = r[q].srcEdge; key_edge[2 * q + 1] = r[q].dstEdge; }
+        if (key_x) { key_x[2 * q] = r[q].srcX; key_x[2 * q + 1] = r[q].dstX; }
+        if (last_vert) last_vert[q] = r[q].lastVert;
+        if (route) route[q] = r[q].route;
+    }
     return SVG_OK;
 }
 

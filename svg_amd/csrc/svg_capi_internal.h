@@ -31,4 +31,5 @@ struct svg_graph_s {
 int svg_set_error(int code, const std::string& msg);
 int svg_ensure_device();
 void svg_solver_release(svg_graph_s* g);
+void svg_point_release(svg_graph_s* g);
 int svg_mesh_upload_impl(svg_mesh_s* m, bool aos);

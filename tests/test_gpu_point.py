@@ -105,3 +105,24 @@ def test_point_query_rejects_bad_faces(graph):
         g.SolveSSSDPoints(len(tri), xyz[0], 0, xyz[1])
     with pytest.raises(S.SvgError):
         g.SolveSSSDPoints(0, [np.nan, 0, 0], 1, xyz[1])
+
+
+def test_point_batch_matches_oracle(graph):
+    name, xyz, tri, K, g, om, rows, tails, twin = graph
+    rng = np.random.default_rng(23)
+    qs = _pairs(rng, tri, twin, 70)
+    faces = np.array([(f0, f1) for f0, _, f1, _ in qs], np.int32)
+    pts = np.array([(w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]) for f0, w0, f1, w1 in qs])
+    print(f"{name} point batch of {len(qs)}", flush=True)
+    b = g.SolveSSSDPointsBatch(faces, pts)
+    for i, (f0, f1) in enumerate(faces):
+        o = O.sssd_point(om, rows, tails, K, int(f0), pts[i, 0], int(f1), pts[i, 1])
+        got = (b["route"][i], b["dist"][i], b["src_edge"][i], b["dst_edge"][i], b["last_vert"][i],
+               b["src_x"][i], b["dst_x"][i])
+        want = (o["route"], o["dist"], o["src_edge"], o["dst_edge"], o["last_vert"], o["src_x"], o["dst_x"])
+        assert got == want, (i, got, want)
+    # the single-query path (LDS-resident runs) agrees with the batch
+    for i in range(0, len(qs), 9):
+        r = g.SolveSSSDPoints(int(faces[i, 0]), pts[i, 0], int(faces[i, 1]), pts[i, 1])
+        assert (r.dist, r.src_edge, r.dst_edge, r.last_vert) == (b["dist"][i], b["src_edge"][i],
+                                                                  b["dst_edge"][i], b["last_vert"][i])
