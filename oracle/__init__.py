@@ -153,20 +153,25 @@ def ich_point(mesh: Mesh, face: int, p, K: int = -1, split_coef: int = 10, vert_
 
 def sssd_point(mesh: Mesh, rows, tails, K: int, f0: int, p0, f1: int, p1,
                split_coef: int = 10, vert_coef: int = 2):
-    """SVG::SolveSSSD(f0, p0, f1, p1) on the graph (rows, tails) built with K.
+    """SVG::SolveSSSD(f0, p0, f1, p1) on the graph built with K: `rows` is
+    the row array (with `tails`) or a Graph (tails ignored).
 
     Returns a dict: dist, src_edge, src_x, dst_edge, dst_x, last_vert, route
     (0 direct, 1 through the graph, -1 unreachable), gdist, gparent."""
     V = mesh.nv
-    adj = np.ascontiguousarray(rows["adj"], dtype=np.int32).ravel()
-    w = np.ascontiguousarray(rows["geodDist"], dtype=np.float64).ravel()
-    tl = np.ascontiguousarray(tails, dtype=np.int32)
+    if isinstance(rows, Graph):
+        adj, w, tl, Kg = rows.adj, rows.w, rows.tails, rows.K
+    else:
+        adj = np.ascontiguousarray(rows["adj"], dtype=np.int32).ravel()
+        w = np.ascontiguousarray(rows["geodDist"], dtype=np.float64).ravel()
+        tl = np.ascontiguousarray(tails, dtype=np.int32)
+        Kg = rows.shape[1]
     dout = np.zeros(3, np.float64)
     iout = np.zeros(4, np.int32)
     gdist = np.empty(V, np.float64)
     gpar = np.empty(V, np.int32)
     stat = np.zeros(3, np.int64)
-    rc = lib().oracle_sssd_point(mesh.h, K, split_coef, vert_coef, rows.shape[1], adj, w, tl,
+    rc = lib().oracle_sssd_point(mesh.h, K, split_coef, vert_coef, Kg, adj, w, tl,
                                  f0, np.ascontiguousarray(p0, dtype=np.float64).ravel(), f1,
                                  np.ascontiguousarray(p1, dtype=np.float64).ravel(), dout, iout,
                                  gdist, gpar, stat)

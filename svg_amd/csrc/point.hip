@@ -42,6 +42,9 @@ namespace svg {
 namespace {
 
 constexpr unsigned kSplitCoef = 10, kVertCoef = 2;   // kernel.cu:188-191 (K*10, K*2 entries)
+// Runs per 64-wide wave in k_point_ich_many: a run is a latency-bound chain,
+// so few lanes per wave (less divergence) and more waves per CU pay.
+constexpr int kRunsPerWave = 1;
 
 // Run r of the batch is query r / 2 from its first point (r even: p0, keeping
 // the windows that cross into f1) or its second (r odd: p1).
@@ -144,12 +147,14 @@ __global__ void __launch_bounds__(64) k_point_ich_lds(PointArgs a)
     for (unsigned i = threadIdx.x; i < a.vsz + 1; i += blockDim.x) dst[i] = p.vt[i];
 }
 
-// Many queries: one run per thread, tables in global memory (the plain build
-// kernel's layout), so the device holds thousands of runs at once.
+// Many queries: one run per lane of the first kRunsPerWave lanes of each
+// wave, tables in global memory (the plain build kernel's layout), so the
+// device holds thousands of runs at once.
 __global__ void __launch_bounds__(64) k_point_ich_many(PointArgs a)
 {
-    const int total = gridDim.x * blockDim.x;
-    for (int run = blockIdx.x * blockDim.x + threadIdx.x; run < a.nRuns; run += total) {
+    if ((int)threadIdx.x >= kRunsPerWave) return;
+    const int total = gridDim.x * kRunsPerWave;
+    for (int run = blockIdx.x * kRunsPerWave + threadIdx.x; run < a.nRuns; run += total) {
         PlainPolicy p;
         make_policy(a, run, p);
         point_run(a, run, p);
@@ -319,7 +324,7 @@ std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* face
     if (n == 1 && w.ldsOk == 1) {
         hipLaunchKernelGGL(k_point_ich_lds, dim3(2), dim3(64), lds_bytes(w.vsz), stream, a);
     } else {
-        const int blocks = (2 * n + 63) / 64;
+        const int blocks = (2 * n + kRunsPerWave - 1) / kRunsPerWave;
         hipLaunchKernelGGL(k_point_ich_many, dim3(blocks), dim3(64), 0, stream, a);
     }
     PT_TRY(hipGetLastError());

@@ -1224,7 +1224,8 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
     if (!e.empty()) return svg_set_error(SVG_E_MEM, e);
     PointWork& w = *g->point;
     const int hopCap = 4 * (hm.nf + hm.nv) + 16;
-    const int S = std::min(64, std::max(1, nq));
+    // graph stages share the solver's rounds: 256 slots (2 MB each at V = 72K)
+    const int S = std::min(256, std::max(1, nq));
     SolverState* st = nullptr;
     rc = get_solver(g, S, &st);
     if (rc) return rc;
