@@ -146,6 +146,42 @@ def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msa
     return out, (arrival, reqs, sssd_out)
 
 
+def point_workload(xyz, tri, n, seed=5):
+    """n surface-point pairs: half with f1 a face index within 40 of f0 (nearby
+    on the generated grids), half uniform; points Dirichlet(2,2,2) in their faces."""
+    rng = np.random.default_rng(seed)
+    F = len(tri)
+    f0 = rng.integers(F, size=n)
+    f1 = rng.integers(F, size=n)
+    near = rng.random(n) < 0.5
+    f1[near] = (f0[near] + rng.integers(1, 40, size=int(near.sum()))) % F
+    w0 = rng.dirichlet([2.0, 2.0, 2.0], size=n)
+    w1 = rng.dirichlet([2.0, 2.0, 2.0], size=n)
+    pts = np.stack([np.einsum("ij,ijk->ik", w0, xyz[tri[f0]]), np.einsum("ij,ijk->ik", w1, xyz[tri[f1]])], 1)
+    return np.stack([f0, f1], 1).astype(np.int32), pts
+
+
+def point_queries(g, xyz, tri, n, n_single=16):
+    """Surface-point SSSD (SVG.cu:253-352): one batch of n queries, then
+    n_single queries one at a time."""
+    faces, pts = point_workload(xyz, tri, n)
+    g.SolveSSSDPointsBatch(faces[:64], pts[:64])          # buffers and solver slots
+    t0 = time.perf_counter()
+    b = g.SolveSSSDPointsBatch(faces, pts)
+    dt = time.perf_counter() - t0
+    lat = {0: [], 1: []}
+    for i in range(n_single):
+        t1 = time.perf_counter()
+        r = g.SolveSSSDPoints(int(faces[i, 0]), pts[i, 0], int(faces[i, 1]), pts[i, 1])
+        lat[max(r.route, 0)].append((time.perf_counter() - t1) * 1e3)
+    routes = np.bincount(b["route"] + 1, minlength=3)
+    return {"n": n, "queries_per_s": n / dt, "seconds": dt,
+            "routes": {"direct": int(routes[1]), "graph": int(routes[2]), "none": int(routes[0])},
+            "single_ms_p50": {"direct": float(np.median(lat[0])) if lat[0] else None,
+                              "graph": float(np.median(lat[1])) if lat[1] else None},
+            "workload": "half near (face index within 40), half uniform; Dirichlet(2,2,2) points"}, faces, pts
+
+
 def calib_copy(mb):
     """Device-to-device copy of a known size, to scale hardware byte counters."""
     import ctypes as C
@@ -189,6 +225,8 @@ def main():
                     help="submit every request at once (queue sized to hold them): throughput, not latency")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (byte-counter calibration)")
+    ap.add_argument("--points", type=int, default=2048,
+                    help="surface-point queries in the batch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -301,6 +339,14 @@ def main():
                                                        "chain_rows", "p50_ms", "p99_ms", "achieved_qps")}
                                     for r in runs]
 
+    # ---- surface-point queries
+    pq_faces = None
+    if args.points > 0:
+        pq, pq_faces, pq_pts = point_queries(g_keep, xyz, tri, args.points)
+        res["point_queries"] = pq
+        log(f"point queries: {pq['queries_per_s']:.0f}/s over {pq['n']} ({pq['routes']}); "
+            f"single p50 {pq['single_ms_p50']} ms")
+
     # ---- CPU baseline: the oracle on a bounded sample of the same build
     if not args.no_cpu:
         import oracle as O
@@ -319,6 +365,26 @@ def main():
                                "kind": "port",
                                "sample": f"oracle build of sources [0,{n}) of {V} "
                                          f"({secs:.1f} s on {cores} threads)"}
+        if pq_faces is not None:
+            # the oracle's point queries on the same graph, one core, ~3 s
+            Vg, Kg, _ = g_keep.info()
+            tails = np.empty(Vg, np.int32)
+            adj = np.empty(Vg * Kg, np.int32)
+            w = np.empty(Vg * Kg, np.float64)
+            S.check(S.lib().svg_graph_to_host(g_keep.g, tails.ctypes.data, adj.ctypes.data, w.ctypes.data,
+                                              None, None, None, None))
+            og = O.Graph.from_arrays(xyz, adj, w, tails, Kg)
+            del adj, w
+            t0 = time.perf_counter()
+            m_done = 0
+            while m_done < len(pq_faces) and (time.perf_counter() - t0 < 3.0 or m_done < 4):
+                f0, f1 = (int(x) for x in pq_faces[m_done])
+                O.sssd_point(om, og, None, Kg, f0, pq_pts[m_done, 0], f1, pq_pts[m_done, 1])
+                m_done += 1
+            secs_p = time.perf_counter() - t0
+            res["point_queries"]["cpu_baseline"] = {
+                "value": m_done / secs_p, "unit": "queries/s", "cores": 1, "kind": "port",
+                "sample": f"oracle on the first {m_done} queries of the batch ({secs_p:.1f} s, one thread)"}
     print(json.dumps(res))
 
 

@@ -128,6 +128,21 @@ public:
         check(svg_sssd(g_, s, t, &d, nullptr, 0, nullptr));
         return d;
     }
+    // SVG.cu:253-352: distance between point p0 of face f0 and point p1 of
+    // face f1 (xyz[3] each); res receives geodDist and the key points,
+    // *saddleVertNearDst the last graph vertex (-1: answered directly).
+    void SolveSSSD(int f0, const double* p0, int f1, const double* p1, SVGNode* res, int* saddleVertNearDst)
+    {
+        double d = 0.0, kx[2] = {0.0, 0.0};
+        int32_t ke[2] = {-1, -1}, last = -1;
+        int route = 0;
+        check(svg_sssd_point(g_, f0, p0, f1, p1, &d, ke, kx, &last, &route, nullptr, 0, nullptr));
+        res->adjNode = -1;
+        res->geodDist = d;
+        res->nextToSrcEdge = (unsigned)ke[0]; res->nextToDstEdge = (unsigned)ke[1];
+        res->nextToSrcX = kx[0]; res->nextToDstX = kx[1];
+        if (saddleVertNearDst) *saddleVertNearDst = last;
+    }
     void SolveMSAD(const int* sources, const double* weights, int Ns, std::vector<GraphDistInfo>& out)
     {                                                  // SVG.cu:354-367
         int V = 0;

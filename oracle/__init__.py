@@ -194,6 +194,18 @@ class Graph:
         self.tails = np.ascontiguousarray(tails, dtype=np.int32)
 
     @classmethod
+    def from_arrays(cls, xyz, adj, w, tails, K: int):
+        """From flat K-stride arrays (adj[V*K], w[V*K], tails[V])."""
+        g = cls.__new__(cls)
+        g.xyz = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        g.V = len(tails)
+        g.K = K
+        g.adj = np.ascontiguousarray(adj, dtype=np.int32).ravel()
+        g.w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+        g.tails = np.ascontiguousarray(tails, dtype=np.int32)
+        return g
+
+    @classmethod
     def from_csr(cls, xyz, row_ptr, adj, w):
         V = len(row_ptr) - 1
         deg = np.diff(row_ptr).astype(np.int32)
