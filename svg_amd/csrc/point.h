@@ -37,6 +37,8 @@ struct PointWork {
     PointRes* res = nullptr;             // device results, per query
     int32_t* fDev = nullptr;             // faces (f0, f1) per query
     double* pDev = nullptr;              // points (p0, p1) per query
+    int32_t* jobsDev = nullptr;          // (query, slot) pairs of a join launch
+    int32_t* jobsHost = nullptr;         // pinned staging of the pairs
     int ldsOk = -1;                      // a run's heaps and vertex table fit a block's LDS (-1: not checked)
     std::vector<uint8_t> hostVt;         // host copy of one p0 vertex table
 };
@@ -55,11 +57,11 @@ std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* face
 // slot order (SVG.cu:283-291).
 std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist);
 
-// Joins query q's graph stage (dist bits / parents of V vertices, device)
-// with its p1 run and backtraces both ends, into its device result
-// (asynchronous on stream).
-std::string point_join(PointWork& w, const DevMesh& m, int q, const unsigned long long* gdist,
-                       const int32_t* gparent, int hopCap, hipStream_t stream);
+// Joins finished graph stages with their p1 runs and backtraces both ends,
+// into the device results (asynchronous on stream).  jobs = (query, solver
+// slot) pairs; gdist / gparent are the solver's arrays (slot-major, V each).
+std::string point_join(PointWork& w, const DevMesh& m, const std::vector<int32_t>& jobs,
+                       const unsigned long long* gdist, const int32_t* gparent, int hopCap, hipStream_t stream);
 
 // Copies the first n device results to out.
 std::string point_results(PointWork& w, int n, hipStream_t stream, PointRes* out);

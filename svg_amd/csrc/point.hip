@@ -163,9 +163,16 @@ __global__ void __launch_bounds__(64) k_point_ich_many(PointArgs a)
 
 constexpr int kJoinThreads = 256;
 
-__global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, int q, const unsigned long long* gdist,
-                                                              const int32_t* gparent)
+// One block per finished graph stage: jobs[2b] = query, jobs[2b+1] = its
+// solver slot (graph arrays at slot * V).
+__global__ void __launch_bounds__(kJoinThreads) k_point_join(PointArgs a, const int32_t* jobs,
+                                                              const unsigned long long* gdistAll,
+                                                              const int32_t* gparentAll)
 {
+    const int q = jobs[2 * blockIdx.x];
+    const size_t sb = (size_t)jobs[2 * blockIdx.x + 1] * (size_t)a.nv;
+    const unsigned long long* gdist = gdistAll + sb;
+    const int32_t* gparent = gparentAll + sb;
     // min over the p1 table's slots of graph + ICH distance; ties keep the
     // lowest slot (the reference's strict-< scan in slot order)
     __shared__ double sBest[kJoinThreads];
@@ -287,6 +294,8 @@ std::string point_alloc(PointWork& w, int K, int cap)
     PT_TRY(hipMalloc(&w.res, sizeof(PointRes) * (size_t)cap));
     PT_TRY(hipMalloc(&w.fDev, sizeof(int32_t) * 2 * (size_t)cap));
     PT_TRY(hipMalloc(&w.pDev, sizeof(double) * 6 * (size_t)cap));
+    PT_TRY(hipMalloc(&w.jobsDev, sizeof(int32_t) * 2 * (size_t)cap));
+    PT_TRY(hipHostMalloc(&w.jobsHost, sizeof(int32_t) * 2 * (size_t)cap, hipHostMallocDefault));
     w.hostVt.resize(sizeof(PVertItem) * (w.vsz + 1));
     return std::string();
 }
@@ -294,9 +303,10 @@ std::string point_alloc(PointWork& w, int K, int cap)
 void point_free(PointWork& w)
 {
     hipFree(w.sp); hipFree(w.vt); hipFree(w.wq); hipFree(w.pq);
-    hipFree(w.stored); hipFree(w.res); hipFree(w.fDev); hipFree(w.pDev);
+    hipFree(w.stored); hipFree(w.res); hipFree(w.fDev); hipFree(w.pDev); hipFree(w.jobsDev);
+    if (w.jobsHost) hipHostFree(w.jobsHost);
     w.sp = w.vt = w.wq = w.pq = w.stored = nullptr;
-    w.res = nullptr; w.fDev = nullptr; w.pDev = nullptr;
+    w.res = nullptr; w.fDev = nullptr; w.pDev = nullptr; w.jobsDev = nullptr; w.jobsHost = nullptr;
     w.K = 0; w.cap = 0;
 }
 
@@ -348,11 +358,19 @@ std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int
     return std::string();
 }
 
-std::string point_join(PointWork& w, const DevMesh& m, int q, const unsigned long long* gdist,
-                       const int32_t* gparent, int hopCap, hipStream_t stream)
+std::string point_join(PointWork& w, const DevMesh& m, const std::vector<int32_t>& jobs,
+                       const unsigned long long* gdist, const int32_t* gparent, int hopCap, hipStream_t stream)
 {
+    const int n = (int)(jobs.size() / 2);
+    if (n == 0) return std::string();
+    if (n > w.cap) return "too many joins";
+    // the job list travels through pinned memory; the previous batch's
+    // copy has completed once its join kernel has (same stream)
+    PT_TRY(hipStreamSynchronize(stream));
+    memcpy(w.jobsHost, jobs.data(), sizeof(int32_t) * jobs.size());
+    PT_TRY(hipMemcpyAsync(w.jobsDev, w.jobsHost, sizeof(int32_t) * jobs.size(), hipMemcpyHostToDevice, stream));
     PointArgs a = make_args(w, m, hopCap, 0);
-    hipLaunchKernelGGL(k_point_join, dim3(1), dim3(kJoinThreads), 0, stream, a, q, gdist, gparent);
+    hipLaunchKernelGGL(k_point_join, dim3(n), dim3(kJoinThreads), 0, stream, a, w.jobsDev, gdist, gparent);
     PT_TRY(hipGetLastError());
     return std::string();
 }

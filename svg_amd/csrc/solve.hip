@@ -1262,11 +1262,16 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
             int running = 0;
             e = solver_round(*st, g->d, dm, running);
             if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+            std::vector<int32_t> jobs;
             for (int sl = 0; sl < st->S; ++sl) {
                 if (slotQuery[sl] == -1 || !st->spHost[sl].done) continue;
-                const size_t sb = (size_t)sl * st->V;
-                e = point_join(w, dm, slotQuery[sl], st->dist + sb, st->parent + sb, hopCap, st->stream);
-                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                jobs.push_back(slotQuery[sl]);
+                jobs.push_back(sl);
+            }
+            e = point_join(w, dm, jobs, st->dist, st->parent, hopCap, st->stream);
+            if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+            for (int sl = 0; sl < st->S; ++sl) {
+                if (slotQuery[sl] == -1 || !st->spHost[sl].done) continue;
                 if (nq == 1 && path_cap > 0) {
                     PointRes r1;
                     e = point_results(w, 1, st->stream, &r1);
