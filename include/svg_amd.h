@@ -134,15 +134,19 @@ int svg_msad(svg_graph_t g, const int32_t *src, const double *w, int ns, double 
  * nextToDstEdge/X; *last_vert the last graph vertex (saddleVertNearDst, -1
  * when direct); *route 0 direct, 1 through the graph, -1 nothing joins the
  * points (*dist = DBL_MAX).  path (optional, cap entries) receives the graph
- * vertices from last_vert back to the first. */
+ * vertices from last_vert back to the first.  A point on a vertex of its
+ * face makes the propagation's keys NaN (the reference loops there): the
+ * call returns SVG_E_ARG. */
 int svg_sssd_point(svg_graph_t g, int f0, const double *p0, int f1, const double *p1, double *dist,
                    int32_t *key_edge, double *key_x, int32_t *last_vert, int *route, int32_t *path,
                    int path_cap, int *path_len);
 /* Many surface-point queries at once (no reference counterpart; the
  * reference answers one per call): faces[2*nq] = (f0, f1), points[6*nq] =
  * (p0, p1) per query; dist[nq]; key_edge[2*nq], key_x[2*nq], last_vert[nq],
- * route[nq] (may be NULL) as svg_sssd_point.  Each query's ICH runs are one
- * thread each; the graph stages share the solver's slots. */
+ * route[nq] (may be NULL) as svg_sssd_point; a query whose propagation
+ * stalls (a point on a vertex of its face) gets route -2 and a NaN distance.
+ * Each query's ICH runs are one thread each; the graph stages share the
+ * solver's slots. */
 int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t *faces, const double *points, double *dist,
                          int32_t *key_edge, double *key_x, int32_t *last_vert, int32_t *route);
 /* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */

@@ -175,6 +175,8 @@ def sssd_point(mesh: Mesh, rows, tails, K: int, f0: int, p0, f1: int, p1,
                                  f0, np.ascontiguousarray(p0, dtype=np.float64).ravel(), f1,
                                  np.ascontiguousarray(p1, dtype=np.float64).ravel(), dout, iout,
                                  gdist, gpar, stat)
+    if rc == -2:
+        raise RuntimeError("a point run stalled on unordered keys (a point on a vertex of its face)")
     if rc:
         raise ValueError("bad argument")
     return dict(dist=float(dout[0]), src_x=float(dout[1]), dst_x=float(dout[2]),

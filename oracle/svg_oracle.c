@@ -416,6 +416,7 @@ typedef struct {
     int keptFace;                     /* keptFaces[0], -1: none */
     OWindow stored[O_STORED_WIN];     /* storedWindows */
     int nStored;
+    int stalled;                      /* the queue tops were unordered (NaN keys) */
 } OIch;
 
 static void ich_init(OIch *s, const OMesh *m, int K, unsigned splitCoef, unsigned vertCoef)
@@ -463,6 +464,7 @@ static void ich_clear(OIch *s)
     s->srcFace = -1;
     s->keptFace = -1;
     s->nStored = 0;
+    s->stalled = 0;
     s->totalCalcVertNum = 0;
 }
 
@@ -1116,6 +1118,11 @@ static void ich_execute_point(OIch *s, int K)
             OWinQ_pop(&s->winQ);
             s->cnt[C_WIN_POP]++;
             if (win.level > m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
+            if (win.minDist != win.minDist) {          /* NaN: see the stall below */
+                s->cnt[C_ERRORS]++;
+                s->stalled = 1;
+                break;
+            }
             int tw = m->e[win.edgeID].twin;
             if (tw != -1 && s->keptFace >= 0 && m->e[tw].face == s->keptFace &&
                 s->nStored < O_STORED_WIN)
@@ -1127,6 +1134,14 @@ static void ich_execute_point(OIch *s, int K)
             s->cnt[C_PSEUDO_POP]++;
             if (pw.level >= (unsigned)m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
             gen_sub_wins(s, &pw);
+        } else {
+            /* neither branch: a NaN key.  A point on a vertex of its face
+             * puts the source on a window's line and its windows' keys become
+             * NaN; the reference then loops or expands NaN windows until
+             * their level passes F.  The run stops at the first NaN key. */
+            s->cnt[C_ERRORS]++;
+            s->stalled = 1;
+            break;
         }
         if (K != -1 && s->totalCalcVertNum >= K) break;
     }
@@ -1541,7 +1556,8 @@ int oracle_ich_point(const OMesh *m, int f, const double *p, int K, unsigned spl
  * Decisions where the reference is undefined: SVGNode X fields start at 0.0;
  * a key-point edge without a twin is kept as it is when the reference would
  * take its twin; no join vertex gives route -1 (the reference backtraces
- * vertex 0xFFFFFFFF).
+ * vertex 0xFFFFFFFF).  Returns -2 when a run stalls on unordered keys (a
+ * point on a vertex of its face), where the reference loops.
  */
 int oracle_sssd_point(const OMesh *m, int K, unsigned splitCoef, unsigned vertCoef,
                       int Kg, const int *adj, const double *w, const int *tails,
@@ -1564,12 +1580,13 @@ int oracle_sssd_point(const OMesh *m, int K, unsigned splitCoef, unsigned vertCo
     memset(cnt, 0, sizeof(cnt));
 
     double resD = 0.0, resSX = 0.0, resDX = 0.0;
-    int resSE = -1, resDE = -1, last = -1, route = 0;
+    int resSE = -1, resDE = -1, last = -1, route = 0, rcode = 0;
 
     ich_clear(a);
     a->srcFace = f0; memcpy(a->spos, p0, sizeof(a->spos));
     a->keptFace = f1;
     ich_execute_point(a, K);
+    if (a->stalled) { rcode = -2; goto done; }
     OPath fp;
     fp.srcX = resSX; fp.dstX = resDX; fp.srcEdge = fp.dstEdge = -1; fp.srcId = 0;
     resD = face_path_to(a, f1, p1, &fp);
@@ -1593,6 +1610,7 @@ int oracle_sssd_point(const OMesh *m, int K, unsigned splitCoef, unsigned vertCo
         ich_clear(b);
         b->srcFace = f1; memcpy(b->spos, p1, sizeof(b->spos));
         ich_execute_point(b, K);
+        if (b->stalled) { rcode = -2; goto done; }
         double minDist = DBL_MAX;
         for (unsigned i = 0; i < b->vert.size; ++i) {
             int vi = b->vert.data[i].index;
@@ -1636,11 +1654,12 @@ int oracle_sssd_point(const OMesh *m, int K, unsigned splitCoef, unsigned vertCo
     }
     dout[0] = resD; dout[1] = resSX; dout[2] = resDX;
     iout[0] = resSE; iout[1] = resDE; iout[2] = last; iout[3] = route;
+done:
     ich_destroy(a); ich_destroy(b); free(a); free(b);
     if (!gdist) free(gd);
     if (!gparent) free(gp);
     free(sid); free(q.a); free(idx);
-    return 0;
+    return rcode;
 }
 
 /* Many SSSD queries, timed, for the CPU baseline: one query per thread. */

@@ -124,8 +124,10 @@ struct Ich {
     int K;
     bool pathPassVert;
     int srcFace;            // sourcePointFace: -1 for a vertex source
+    bool stalled;           // execute_point stopped on unordered (NaN) queue keys
 
-    SVG_HD Ich(P& pol, int src, int K_) : p(pol), source(src), total(0), K(K_), pathPassVert(false), srcFace(-1) {}
+    SVG_HD Ich(P& pol, int src, int K_)
+        : p(pol), source(src), total(0), K(K_), pathPassVert(false), srcFace(-1), stalled(false) {}
 
     SVG_HD void push_win(const Win& w) { p.count(CNT_WIN_PUSH, 1); p.wpush(w); }
     SVG_HD void push_pseudo(const Pseudo& q) { p.count(CNT_PSEUDO_PUSH, 1); p.ppush(q); }
@@ -737,6 +739,11 @@ struct Ich {
                 p.wpop();
                 p.count(CNT_WIN_POP, 1);
                 if (top.level > nf) continue;
+                if (top.minDist != top.minDist) {      // NaN key: stop, as below
+                    p.count(CNT_ERRORS, 1);
+                    stalled = true;
+                    break;
+                }
                 if (top.twin != -1 && keptFace >= 0 && top.twin / 3 == keptFace && nStored < storeCap)
                     stored[nStored++] = top;
                 propagate(top);
@@ -746,7 +753,11 @@ struct Ich {
                 if (ptop.level >= nf) continue;
                 gen_sub_wins(ptop);
             } else if (haveTop || havePtop) {
-                p.count(CNT_ERRORS, 1);                  // unordered keys (NaN)
+                // unordered keys (NaN: a point on a vertex of its face puts
+                // the source on a window's line); the reference loops here.
+                // The run stops at the first NaN key it meets.
+                p.count(CNT_ERRORS, 1);
+                stalled = true;
                 break;
             }
             if (K != -1 && total >= K) break;

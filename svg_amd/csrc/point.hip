@@ -117,7 +117,7 @@ __device__ void point_run(const PointArgs& a, int run, PlainPolicy& p)
         res->srcEdge = r.srcEdge; res->dstEdge = r.dstEdge;
         res->lastVert = -1; res->firstVert = -1; res->route = 0;
     }
-    atomicOr(&res->status, p.flags);
+    atomicOr(&res->status, p.flags | (ich.stalled ? (side == 0 ? SF_STALL_P0 : SF_STALL_P1) : 0u));
 }
 
 // LDS bytes of a run's window heap, pseudo-source heap and vertex table.

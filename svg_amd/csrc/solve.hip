@@ -1236,10 +1236,23 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
         const int n = std::min(chunk, nq - c0);
         e = point_ich(w, dm, n, faces + 2 * (size_t)c0, pts + 6 * (size_t)c0, hopCap, st->stream, out + c0);
         if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        // a run that stalled (a point on a vertex of its face) answers
+        // nothing: route -2, distance NaN
         std::vector<int> todo;
-        for (int q = 0; q < n; ++q)
-            if (out[c0 + q].srcEdge == -1) todo.push_back(q);
-        if (todo.empty()) continue;
+        std::vector<int> stalled;
+        for (int q = 0; q < n; ++q) {
+            const uint32_t stt = out[c0 + q].status;
+            if ((stt & SF_STALL_P0) || (out[c0 + q].srcEdge == -1 && (stt & SF_STALL_P1))) stalled.push_back(q);
+            else if (out[c0 + q].srcEdge == -1) todo.push_back(q);
+        }
+        auto mark_stalled = [&]() {
+            for (int q : stalled) {
+                PointRes& r = out[c0 + q];
+                r.dist = std::nan(""); r.route = -2;
+                r.srcEdge = r.dstEdge = -1; r.srcX = r.dstX = 0.0; r.lastVert = r.firstVert = -1;
+            }
+        };
+        if (todo.empty()) { mark_stalled(); continue; }
         std::vector<int> slotQuery((size_t)st->S, -1);
         size_t next = 0, done = 0;
         while (done < todo.size()) {
@@ -1290,6 +1303,7 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
         }
         e = point_results(w, n, st->stream, out + c0);
         if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        mark_stalled();
     }
     return SVG_OK;
 }
@@ -1306,6 +1320,9 @@ extern "C" int svg_sssd_point(svg_graph_t g, int f0, const double* p0, int f1, c
     PointRes r;
     int rc = point_queries(g, 1, faces, pts, &r, path, path_cap, path_len);
     if (rc) return rc;
+    if (r.route == -2)
+        return svg_set_error(SVG_E_ARG, "the propagation from a point stalled on unordered keys "
+                                        "(a point on a vertex of its face)");
     *dist = r.dist;
     if (key_edge) { key_edge[0] = r.srcEdge; key_edge[1] = r.dstEdge; }
     if (key_x) { key_x[0] = r.srcX; key_x[1] = r.dstX; }

@@ -27,6 +27,8 @@ enum : uint32_t {
     SF_CAPACITY = 8,        // the batched tables ran out of room: recomputed on the plain path
     SF_ERROR = 16,          // hop cap / unreachable branch
     SF_REDONE = 32,         // row produced by the exact plain path after a batched flag
+    SF_STALL_P0 = 64,       // point query: the p0 run stopped on unordered (NaN) keys
+    SF_STALL_P1 = 128,      // point query: the p1 run did
 };
 
 }  // namespace svg

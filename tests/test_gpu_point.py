@@ -20,6 +20,7 @@ MESHES = {
     "sphere642_K40": (lambda: mg.noisy_sphere(seed=9, splits4=3, centroid_split=False), 40),
     "torus40x24_K80": (lambda: mg.bumpy_torus(nu=40, nv=24), 80),
     "grid12_full": (lambda: mg.planar_grid(n=12, jitter=0.3, seed=7), 144),
+    "disk_K40": (lambda: mg.planar_disk(), 40),          # open: boundary faces and vertices
 }
 
 
@@ -39,6 +40,12 @@ def graph(request):
     om = O.Mesh(xyz, tri)
     twin = om.export()["twin"]
     return request.param, xyz, tri, K, g, om, rows, h["tails"], twin
+
+
+def _same(a, b):
+    """Element-wise equality of two result tuples, NaN equal to NaN (a point
+    on an edge of its face can give a NaN key-point position on both sides)."""
+    return all(x == y or (x != x and y != y) for x, y in zip(a, b))
 
 
 def _pairs(rng, tri, twin, n):
@@ -126,3 +133,37 @@ def test_point_batch_matches_oracle(graph):
         r = g.SolveSSSDPoints(int(faces[i, 0]), pts[i, 0], int(faces[i, 1]), pts[i, 1])
         assert (r.dist, r.src_edge, r.dst_edge, r.last_vert) == (b["dist"][i], b["src_edge"][i],
                                                                   b["dst_edge"][i], b["last_vert"][i])
+
+
+def test_point_special_positions_match_oracle(graph):
+    """Same face, points on an edge of their face, on boundary faces; a point
+    on a vertex stalls both (NaN window keys) and is refused by both."""
+    name, xyz, tri, K, g, om, rows, tails, twin = graph
+    rng = np.random.default_rng(31)
+    boundary = [f for f in range(len(tri)) if (twin[3 * f:3 * f + 3] < 0).any()]
+    cases = []
+    for i in range(12):
+        f0 = int(rng.integers(len(tri)))
+        cases.append((f0, rng.dirichlet([2, 2, 2]), f0, rng.dirichlet([2, 2, 2])))        # same face
+        f1 = int(rng.integers(len(tri)))
+        cases.append((f0, np.array([0.5, 0.5, 0.0]), f1, np.array([0.0, 0.3, 0.7])))     # on edges
+        if boundary:
+            fb = int(rng.choice(boundary))
+            cases.append((fb, rng.dirichlet([2, 2, 2]), f0, rng.dirichlet([2, 2, 2])))
+    for f0, w0, f1, w1 in cases:
+        p0, p1 = w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]
+        print(f"{name} special f{f0} {w0.round(2)} -> f{f1} {w1.round(2)}", flush=True)
+        r = g.SolveSSSDPoints(f0, p0, f1, p1)
+        o = O.sssd_point(om, rows, tails, K, f0, p0, f1, p1)
+        got = (r.route, r.dist, r.src_edge, r.dst_edge, r.last_vert, r.src_x, r.dst_x)
+        want = (o["route"], o["dist"], o["src_edge"], o["dst_edge"], o["last_vert"], o["src_x"], o["dst_x"])
+        assert _same(got, want), (got, want)
+    f0, f1 = 0, len(tri) // 2
+    p0, p1 = xyz[tri[f0, 0]], xyz[tri[f1, 1]]
+    with pytest.raises(RuntimeError):
+        O.sssd_point(om, rows, tails, K, f0, p0, f1, p1)
+    with pytest.raises(S.SvgError):
+        g.SolveSSSDPoints(f0, p0, f1, p1)
+    b = g.SolveSSSDPointsBatch(np.array([[f0, f1], [f0, f1]], np.int32),
+                               np.array([[p0, p1], [w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]]]))
+    assert b["route"][0] == -2 and np.isnan(b["dist"][0]) and b["route"][1] >= 0

@@ -204,3 +204,26 @@ def test_sssd_point_graph_route_composes_pinned_parts():
         assert np.array_equal(D0, r["gdist"])
         assert r["dist"] == np.min(D0[r1] + d1[r1])
     assert seen[0] > 5 and seen[1] > 5
+
+
+def test_point_on_a_vertex_stalls_and_edge_points_are_exact(planar_full):
+    """A point on a vertex of its face puts the source on a window's line (NaN
+    keys, where the reference loops): the query is refused.  A point on an
+    edge of its face is an ordinary source: direct answers stay Euclidean to
+    1e-8 (the face-point placement, ICHDevice.cu:186-197, takes a zero
+    Heron area as the square root of rounding noise)."""
+    xyz, tri, m, K, rows, tails = planar_full
+    with pytest.raises(RuntimeError):
+        O.sssd_point(m, rows, tails, K, 3, xyz[tri[3, 0]], 100, xyz[tri[100, 1]])
+    rng = np.random.default_rng(8)
+    n = 0
+    for _ in range(40):
+        f0, f1 = (int(x) for x in rng.integers(0, len(tri), 2))
+        p0 = np.array([0.5, 0.5, 0.0]) @ xyz[tri[f0]]
+        p1 = np.array([0.0, 0.3, 0.7]) @ xyz[tri[f1]]
+        r = O.sssd_point(m, rows, tails, K, f0, p0, f1, p1)
+        if r["route"] == 0:
+            eu = np.linalg.norm(p1 - p0)
+            assert abs(r["dist"] - eu) <= 1e-8 * eu
+            n += 1
+    assert n > 20
