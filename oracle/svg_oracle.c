@@ -60,7 +60,8 @@
 #define O_WINPQ_SIZE 1024                  /* SVG.cuh:13 */
 #define O_PSEUDOWINPQ_SIZE 1024            /* SVG.cuh:14 */
 #define O_NONE 0xFFFFFFFFu
-#define O_STORED_WIN 100                   /* STORED_WIN_BUF_SIZE (SVG.cuh:15), Q2 */
+#define O_STORED_WIN 100                   /* STORED_WIN_BUF_SIZE (SVG.cuh:15); the reference never sets
+                                              storedWindowsSize (ICHDevice.cuh:150) */
 
 /* ------------------------------------------------------------------------ */
 /* acos, fdlibm method (D4)                                                  */
@@ -1100,7 +1101,7 @@ static void ich_execute_point(OIch *s, int K)
             OWinQ_pop(&s->winQ);
             s->cnt[C_WIN_POP]++; s->cnt[C_WIN_STALE]++;
         }
-        /* Q3: the stale test of the pseudo-source queue is guarded by the
+        /* The stale test of the pseudo-source queue is guarded by the
          * window queue's top (ICHDevice.cu:84), i.e. slot 1 of its array,
          * which keeps the last window popped once the queue is empty.  While
          * that window descends from the point (pseudo-source id V) stale
