@@ -66,6 +66,46 @@ struct PointArgs {
     int nRuns;
 };
 
+// The reference-layout policy with pops that carry only the moving item's
+// key through the sift-down: the last item stays in its slot (the sift only
+// writes parent slots below it) and is copied once at the end.  Same heap
+// contents as PlainPolicy's pops, without an 80-byte item live across the
+// loop (which the compiler keeps in scratch memory).
+struct PointPolicy : PlainPolicy {
+    __device__ void wpop()
+    {
+        --wtail;
+        const double key = wq[wtail].key;
+        int cur = 1, l = 2, r = 3, mc = -1;
+        if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+        else if (l < wtail) mc = l;
+        while (mc != -1 && wq[mc].key < key) {
+            wq[cur] = wq[mc];
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+            else if (l < wtail) mc = l;
+            else mc = -1;
+        }
+        if (cur != wtail) wq[cur] = wq[wtail];
+    }
+    __device__ void ppop()
+    {
+        --ptail;
+        const double key = pq[ptail].key;
+        int cur = 1, l = 2, r = 3, mc = -1;
+        if (r < ptail) mc = pq[l].key < pq[r].key ? l : r;
+        else if (l < ptail) mc = l;
+        while (mc != -1 && pq[mc].key < key) {
+            pq[cur] = pq[mc];
+            cur = mc; l = cur * 2; r = cur * 2 + 1;
+            if (r < ptail) mc = pq[l].key < pq[r].key ? l : r;
+            else if (l < ptail) mc = l;
+            else mc = -1;
+        }
+        if (cur != ptail) pq[cur] = pq[ptail];
+    }
+};
+
 __device__ void make_policy(const PointArgs& a, int run, PlainPolicy& p)
 {
     p.E = a.edges; p.Vt = a.verts; p.cA = a.cornerA; p.cB = a.cornerB;
@@ -92,12 +132,12 @@ __device__ void face_dists(const PointArgs& a, int f, const double* q, int fv[3]
 }
 
 // One ICH run from its point; the p0 run then answers directly through f1.
-__device__ void point_run(const PointArgs& a, int run, PlainPolicy& p)
+__device__ void point_run(const PointArgs& a, int run, PointPolicy& p)
 {
     const int q = run >> 1, side = run & 1;
     const int f = a.f[2 * q + side];
     p.clear();
-    Ich<PlainPolicy> ich(p, -1, a.K);
+    Ich<PointPolicy> ich(p, -1, a.K);
     ich.srcFace = f;
     int fv[3];
     double d[3];
@@ -135,7 +175,7 @@ __global__ void __launch_bounds__(64) k_point_ich_lds(PointArgs a)
 {
     extern __shared__ double ldsBuf[];
     const int run = blockIdx.x;
-    PlainPolicy p;
+    PointPolicy p;
     make_policy(a, run, p);
     PVertItem* dst = p.vt;
     unsigned char* b = (unsigned char*)ldsBuf;
@@ -155,7 +195,7 @@ __global__ void __launch_bounds__(64) k_point_ich_many(PointArgs a)
     if ((int)threadIdx.x >= kRunsPerWave) return;
     const int total = gridDim.x * kRunsPerWave;
     for (int run = blockIdx.x * kRunsPerWave + threadIdx.x; run < a.nRuns; run += total) {
-        PlainPolicy p;
+        PointPolicy p;
         make_policy(a, run, p);
         point_run(a, run, p);
     }
