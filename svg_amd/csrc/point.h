@@ -53,9 +53,11 @@ void point_free(PointWork& w);
 std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* faces, const double* pts, int hopCap,
                       hipStream_t stream, PointRes* out);
 
-// Graph seeds of query q: every vertex its p0 run reached, in its table's
-// slot order (SVG.cu:283-291).
-std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist);
+// Every vertex the run of query q from side 0 (p0: the graph seeds,
+// SVG.cu:283-291) or side 1 (p1: the exits the join scans) reached, with its
+// distance, in its table's slot order.
+std::string point_seeds(PointWork& w, int q, int side, hipStream_t stream, std::vector<int>& src,
+                        std::vector<double>& dist);
 
 // Joins finished graph stages with their p1 runs and backtraces both ends,
 // into the device results (asynchronous on stream).  jobs = (query, solver

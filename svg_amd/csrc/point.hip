@@ -383,9 +383,10 @@ std::string point_ich(PointWork& w, const DevMesh& m, int n, const int32_t* face
     return std::string();
 }
 
-std::string point_seeds(PointWork& w, int q, hipStream_t stream, std::vector<int>& src, std::vector<double>& dist)
+std::string point_seeds(PointWork& w, int q, int side, hipStream_t stream, std::vector<int>& src,
+                        std::vector<double>& dist)
 {
-    const PVertItem* dvt = (const PVertItem*)w.vt + (size_t)(2 * q) * (w.vsz + 1);
+    const PVertItem* dvt = (const PVertItem*)w.vt + (size_t)(2 * q + side) * (w.vsz + 1);
     PT_TRY(hipMemcpyAsync(w.hostVt.data(), dvt, w.hostVt.size(), hipMemcpyDeviceToHost, stream));
     PT_TRY(hipStreamSynchronize(stream));
     const PVertItem* vt = (const PVertItem*)w.hostVt.data();

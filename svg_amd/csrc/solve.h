@@ -20,7 +20,10 @@
 
 namespace svg {
 
-enum : int { Q_SSSD = 0, Q_MSAD = 1 };
+// Q_JOIN: a multi-source Dijkstra that only has to be exact where it meets a
+// set of exit vertices with offsets (surface-point queries): it stops once no
+// frontier key can beat the best exit distance + offset found so far.
+enum : int { Q_SSSD = 0, Q_MSAD = 1, Q_JOIN = 2 };
 
 struct SlotParam {              // device-resident, one per slot
     int kind;                   // Q_SSSD / Q_MSAD, -1 = idle
@@ -36,7 +39,8 @@ struct SlotParam {              // device-resident, one per slot
     int pad;
     long long relaxed;          // arcs scanned
     long long settled;          // entries expanded
-    double result;              // SSSD: distance to the target at the last bookkeeping
+    double result;              // SSSD: distance to the target at the last bookkeeping;
+                                // JOIN: best exit distance + offset so far (an upper bound)
 };
 
 constexpr int kCBlocks = 512;     // classify blocks = deferred-list regions
@@ -97,7 +101,8 @@ struct SolverState {
     uint32_t* stamp = nullptr;            // S*V round of last improvement
     uint32_t* inq = nullptr;              // S*V frontier membership
     int32_t* srcid = nullptr;             // S*V
-    float* hdist = nullptr;               // S*V straight-line distance to the SSSD target (rounded down)
+    float* hdist = nullptr;               // S*V straight-line distance to the SSSD target (rounded down);
+                                          // JOIN: exit offsets rounded up, +inf off the exit set
     uint64_t* dl[2] = {nullptr, nullptr};   // deferred lists (kCBlocks regions), by parity
     uint64_t* pl[2] = {nullptr, nullptr};   // kPLists push lists, by parity
     ExpEnt* exp[2] = {nullptr, nullptr};  // rows of a round (kCBlocks regions), by parity
@@ -133,6 +138,8 @@ struct Admission {
     bool wantParent = false;
     std::vector<int> src;
     std::vector<double> w;
+    std::vector<int> exitV;         // Q_JOIN: exit vertices and their offsets
+    std::vector<double> exitOff;
 };
 // Seeds several slots with one init launch and one seed launch (on s.stream).
 std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vector<Admission>& adm);

@@ -71,6 +71,15 @@ struct SolveArgs {
     int chainBudget;                    // rows a wave may chain per expand pass
 };
 
+// A join slot's bound (SlotParam::result), lowered by atomicMin on its bits
+// while expand passes run.
+__device__ __forceinline__ double join_bound(const SlotParam& sp)
+{
+    const unsigned long long b =
+        __hip_atomic_load((const unsigned long long*)&sp.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)b);
+}
+
 __device__ __forceinline__ double heur(const double* __restrict__ pos, int v, int t)
 {
     // (verts[v].pos - verts[t].pos).length()  (SVG.cu:402)
@@ -136,8 +145,9 @@ __global__ void k_slot_init(unsigned long long* dist, int32_t* parent, uint32_t*
         inq[o] = 0;
         srcid[o] = -1;
         const int t = targets[k];
-        // rounded toward zero, it stays a lower bound (admissible) in 4 bytes
-        hdist[o] = t >= 0 ? __double2float_rz(heur(pos, v, t)) : 0.0f;
+        // rounded toward zero, it stays a lower bound (admissible) in 4 bytes;
+        // a join slot (t = -2) starts with no exits
+        hdist[o] = t >= 0 ? __double2float_rz(heur(pos, v, t)) : (t == -2 ? __int_as_float(0x7f800000) : 0.0f);
     }
 }
 
@@ -174,6 +184,10 @@ __device__ void book_slot(const SolveArgs& a, int q, const SlotAcc* acc, double 
         if (sp.kind == Q_SSSD) {
             const double ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
             finished = nm * (1.0 - 1e-12) > ub;
+        } else if (sp.kind == Q_JOIN) {
+            // every vertex still to settle is at least nm away: its exit sum
+            // exceeds the bound (which is at least the exact best sum)
+            finished = nm * (1.0 - 1e-12) > join_bound(sp);
         }
         if (finished) doneNew = 1;
         else thrNew = nm + sp.delta;
@@ -283,7 +297,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
             }
             const int kind = sp.kind;
             lThr[q] = t; lDone[q] = d; lKind[q] = kind; lGen[q] = sp.gen;
-            lUb[q] = kind == Q_SSSD ? __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]) : 0.0;
+            lUb[q] = kind == Q_SSSD ? __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target])
+                                    : (kind == Q_JOIN ? join_bound(sp) : 0.0);
         }
     }
     if (a.doParent) parent_part(a, a.c ^ 1, a.round - 1, a.wonPrev, a.expPrev, pre, blockIdx.x, gridDim.x);
@@ -331,6 +346,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 const SlotParam& sp = a.sp[q];
                 thr = a.thr[a.c * a.S + q]; done = a.doneP[a.c * a.S + q]; kind = sp.kind; gen = sp.gen;
                 if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[(size_t)q * a.V + sp.target]);
+                else if (kind == Q_JOIN) ub = join_bound(sp);
             }
             if (kind < 0 || (int)(e >> 48) != (gen & 0xFFFF)) {
                 // entry of an earlier query in this slot: dropped
@@ -344,6 +360,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_classify(SolveArgs a)
                 if (kind == Q_SSSD) {
                     key = dv + (double)a.hdist[o];
                     prune = key * (1.0 - 1e-12) > ub;
+                } else if (kind == Q_JOIN) {
+                    prune = dv * (1.0 - 1e-12) > ub;
                 }
                 if (prune) {
                     // no path through v can beat the answer any more
@@ -524,7 +542,9 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
         const size_t row = (size_t)vv * a.K;
         double ub = DBL_MAX;
         if (kind == Q_SSSD) ub = __longlong_as_double((long long)a.dist[sb + t]);
+        else if (kind == Q_JOIN) ub = join_bound(a.sp[qq]);
         unsigned long long rowMin = ~0ull;        // smallest key left to a later round
+        unsigned long long exitMin = ~0ull;       // JOIN: smallest exit sum this row made
         for (int b = 0; b < tl; b += 64 * kSteps) {
             int xs[kSteps];
             unsigned long long nb[kSteps], val[kSteps];
@@ -544,7 +564,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                 nb[st] = (unsigned long long)__double_as_longlong(dvv + ws[st]);
                 val[st] = a.dist[sb + xs[st]];
             }
-            if (kind == Q_SSSD) {
+            if (kind == Q_SSSD || kind == Q_JOIN) {
 #pragma unroll
                 for (int st = 0; st < kSteps; ++st) hs[st] = a.hdist[sb + xs[st]];
             } else {
@@ -566,6 +586,14 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                         const double nd = __longlong_as_double((long long)nb[st]);
                         if ((nd + (double)hs[st] * (1.0 - 1e-12)) > ub) cand &= ~(1u << st);
                     }
+            } else if (kind == Q_JOIN) {
+                // JOIN: nothing farther than the best exit sum can matter
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st)
+                    if ((cand >> st) & 1u) {
+                        const double nd = __longlong_as_double((long long)nb[st]);
+                        if (nd * (1.0 - 1e-12) > ub) cand &= ~(1u << st);
+                    }
             }
             if (!__ballot(cand != 0)) continue;
             // atomicMins; val becomes the previous distance
@@ -580,6 +608,17 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
                     else if (nb[st] == val[st] && wantParent) win |= 1u << st;
                 }
             if (wantParent) win |= imp;
+            if (kind == Q_JOIN) {
+                // an improved exit lowers the bound: distance + offset (the
+                // offset rounded up, so the bound stays at or above the exact sum)
+#pragma unroll
+                for (int st = 0; st < kSteps; ++st)
+                    if (((imp >> st) & 1u) && hs[st] < __int_as_float(0x7f800000)) {
+                        const unsigned long long xb = (unsigned long long)__double_as_longlong(
+                            __longlong_as_double((long long)nb[st]) + (double)hs[st]);
+                        exitMin = xb < exitMin ? xb : exitMin;
+                    }
+            }
             // val becomes the key (A*: distance + bound)
 #pragma unroll
             for (int st = 0; st < kSteps; ++st)
@@ -649,6 +688,13 @@ __global__ void __launch_bounds__(kSolveBlock) k_expand(SolveArgs a)
             rowMin = o < rowMin ? o : rowMin;
         }
         if (lane == 0 && rowMin != ~0ull) atomicMin(&acc[qq].minImp, rowMin);
+        if (kind == Q_JOIN) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(exitMin, off);
+                exitMin = o < exitMin ? o : exitMin;
+            }
+            if (lane == 0 && exitMin != ~0ull) atomicMin((unsigned long long*)&a.sp[qq].result, exitMin);
+        }
     };
 
     // the wave's rows, then the rows it chained (the queue grows while it is
@@ -890,15 +936,22 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
                         const int* __restrict__ targets, const int* __restrict__ wantP,
                         const int* __restrict__ gens, const int* __restrict__ off,
                         const int* __restrict__ seeds, const double* __restrict__ wts, double deltaSssd,
-                        double deltaMsad)
+                        double deltaMsad, const int* __restrict__ xoff, const int* __restrict__ xv,
+                        const double* __restrict__ xw, float* hdistW)
 {
-    if (threadIdx.x != 0) return;
     const int k = blockIdx.x;
     const int slot = slots[k], kind = kinds[k], target = targets[k], gen = gens[k];
-    const double delta = kind == Q_SSSD ? deltaSssd : deltaMsad;
     const size_t base = (size_t)slot * a.V;
+    if (kind == Q_JOIN) {
+        // exit offsets, rounded up so that sums with them bound the exact ones
+        for (int i = xoff[k] + threadIdx.x; i < xoff[k + 1]; i += blockDim.x)
+            hdistW[base + xv[i]] = __double2float_ru(xw[i]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double delta = kind == Q_SSSD ? deltaSssd : deltaMsad;
     uint64_t* front = const_cast<uint64_t*>(a.pIn);
-    double mk = DBL_MAX;
+    double mk = DBL_MAX, xb = __int_as_float(0x7f800000);
     for (int i = off[k]; i < off[k + 1]; ++i) {
         const int v = seeds[i];
         const double w = wts[i];
@@ -913,12 +966,16 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
         }
         const double key = kind == Q_SSSD ? w + heur(a.pos, v, target) : w;
         mk = key < mk ? key : mk;
+        if (kind == Q_JOIN) {
+            const double x = w + (double)hdistW[base + v];
+            xb = x < xb ? x : xb;
+        }
     }
     SlotParam p;
     p.kind = kind; p.target = target; p.wantParent = wantP[k]; p.done = 0;
     p.thr = mk + delta; p.delta = delta; p.minDeferred = ~0ull;
     p.expanded = 0; p.rounds = 0; p.gen = gen; p.pad = 0; p.relaxed = 0; p.settled = 0;
-    p.result = DBL_MAX;
+    p.result = kind == Q_JOIN ? xb : DBL_MAX;
     a.sp[slot] = p;
     a.thr[a.c * a.S + slot] = p.thr;
     a.doneP[a.c * a.S + slot] = 0;
@@ -932,19 +989,23 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const int si = s.stageIdx;
     s.stageIdx ^= 1;
     HTRY(hipEventSynchronize(s.stageEv[si]));
-    std::vector<int> ints;          // slots, kinds, targets, wantP, gens, off[n+1], seeds
-    ints.reserve((size_t)6 * n + 64);
-    size_t nseeds = 0;
+    // slots, kinds, targets, wantP, gens, off[n+1], seeds, xoff[n+1], exits
+    std::vector<int> ints;
+    ints.reserve((size_t)7 * n + 64);
+    size_t nseeds = 0, nexits = 0;
     for (const Admission& a : adm) {
         if (a.slot < 0 || a.slot >= s.S) return "slot out of range";
         if (a.src.empty() || a.src.size() != a.w.size()) return "bad seeds";
         for (int v : a.src) if (v < 0 || v >= s.V) return "source out of range";
         if (a.kind == Q_SSSD && (a.target < 0 || a.target >= s.V)) return "target out of range";
+        if (a.exitV.size() != a.exitOff.size()) return "bad exits";
+        for (int v : a.exitV) if (v < 0 || v >= s.V) return "exit out of range";
         nseeds += a.src.size();
+        nexits += a.kind == Q_JOIN ? a.exitV.size() : 0;
     }
     for (const Admission& a : adm) ints.push_back(a.slot);
     for (const Admission& a : adm) ints.push_back(a.kind);
-    for (const Admission& a : adm) ints.push_back(a.kind == Q_SSSD ? a.target : -1);
+    for (const Admission& a : adm) ints.push_back(a.kind == Q_SSSD ? a.target : (a.kind == Q_JOIN ? -2 : -1));
     for (const Admission& a : adm) ints.push_back(a.wantParent ? 1 : 0);
     for (const Admission& a : adm) {
         s.gens[a.slot] = (s.gens[a.slot] + 1) & 0xFFFF;
@@ -954,8 +1015,14 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     for (const Admission& a : adm) { ints.push_back(o); o += (int)a.src.size(); }
     ints.push_back(o);
     for (const Admission& a : adm) ints.insert(ints.end(), a.src.begin(), a.src.end());
+    const size_t xoffPos = ints.size();
+    o = 0;
+    for (const Admission& a : adm) { ints.push_back(o); o += a.kind == Q_JOIN ? (int)a.exitV.size() : 0; }
+    ints.push_back(o);
+    for (const Admission& a : adm)
+        if (a.kind == Q_JOIN) ints.insert(ints.end(), a.exitV.begin(), a.exitV.end());
     const size_t intBytes = ((ints.size() * sizeof(int) + 15) / 16) * 16;
-    const size_t bytes = intBytes + nseeds * sizeof(double);
+    const size_t bytes = intBytes + (nseeds + nexits) * sizeof(double);
     if (bytes > s.stageCap[si]) {
         HTRY(hipStreamSynchronize(s.stream));       // the old buffer may still be read
         hipFree(s.stage[si]);
@@ -967,6 +1034,8 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     std::memcpy(s.stageHost[si], ints.data(), ints.size() * sizeof(int));
     double* wh = (double*)((char*)s.stageHost[si] + intBytes);
     for (const Admission& a : adm) for (double w : a.w) *wh++ = w;
+    for (const Admission& a : adm)
+        if (a.kind == Q_JOIN) for (double w : a.exitOff) *wh++ = w;
     HTRY(hipMemcpyAsync(s.stage[si], s.stageHost[si], bytes, hipMemcpyHostToDevice, s.stream));
     HTRY(hipEventRecord(s.stageEv[si], s.stream));
     const int* d = (const int*)s.stage[si];
@@ -978,13 +1047,16 @@ std::string solver_admit_many(SolverState& s, const DevMesh& m, const std::vecto
     const int* dOff = d + 5 * n;
     const int* dSeeds = d + 6 * n + 1;
     const double* dW = (const double*)((const char*)s.stage[si] + intBytes);
+    const int* dXoff = d + xoffPos;
+    const int* dXv = d + xoffPos + n + 1;
+    const double* dXw = dW + nseeds;
     const size_t total = (size_t)n * s.V;
     hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, (total + 255) / 256)), dim3(256), 0, s.stream,
                        s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, n, s.V);
     DevGraph dummy;
     SolveArgs args = make_args(s, dummy, m);
     hipLaunchKernelGGL(k_admit, dim3(n), dim3(64), 0, s.stream, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
-                       dSeeds, dW, s.delta, s.deltaMsad);
+                       dSeeds, dW, s.delta, s.deltaMsad, dXoff, dXv, dXw, s.hdist);
     s.launches += 2;
     for (const Admission& a : adm) {
         SlotParam& p = s.spHost[a.slot];
@@ -1230,8 +1302,6 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
     rc = get_solver(g, S, &st);
     if (rc) return rc;
     if (path_len) *path_len = 0;
-    std::vector<int> src;
-    std::vector<double> wts;
     for (int c0 = 0; c0 < nq; c0 += chunk) {
         const int n = std::min(chunk, nq - c0);
         e = point_ich(w, dm, n, faces + 2 * (size_t)c0, pts + 6 * (size_t)c0, hopCap, st->stream, out + c0);
@@ -1260,11 +1330,13 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
             for (int sl = 0; sl < st->S && next < todo.size(); ++sl) {
                 if (slotQuery[sl] != -1) continue;
                 const int q = todo[next++];
-                e = point_seeds(w, q, st->stream, src, wts);
-                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
                 Admission a;
-                a.slot = sl; a.kind = Q_MSAD; a.target = -1; a.wantParent = true;
-                a.src = src; a.w = wts;
+                e = point_seeds(w, q, 0, st->stream, a.src, a.w);
+                if (e.empty()) e = point_seeds(w, q, 1, st->stream, a.exitV, a.exitOff);
+                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                // the Dijkstra of SVG.cu:292, stopped once nothing left can
+                // improve the join over the p1 run's vertices (same answer)
+                a.slot = sl; a.kind = Q_JOIN; a.target = -1; a.wantParent = true;
                 adm.push_back(std::move(a));
                 slotQuery[sl] = q;
             }
