@@ -1,8 +1,10 @@
-// Command-line driver with the reference's flows (kernel.cu:17-105, 288-402,
-// 404-456): build + save, A* SSSD, MSAD -- non-interactive.
+// Command-line driver with the reference's flows (kernel.cu:17-105, 107-221,
+// 288-402, 404-456): build + save, A* SSSD, surface-point SSSD, MSAD --
+// non-interactive.
 //
 //   svg_cli build <in.obj> <K> [out.svg] [plain]
 //   svg_cli sssd  <in.obj> <K> <s> <t>
+//   svg_cli point <in.obj> <K> <f0> <x0> <y0> <z0> <f1> <x1> <y1> <z1>
 //   svg_cli msad  <in.obj> <K> <src> [src ...]
 #include <chrono>
 #include <cstdio>
@@ -16,7 +18,7 @@
 int main(int argc, char** argv)
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s build|sssd|msad <in.obj> <K> ...\n", argv[0]);
+        std::fprintf(stderr, "usage: %s build|sssd|point|msad <in.obj> <K> ...\n", argv[0]);
         return 2;
     }
     const std::string cmd = argv[1];
@@ -47,6 +49,17 @@ int main(int argc, char** argv)
         } else if (cmd == "sssd" && argc >= 6) {
             const double d = svg.SolveSSSD(std::atoi(argv[4]), std::atoi(argv[5]));
             std::printf("Distance: %.17g\n", d);
+        } else if (cmd == "point" && argc >= 12) {
+            // kernel.cu:160-202: faces and points in, distance and key points out
+            const double p0[3] = {std::atof(argv[5]), std::atof(argv[6]), std::atof(argv[7])};
+            const double p1[3] = {std::atof(argv[9]), std::atof(argv[10]), std::atof(argv[11])};
+            svg_amd::SVG::SVGNode res;
+            int lastVert = -1;
+            svg.SolveSSSD(std::atoi(argv[4]), p0, std::atoi(argv[8]), p1, &res, &lastVert);
+            std::printf("Distance: %.17g\n", res.geodDist);
+            std::printf("Next to src edge: %d\nNext to src pos: %.17g\n", (int)res.nextToSrcEdge, res.nextToSrcX);
+            std::printf("Next to dst edge: %d\nNext to dst pos: %.17g\n", (int)res.nextToDstEdge, res.nextToDstX);
+            std::printf("Last graph vertex: %d\n", lastVert);
         } else if (cmd == "msad" && argc >= 5) {
             std::vector<int> src;
             for (int i = 4; i < argc; ++i) src.push_back(std::atoi(argv[i]));

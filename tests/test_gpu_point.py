@@ -167,3 +167,34 @@ def test_point_special_positions_match_oracle(graph):
     b = g.SolveSSSDPointsBatch(np.array([[f0, f1], [f0, f1]], np.int32),
                                np.array([[p0, p1], [w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]]]))
     assert b["route"][0] == -2 and np.isnan(b["dist"][0]) and b["route"][1] >= 0
+
+
+def test_cli_point_query(tmp_path):
+    """svg_cli point (the reference's interactive point driver, kernel.cu:160-202,
+    as one command) prints the library's answer."""
+    import os
+    import subprocess
+    xyz, tri = mg.noisy_sphere(seed=9, splits4=3, centroid_split=False)
+    obj = tmp_path / "s.obj"
+    with open(obj, "w") as fh:
+        for p in xyz:
+            fh.write("v %.17g %.17g %.17g\n" % tuple(p))
+        for t in tri:
+            fh.write("f %d %d %d\n" % tuple(t + 1))
+    m = S.Mesh()
+    assert m.LoadFromFile(str(obj))
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(40)
+    g.ConstructSVG()
+    f0, f1 = 5, 400
+    p0 = np.array([0.2, 0.3, 0.5]) @ xyz[tri[f0]]       # %.17g round-trips exactly
+    p1 = np.array([0.6, 0.1, 0.3]) @ xyz[tri[f1]]
+    r = g.SolveSSSDPoints(f0, p0, f1, p1)
+    cli = os.path.join(os.path.dirname(S.__file__), "svg_cli")
+    args = [cli, "point", str(obj), "40", str(f0)] + ["%.17g" % x for x in p0] + [str(f1)] + \
+           ["%.17g" % x for x in p1]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    d = float(out.stdout.split("Distance:")[1].split()[0])
+    assert d == r.dist
