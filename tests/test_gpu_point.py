@@ -198,3 +198,41 @@ def test_cli_point_query(tmp_path):
     assert out.returncode == 0, out.stderr
     d = float(out.stdout.split("Distance:")[1].split()[0])
     assert d == r.dist
+
+
+def test_point_queries_torus72k_K500_match_oracle():
+    """The benchmark's graph (bumpy torus, V = 72,000, K = 500): batch and
+    single answers equal the oracle's on the device-built rows."""
+    xyz, tri = mg.workload("bumpy_torus")
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(500, 10, 2)
+    g.ConstructSVG()
+    V, K, _ = g.info()
+    tails = np.empty(V, np.int32)
+    adj = np.empty(V * K, np.int32)
+    w = np.empty(V * K, np.float64)
+    S.check(S.lib().svg_graph_to_host(g.g, tails.ctypes.data, adj.ctypes.data, w.ctypes.data,
+                                      None, None, None, None))
+    og = O.Graph.from_arrays(xyz, adj, w, tails, K)
+    del adj, w
+    om = O.Mesh(xyz, tri)
+    twin = om.export()["twin"]
+    rng = np.random.default_rng(41)
+    qs = _pairs(rng, tri, twin, 12)
+    faces = np.array([(f0, f1) for f0, _, f1, _ in qs], np.int32)
+    pts = np.array([(w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]) for f0, w0, f1, w1 in qs])
+    print("torus72k point batch", flush=True)
+    b = g.SolveSSSDPointsBatch(faces, pts)
+    routes = set()
+    for i, (f0, f1) in enumerate(faces):
+        o = O.sssd_point(om, og, None, K, int(f0), pts[i, 0], int(f1), pts[i, 1])
+        got = (b["route"][i], b["dist"][i], b["src_edge"][i], b["dst_edge"][i], b["last_vert"][i],
+               b["src_x"][i], b["dst_x"][i])
+        want = (o["route"], o["dist"], o["src_edge"], o["dst_edge"], o["last_vert"], o["src_x"], o["dst_x"])
+        assert _same(got, want), (i, got, want)
+        routes.add(o["route"])
+    assert routes == {0, 1}
+    r = g.SolveSSSDPoints(int(faces[0, 0]), pts[0, 0], int(faces[0, 1]), pts[0, 1])
+    assert (r.dist, r.route, r.last_vert) == (b["dist"][0], b["route"][0], b["last_vert"][0])
