@@ -12,8 +12,10 @@
  *   - its fixed-capacity containers with their exact tie / capacity rules
  *     (PriorityQueue.cuh:51-123, MyHashTable.cuh:51-117,
  *     PriorityQueueWithHandle.cuh:53-135),
- *   - SVG row extraction (SVG.cu:34-76) and
- *   - the A* / Dijkstra graph solvers (SVG.cu:241-251, 354-413).
+ *   - SVG row extraction (SVG.cu:34-76),
+ *   - the A* / Dijkstra graph solvers (SVG.cu:241-251, 354-413) and
+ *   - surface-point sources and the two-point query (ICHDevice.cu:56-65,
+ *     93-107, 143-334, 583-615; SVG.cu:253-352).
  *
  * Decisions that the reference leaves open (it cannot build open meshes and
  * leaves some fields undefined):
@@ -39,9 +41,15 @@
  * (tetrahedron, flat grid); ICH distances by the planar closed form
  * (Euclidean) and the saddle-cone closed form; neighbour sets by the planar
  * and cone closed forms; heaps and hash tables by brute force; the solvers
- * by Bellman-Ford; o_acos against libm within 1 ulp.  The K-stop point and
- * heap tie order on curved meshes are pinned only by following the cited
- * lines (the reference cannot be built here: it needs the CUDA headers).
+ * by Bellman-Ford; o_acos against libm within 1 ulp.  Point sources
+ * (tests/test_oracle_point.py): Euclidean on a flat mesh, the cone closed
+ * form, the K-stop count; the two-point query: Euclidean direct answers with
+ * key points on the segment, the through-a-vertex closed form on a flat mesh
+ * with a complete graph, and the graph route as the pinned MSAD joined with
+ * the second run.  The K-stop point and heap tie order on curved meshes (and
+ * which stale pseudo-sources the point-source guard lets through) are pinned
+ * only by following the cited lines (the reference cannot be built here: it
+ * needs the CUDA headers).
  *
  * Build: oracle/Makefile, gcc -O2 -ffp-contract=off -fopenmp.
  */
