@@ -6,15 +6,15 @@
 // (plain_policy.h); the p0 run then answers directly through f1
 // (BuildGeodesicPathTo(face, pos), ICHDevice.cu:143-334).  A single query
 // runs each propagation in one block with its heaps in LDS
-// (k_point_ich_lds); many queries run one propagation per thread
-// (k_point_ich_many).  The reference runs
-// the p0 propagation twice more (SVG.cu:294-296, 330-332) only to rebuild
-// state these resident tables still hold.  When the direct answer has no
-// source-side key point (SVG.cu:284) the graph stage runs on the solver (a
-// multi-source Dijkstra seeded with the p0 run's vertex distances) and
-// k_point_join takes the minimum over the p1 run's vertices of graph distance
-// plus ICH distance (first table slot on ties, SVG.cu:299-308) and backtraces
-// both ends (SVG.cu:309-351).
+// (k_point_ich_lds); many queries run one propagation per wave
+// (k_point_ich_many).  The reference runs the p0 propagation twice more
+// (SVG.cu:294-296, 330-332) only to rebuild state these resident tables
+// still hold.  When the direct answer has no source-side key point
+// (SVG.cu:284) the graph stage runs on the solver as a join slot (a
+// multi-source Dijkstra seeded with the p0 run's vertex distances that stops
+// once nothing left can improve the join) and k_point_join takes the minimum
+// over the p1 run's vertices of graph distance plus ICH distance (first table
+// slot on ties, SVG.cu:299-308) and backtraces both ends (SVG.cu:309-351).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
