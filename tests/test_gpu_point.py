@@ -236,3 +236,33 @@ def test_point_queries_torus72k_K500_match_oracle():
     assert routes == {0, 1}
     r = g.SolveSSSDPoints(int(faces[0, 0]), pts[0, 0], int(faces[0, 1]), pts[0, 1])
     assert (r.dist, r.route, r.last_vert) == (b["dist"][0], b["route"][0], b["last_vert"][0])
+
+
+def test_point_batch_spans_chunks():
+    """More queries than one chunk of device tables (2,048): every answer
+    still equals the oracle's."""
+    xyz, tri = mg.noisy_sphere(seed=9, splits4=3, centroid_split=False)
+    K = 40
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(K)
+    g.ConstructSVG()
+    h = g.CopySVGToHost()
+    rows = np.zeros(h["adj"].shape, dtype=O.ROW_DTYPE)
+    rows["adj"] = h["adj"]
+    rows["geodDist"] = h["dist"]
+    og = O.Graph(xyz, rows, h["tails"])
+    om = O.Mesh(xyz, tri)
+    rng = np.random.default_rng(77)
+    n = 2100
+    faces = rng.integers(0, len(tri), size=(n, 2)).astype(np.int32)
+    w = rng.dirichlet([2, 2, 2], size=(n, 2))
+    pts = np.einsum("nsj,nsjk->nsk", w, xyz[tri[faces]])
+    print(f"point batch of {n}", flush=True)
+    b = g.SolveSSSDPointsBatch(faces, pts)
+    for i in list(range(0, n, 37)) + [2047, 2048, 2099]:
+        o = O.sssd_point(om, og, None, K, int(faces[i, 0]), pts[i, 0], int(faces[i, 1]), pts[i, 1])
+        got = (b["route"][i], b["dist"][i], b["src_edge"][i], b["dst_edge"][i], b["last_vert"][i])
+        want = (o["route"], o["dist"], o["src_edge"], o["dst_edge"], o["last_vert"])
+        assert _same(got, want), (i, got, want)
