@@ -942,17 +942,50 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
     const int k = blockIdx.x;
     const int slot = slots[k], kind = kinds[k], target = targets[k], gen = gens[k];
     const size_t base = (size_t)slot * a.V;
+    const double delta = kind == Q_SSSD ? deltaSssd : deltaMsad;
+    uint64_t* front = const_cast<uint64_t*>(a.pIn);
+    double mk = DBL_MAX, xb = __int_as_float(0x7f800000);
     if (kind == Q_JOIN) {
         // exit offsets, rounded up so that sums with them bound the exact ones
         for (int i = xoff[k] + threadIdx.x; i < xoff[k + 1]; i += blockDim.x)
             hdistW[base + xv[i]] = __double2float_ru(xw[i]);
         __syncthreads();
+        // the seeds are distinct (one run's vertex table): one per lane of the
+        // block's single wave, one list position reservation per 64
+        const int lane = threadIdx.x;
+        for (int i0 = off[k]; i0 < off[k + 1]; i0 += 64) {
+            const int i = i0 + lane;
+            const bool act = i < off[k + 1];
+            int v = 0;
+            if (act) {
+                v = seeds[i];
+                const double w = wts[i];
+                a.dist[base + v] = (unsigned long long)__double_as_longlong(w);
+                a.parent[base + v] = -1;
+                a.srcid[base + v] = v;
+                a.inq[base + v] = 1;
+                mk = w < mk ? w : mk;
+                const double x = w + (double)hdistW[base + v];
+                xb = x < xb ? x : xb;
+            }
+            const unsigned long long m = __ballot(act);
+            int pos0 = 0;
+            if (lane == 0) pos0 = atomicAdd(&a.cnt->pcnt[a.c][0], __popcll(m));
+            pos0 = __shfl(pos0, 0);
+            if (act) {
+                const int pos = pos0 + __popcll(m & lanes_below(lane));
+                if ((size_t)pos < a.pCap) front[pos] = pack_entry(gen, slot, v);
+                else atomicOr(a.ovFlag, 1);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double om = __shfl_xor(mk, o), ox = __shfl_xor(xb, o);
+            mk = om < mk ? om : mk;
+            xb = ox < xb ? ox : xb;
+        }
     }
     if (threadIdx.x != 0) return;
-    const double delta = kind == Q_SSSD ? deltaSssd : deltaMsad;
-    uint64_t* front = const_cast<uint64_t*>(a.pIn);
-    double mk = DBL_MAX, xb = __int_as_float(0x7f800000);
-    for (int i = off[k]; i < off[k + 1]; ++i) {
+    for (int i = off[k]; kind != Q_JOIN && i < off[k + 1]; ++i) {
         const int v = seeds[i];
         const double w = wts[i];
         a.dist[base + v] = (unsigned long long)__double_as_longlong(w);
@@ -966,10 +999,6 @@ __global__ void k_admit(SolveArgs a, const int* __restrict__ slots, const int* _
         }
         const double key = kind == Q_SSSD ? w + heur(a.pos, v, target) : w;
         mk = key < mk ? key : mk;
-        if (kind == Q_JOIN) {
-            const double x = w + (double)hdistW[base + v];
-            xb = x < xb ? x : xb;
-        }
     }
     SlotParam p;
     p.kind = kind; p.target = target; p.wantParent = wantP[k]; p.done = 0;
