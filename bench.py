@@ -225,7 +225,7 @@ def main():
                     help="submit every request at once (queue sized to hold them): throughput, not latency")
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (byte-counter calibration)")
-    ap.add_argument("--points", type=int, default=2048,
+    ap.add_argument("--points", type=int, default=8192,
                     help="surface-point queries in the batch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

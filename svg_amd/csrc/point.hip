@@ -43,8 +43,9 @@ namespace {
 
 constexpr unsigned kSplitCoef = 10, kVertCoef = 2;   // kernel.cu:188-191 (K*10, K*2 entries)
 // Runs per 64-wide wave in k_point_ich_many: a run is a latency-bound chain,
-// so few lanes per wave (less divergence) and more waves per CU pay.
-constexpr int kRunsPerWave = 1;
+// so few lanes per wave (little divergence); 4 lanes keep 16K runs resident
+// (4 waves per SIMD) where one lane per wave would hold 4K.
+constexpr int kRunsPerWave = 4;
 
 // Run r of the batch is query r / 2 from its first point (r even: p0, keeping
 // the windows that cross into f1) or its second (r odd: p1).

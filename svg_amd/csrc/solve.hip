@@ -1318,8 +1318,11 @@ static int point_queries(svg_graph_s* g, int nq, const int32_t* faces, const dou
         if (!std::isfinite(pts[i])) return svg_set_error(SVG_E_ARG, "point not finite");
     int rc = svg_mesh_upload_impl(const_cast<svg_mesh_s*>(g->mesh), true);
     if (rc) return rc;
-    // a chunk's tables: about 570 KB per query at K = 500
-    const int chunk = std::max(1, std::min(nq, 2048));
+    // a chunk's tables (about 570 KB per query at K = 500) within 8 GB; larger
+    // chunks keep more propagations in flight (8,192 queries: 5.8K/s against
+    // 4.6K/s in chunks of 2,048)
+    const size_t perQuery = point_bytes_per_query(g->d.K);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nq, 8192, ((size_t)8 << 30) / perQuery}));
     if (!g->point) g->point = new PointWork();
     std::string e = point_alloc(*g->point, g->d.K, chunk);
     if (!e.empty()) return svg_set_error(SVG_E_MEM, e);
