@@ -266,3 +266,31 @@ def test_point_batch_spans_chunks():
         got = (b["route"][i], b["dist"][i], b["src_edge"][i], b["dst_edge"][i], b["last_vert"][i])
         want = (o["route"], o["dist"], o["src_edge"], o["dst_edge"], o["last_vert"])
         assert _same(got, want), (i, got, want)
+
+
+@pytest.mark.parametrize("name", ["sphere", "disk"])
+def test_gpu_matches_frozen_oracle_outputs(name):
+    """Rows and point queries of the GPU path equal the frozen oracle outputs
+    (tests/golden/oracle_small.npz) without running the oracle."""
+    import os
+    fix = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_small.npz")
+    with np.load(fix, allow_pickle=False) as z:
+        f = {k[len(name) + 1:]: z[k] for k in z.files if k.startswith(name + "_")}
+    K = int(f["K"])
+    m = S.Mesh.from_arrays(f["xyz"], f["tri"])
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(K)
+    g.ConstructSVG()
+    h = g.CopySVGToHost()
+    assert np.array_equal(h["tails"], f["tails"])
+    for u, t in enumerate(f["tails"]):
+        assert np.array_equal(h["adj"][u, :t], f["adj"][u, :t])
+        assert np.array_equal(h["dist"][u, :t], f["dist"][u, :t])
+        assert np.array_equal(h["src_edge"][u, :t], f["src_edge"][u, :t])
+        assert np.array_equal(h["src_x"][u, :t], f["src_x"][u, :t])
+    b = g.SolveSSSDPointsBatch(f["faces"], f["pts"])
+    got_i = np.stack([b["src_edge"], b["dst_edge"], b["last_vert"], b["route"]], 1)
+    got_d = np.stack([b["dist"], b["src_x"], b["dst_x"]], 1)
+    assert np.array_equal(got_i, f["point_i"])
+    assert np.array_equal(got_d, f["point_d"], equal_nan=True)
