@@ -231,8 +231,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design "
-                                   "note only (sources shard by predicted cost; see DESIGN.md)",
+        print(json.dumps({"error": "the benchmark runs on one GPU only; several GPUs would each "
+                                   "build a cost-balanced share of the sources (not implemented)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
