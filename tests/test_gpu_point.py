@@ -21,6 +21,9 @@ MESHES = {
     "torus40x24_K80": (lambda: mg.bumpy_torus(nu=40, nv=24), 80),
     "grid12_full": (lambda: mg.planar_grid(n=12, jitter=0.3, seed=7), 144),
     "disk_K40": (lambda: mg.planar_disk(), 40),          # open: boundary faces and vertices
+    # K above 511: a single query's heaps and vertex table exceed a block's
+    # LDS, so its runs take the global-memory kernel
+    "sphere642_K600": (lambda: mg.noisy_sphere(seed=9, splits4=3, centroid_split=False), 600),
 }
 
 
