@@ -96,3 +96,51 @@ def test_sssd_batch_equals_single(small_graph):
     out = g.SolveSSSDBatch(pairs)
     for (s, t), d in zip(pairs, out):
         assert d == g.SolveSSSD(int(s), int(t))
+
+
+def test_svg_file_round_trip_and_layout(small_graph, tmp_path):
+    """SaveSVGToFile / LoadSVGFromFile (SVG.cu:208-239): per vertex an int32
+    tail, then tail 40-byte SVGNode records (SVG.cuh:21-27: adjNode, 4 bytes
+    of padding, geodDist, nextToSrcEdge, nextToDstEdge, nextToSrcX,
+    nextToDstX).  The file parses with that layout into the graph's rows, and
+    a graph loaded from it answers queries identically."""
+    m, g, og, h = small_graph
+    path = str(tmp_path / "g.svg")
+    g.SaveSVGToFile(path)
+    node = np.dtype([("adj", "<i4"), ("pad", "<i4"), ("w", "<f8"), ("se", "<u4"), ("de", "<u4"),
+                     ("sx", "<f8"), ("dx", "<f8")])
+    assert node.itemsize == 40
+    raw = open(path, "rb").read()
+    off = 0
+    for u in range(len(h["tails"])):
+        t = int(np.frombuffer(raw, "<i4", 1, off)[0])
+        off += 4
+        assert t == h["tails"][u]
+        rec = np.frombuffer(raw, node, t, off)
+        off += 40 * t
+        assert np.array_equal(rec["adj"], h["adj"][u, :t])
+        assert np.array_equal(rec["w"], h["dist"][u, :t])
+        assert np.array_equal(rec["se"].view(np.int32), h["src_edge"][u, :t])
+        assert np.array_equal(rec["de"].view(np.int32), h["dst_edge"][u, :t])
+        assert np.array_equal(rec["sx"], h["src_x"][u, :t])
+        assert np.array_equal(rec["dx"], h["dst_x"][u, :t])
+    assert off == len(raw)
+    g2 = S.SVG()
+    g2.AssignMesh(m)
+    g2.SetParameters(g.K)
+    g2.LoadSVGFromFile(path)
+    h2 = g2.CopySVGToHost()
+    for k in ("tails", "adj", "dist", "src_edge", "dst_edge", "src_x", "dst_x"):
+        if k == "tails":
+            assert np.array_equal(h2[k], h[k])
+        else:
+            for u, t in enumerate(h["tails"]):
+                assert np.array_equal(h2[k][u, :t], h[k][u, :t]), k
+    assert g2.SolveSSSD(5, 400) == g.SolveSSSD(5, 400)
+    # a truncated file is an error, not a partial graph
+    open(path, "wb").write(raw[: len(raw) // 2])
+    g3 = S.SVG()
+    g3.AssignMesh(m)
+    g3.SetParameters(g.K)
+    with pytest.raises(S.SvgError):
+        g3.LoadSVGFromFile(path)
