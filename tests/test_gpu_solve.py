@@ -144,3 +144,35 @@ def test_svg_file_round_trip_and_layout(small_graph, tmp_path):
     g3.SetParameters(g.K)
     with pytest.raises(S.SvgError):
         g3.LoadSVGFromFile(path)
+
+
+def test_cli_build_sssd_msad(small_graph, tmp_path):
+    """svg_cli (the reference's driver flows as commands, kernel.cu:17-105,
+    288-402, 404-456): build writes the .svg file the library writes, sssd and
+    msad print the library's answers."""
+    import os
+    import subprocess
+    m, g, og, h = small_graph
+    xyz = og.xyz.reshape(-1, 3)
+    tri = np.asarray(mg.noisy_sphere(seed=9, splits4=3, centroid_split=False)[1])
+    obj = tmp_path / "s.obj"
+    with open(obj, "w") as fh:
+        for p in xyz:
+            fh.write("v %.17g %.17g %.17g\n" % tuple(p))
+        for t in tri:
+            fh.write("f %d %d %d\n" % tuple(t + 1))
+    cli = os.path.join(os.path.dirname(S.__file__), "svg_cli")
+    run = lambda *a: subprocess.run([cli, *a], capture_output=True, text=True, timeout=120)
+    out = run("build", str(obj), "40", str(tmp_path / "cli.svg"))
+    assert out.returncode == 0, out.stderr
+    g.SaveSVGToFile(str(tmp_path / "lib.svg"))
+    assert open(tmp_path / "cli.svg", "rb").read() == open(tmp_path / "lib.svg", "rb").read()
+    out = run("sssd", str(obj), "40", "5", "400")
+    assert out.returncode == 0, out.stderr
+    assert float(out.stdout.split("Distance:")[1].split()[0]) == g.SolveSSSD(5, 400)
+    out = run("msad", str(obj), "40", "3", "77")
+    assert out.returncode == 0, out.stderr
+    d, _, _ = g.SolveMSAD([3, 77], [0.0, 0.0])
+    far = float(out.stdout.split("vertex at")[1].split()[0])
+    assert far == d[d < 1e300].max()
+    assert run("nonsense", str(obj), "40").returncode != 0
