@@ -143,6 +143,22 @@ public:
         res->nextToSrcX = kx[0]; res->nextToDstX = kx[1];
         if (saddleVertNearDst) *saddleVertNearDst = last;
     }
+    // Many point pairs at once: faces[2n] = (f0, f1), points[6n] = (p0, p1).
+    // route[q] (optional) is 0 direct, 1 through the graph, -1 nothing joins
+    // the points, -2 a point lies on a vertex of its face (NaN distance).
+    void SolveSSSDBatch(int n, const int32_t* faces, const double* points, std::vector<SVGNode>& res,
+                        std::vector<int>* saddleVertNearDst = nullptr, std::vector<int>* route = nullptr)
+    {
+        std::vector<double> d((size_t)n), kx(2 * (size_t)n);
+        std::vector<int32_t> ke(2 * (size_t)n), last((size_t)n), rt((size_t)n);
+        check(svg_sssd_point_batch(g_, n, faces, points, d.data(), ke.data(), kx.data(), last.data(), rt.data()));
+        res.resize((size_t)n);
+        for (int q = 0; q < n; ++q)
+            res[(size_t)q] = SVGNode{-1, d[(size_t)q], (unsigned)ke[2 * (size_t)q], (unsigned)ke[2 * (size_t)q + 1],
+                                     kx[2 * (size_t)q], kx[2 * (size_t)q + 1]};
+        if (saddleVertNearDst) saddleVertNearDst->assign(last.begin(), last.end());
+        if (route) route->assign(rt.begin(), rt.end());
+    }
     void SolveMSAD(const int* sources, const double* weights, int Ns, std::vector<GraphDistInfo>& out)
     {                                                  // SVG.cu:354-367
         int V = 0;
