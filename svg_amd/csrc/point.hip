@@ -58,7 +58,7 @@ struct PointArgs {
     unsigned ssz, vsz;
     PSplitItem* sp;          // per-run tables, run-major
     PVertItem* vt;
-    PWinItem* wq;
+    unsigned char* wq;       // per-run window heaps (kWinHeapBytes each)
     PPsItem* pq;
     Win* stored;             // kStoredWin per query
     PointRes* res;           // per query
@@ -72,22 +72,74 @@ struct PointArgs {
 // writes parent slots below it) and is copied once at the end.  Same heap
 // contents as PlainPolicy's pops, without an 80-byte item live across the
 // loop (which the compiler keeps in scratch memory).
+// Window heap of a point run: the reference's 1-based binary heap on the
+// same keys with the same tie and capacity rules (PriorityQueue.cuh:51-105),
+// holding 16-B (key, record) entries while the windows sit in a pool of
+// 56-B records.  A sift moves 16 B per level instead of the 80-B item; the
+// pop order is the reference's.
+struct PHeapEnt { double key; int idx; int pad; };
+struct PRec { double b0, b1, d0, d1, ps; int edge, pseudoSrc, birth, level; };
+static_assert(sizeof(PHeapEnt) == 16 && sizeof(PRec) == 56, "record sizes");
+constexpr int kPool = kHeapCap + 1;             // records (the heap holds at most kHeapCap)
+constexpr size_t kWinHeapBytes = sizeof(PHeapEnt) * (kHeapCap + 1) + sizeof(PRec) * kPool + sizeof(int) * kPool;
+
 struct PointPolicy : PlainPolicy {
+    PHeapEnt* wh;            // heap entries 1..kHeapCap
+    PRec* wr;                // record pool
+    int* wfree;              // free record stack
+    int wfreeTop;
+
+    __device__ void bind_window_heap(void* base)
+    {
+        wh = (PHeapEnt*)base;
+        wr = (PRec*)((char*)base + sizeof(PHeapEnt) * (kHeapCap + 1));
+        wfree = (int*)((char*)wr + sizeof(PRec) * kPool);
+    }
+    __device__ void clear()
+    {
+        PlainPolicy::clear();
+        for (int i = 0; i < kPool; ++i) wfree[i] = kPool - 1 - i;
+        wfreeTop = kPool;
+    }
+    __device__ void wtop(Win& w) const
+    {
+        const PRec& r = wr[wh[1].idx];
+        w.edge = r.edge; w.b0 = r.b0; w.b1 = r.b1; w.d0 = r.d0; w.d1 = r.d1; w.ps = r.ps;
+        win_finish(w);
+        w.twin = E[w.edge].twin;
+        w.pseudoSrc = r.pseudoSrc; w.birth = r.birth; w.level = r.level;
+    }
     __device__ void wpop()
     {
+        const int freed = wh[1].idx;
         --wtail;
-        const double key = wq[wtail].key;
+        const PHeapEnt last = wh[wtail];
         int cur = 1, l = 2, r = 3, mc = -1;
-        if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+        if (r < wtail) mc = wh[l].key < wh[r].key ? l : r;
         else if (l < wtail) mc = l;
-        while (mc != -1 && wq[mc].key < key) {
-            wq[cur] = wq[mc];
+        while (mc != -1 && wh[mc].key < last.key) {
+            wh[cur] = wh[mc];
             cur = mc; l = cur * 2; r = cur * 2 + 1;
-            if (r < wtail) mc = wq[l].key < wq[r].key ? l : r;
+            if (r < wtail) mc = wh[l].key < wh[r].key ? l : r;
             else if (l < wtail) mc = l;
             else mc = -1;
         }
-        if (cur != wtail) wq[cur] = wq[wtail];
+        wh[cur] = last;
+        wfree[wfreeTop++] = freed;
+    }
+    __device__ void wpush(const Win& w)
+    {
+        if (wtail - 1 >= kHeapCap) { flags |= SF_WIN_DROP; return; }
+        const int idx = wfree[--wfreeTop];
+        PRec& rec = wr[idx];
+        rec.b0 = w.b0; rec.b1 = w.b1; rec.d0 = w.d0; rec.d1 = w.d1; rec.ps = w.ps;
+        rec.edge = w.edge; rec.pseudoSrc = w.pseudoSrc; rec.birth = w.birth; rec.level = w.level;
+        ++wtail;
+        int cur = wtail - 1, pp = cur / 2;
+        const double key = w.minDist;
+        while (pp > 0 && wh[pp].key > key) { wh[cur] = wh[pp]; cur = pp; pp = cur / 2; }
+        PHeapEnt e; e.key = key; e.idx = idx; e.pad = 0;
+        wh[cur] = e;
     }
     __device__ void ppop()
     {
@@ -114,10 +166,16 @@ __device__ void make_policy(const PointArgs& a, int run, PlainPolicy& p)
     p.ssz = a.ssz; p.vsz = a.vsz;
     p.sp = a.sp + (size_t)run * (a.ssz + 1);
     p.vt = a.vt + (size_t)run * (a.vsz + 1);
-    p.wq = a.wq + (size_t)run * (kHeapCap + 1);
+    p.wq = nullptr;              // the window heap is PointPolicy's (bind_window_heap)
     p.pq = a.pq + (size_t)run * (kHeapCap + 1);
     p.wtail = 1; p.ptail = 1; p.flags = 0;
     p.srcVert = a.nv;            // srcId of a surface point (ICHDevice.cu:596)
+}
+
+__device__ void make_policy(const PointArgs& a, int run, PointPolicy& p)
+{
+    make_policy(a, run, static_cast<PlainPolicy&>(p));
+    p.bind_window_heap((char*)a.wq + (size_t)run * kWinHeapBytes);
 }
 
 // Straight-line distances from point q to the vertices of face f, in the
@@ -164,7 +222,7 @@ __device__ void point_run(const PointArgs& a, int run, PointPolicy& p)
 // LDS bytes of a run's window heap, pseudo-source heap and vertex table.
 inline size_t lds_bytes(unsigned vsz)
 {
-    return sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1) + sizeof(PVertItem) * (vsz + 1);
+    return kWinHeapBytes + sizeof(PPsItem) * (kHeapCap + 1) + sizeof(PVertItem) * (vsz + 1);
 }
 
 // One block per run, for a single query.  A run is one sequential
@@ -180,9 +238,9 @@ __global__ void __launch_bounds__(64) k_point_ich_lds(PointArgs a)
     make_policy(a, run, p);
     PVertItem* dst = p.vt;
     unsigned char* b = (unsigned char*)ldsBuf;
-    p.wq = (PWinItem*)b;
-    p.pq = (PPsItem*)(b + sizeof(PWinItem) * (kHeapCap + 1));
-    p.vt = (PVertItem*)(b + sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1));
+    p.bind_window_heap(b);
+    p.pq = (PPsItem*)(b + kWinHeapBytes);
+    p.vt = (PVertItem*)(b + kWinHeapBytes + sizeof(PPsItem) * (kHeapCap + 1));
     if (threadIdx.x == 0) point_run(a, run, p);
     __syncthreads();
     for (unsigned i = threadIdx.x; i < a.vsz + 1; i += blockDim.x) dst[i] = p.vt[i];
@@ -191,7 +249,9 @@ __global__ void __launch_bounds__(64) k_point_ich_lds(PointArgs a)
 // Many queries: one run per lane of the first kRunsPerWave lanes of each
 // wave, tables in global memory (the plain build kernel's layout), so the
 // device holds thousands of runs at once.
-__global__ void __launch_bounds__(64) k_point_ich_many(PointArgs a)
+// Three waves per SIMD (168 registers, no VGPR spills) beat two (the
+// unconstrained 172 registers): 1,360 against 1,411 ms for 8,192 queries.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_point_ich_many(PointArgs a)
 {
     if ((int)threadIdx.x >= kRunsPerWave) return;
     const int total = gridDim.x * kRunsPerWave;
@@ -300,7 +360,7 @@ PointArgs make_args(PointWork& w, const DevMesh& m, int hopCap, int nRuns)
     a.edges = m.edges; a.verts = m.verts; a.cornerA = m.cornerA; a.cornerB = m.cornerB;
     a.nv = m.nv; a.nf = m.nf; a.K = w.K; a.hopCap = hopCap;
     a.ssz = w.ssz; a.vsz = w.vsz;
-    a.sp = (PSplitItem*)w.sp; a.vt = (PVertItem*)w.vt; a.wq = (PWinItem*)w.wq; a.pq = (PPsItem*)w.pq;
+    a.sp = (PSplitItem*)w.sp; a.vt = (PVertItem*)w.vt; a.wq = (unsigned char*)w.wq; a.pq = (PPsItem*)w.pq;
     a.stored = (Win*)w.stored;
     a.res = w.res;
     a.f = w.fDev; a.p = w.pDev;
@@ -313,8 +373,8 @@ PointArgs make_args(PointWork& w, const DevMesh& m, int hopCap, int nRuns)
 size_t point_bytes_per_query(int K)
 {
     const size_t ssz = (size_t)K * kSplitCoef, vsz = (size_t)K * kVertCoef;
-    return 2 * (sizeof(PSplitItem) * (ssz + 1) + sizeof(PVertItem) * (vsz + 1) +
-                sizeof(PWinItem) * (kHeapCap + 1) + sizeof(PPsItem) * (kHeapCap + 1)) +
+    return 2 * (sizeof(PSplitItem) * (ssz + 1) + sizeof(PVertItem) * (vsz + 1) + kWinHeapBytes +
+                sizeof(PPsItem) * (kHeapCap + 1)) +
            sizeof(Win) * kStoredWin + sizeof(PointRes) + 2 * sizeof(int32_t) + 6 * sizeof(double);
 }
 
@@ -329,7 +389,7 @@ std::string point_alloc(PointWork& w, int K, int cap)
     const size_t runs = 2 * (size_t)cap;
     PT_TRY(hipMalloc(&w.sp, sizeof(PSplitItem) * (w.ssz + 1) * runs));
     PT_TRY(hipMalloc(&w.vt, sizeof(PVertItem) * (w.vsz + 1) * runs));
-    PT_TRY(hipMalloc(&w.wq, sizeof(PWinItem) * (kHeapCap + 1) * runs));
+    PT_TRY(hipMalloc(&w.wq, kWinHeapBytes * runs));
     PT_TRY(hipMalloc(&w.pq, sizeof(PPsItem) * (kHeapCap + 1) * runs));
     PT_TRY(hipMalloc(&w.stored, sizeof(Win) * kStoredWin * (size_t)cap));
     PT_TRY(hipMalloc(&w.res, sizeof(PointRes) * (size_t)cap));
