@@ -145,8 +145,8 @@ int svg_sssd_point(svg_graph_t g, int f0, const double *p0, int f1, const double
  * (p0, p1) per query; dist[nq]; key_edge[2*nq], key_x[2*nq], last_vert[nq],
  * route[nq] (may be NULL) as svg_sssd_point; a query whose propagation
  * stalls (a point on a vertex of its face) gets route -2 and a NaN distance.
- * Each query's ICH runs are one thread each; the graph stages share the
- * solver's slots. */
+ * Each ICH run is one lane (4 per wave, up to 8,192 queries per launch); the
+ * graph stages share the solver's slots. */
 int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t *faces, const double *points, double *dist,
                          int32_t *key_edge, double *key_x, int32_t *last_vert, int32_t *route);
 /* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */
