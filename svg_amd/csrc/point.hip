@@ -109,20 +109,24 @@ struct PointPolicy : PlainPolicy {
         w.twin = E[w.edge].twin;
         w.pseudoSrc = r.pseudoSrc; w.birth = r.birth; w.level = r.level;
     }
+    // Both children's entries are read at once, so a level of the sift-down
+    // is one round trip; the right child wins ties, as in the reference.
     __device__ void wpop()
     {
         const int freed = wh[1].idx;
         --wtail;
         const PHeapEnt last = wh[wtail];
-        int cur = 1, l = 2, r = 3, mc = -1;
-        if (r < wtail) mc = wh[l].key < wh[r].key ? l : r;
-        else if (l < wtail) mc = l;
-        while (mc != -1 && wh[mc].key < last.key) {
-            wh[cur] = wh[mc];
-            cur = mc; l = cur * 2; r = cur * 2 + 1;
-            if (r < wtail) mc = wh[l].key < wh[r].key ? l : r;
-            else if (l < wtail) mc = l;
-            else mc = -1;
+        int cur = 1;
+        for (;;) {
+            const int l = 2 * cur, r = l + 1;
+            if (l >= wtail) break;
+            const PHeapEnt cl = wh[l];
+            const PHeapEnt cr = wh[r < wtail ? r : l];
+            const bool right = r < wtail && !(cl.key < cr.key);
+            const PHeapEnt cm = right ? cr : cl;
+            if (!(cm.key < last.key)) break;
+            wh[cur] = cm;
+            cur = right ? r : l;
         }
         wh[cur] = last;
         wfree[wfreeTop++] = freed;
