@@ -41,8 +41,9 @@ int svg_device_name(char *buf, int cap);
 /* ---------------------------------------------------------------- mesh */
 /* Mesh::LoadFromFile (Mesh.cu:24-126) from arrays: xyz[3*nv] (f64),
  * tri[3*nf] (0-based, counter-clockwise).  Builds half-edges, twins,
- * angle sums and firstEdge with the reference rules; open meshes use
- * decisions D1/D2/D8/D9 (boundary twin -1). */
+ * angle sums and firstEdge with the reference rules; on open meshes a
+ * boundary half-edge has twin -1, a boundary vertex's firstEdge is its
+ * outgoing half-edge whose prev has no twin, and it is never a saddle. */
 int svg_mesh_from_arrays(const double *xyz, int nv, const int32_t *tri, int nf, svg_mesh_t *out);
 /* Mesh::LoadFromFile (Mesh.cu:24-126): 'v x y z' and 'f a b c' lines as
  * there; also 'f' corners written i/t/n, i//n or as negative indices, and

@@ -9,7 +9,11 @@
 // :336-480 (BuildGeodesicPathTo), :525-581 (Initialize, vertex source),
 // :618-726 (PropagateWindow), :728-904 (pseudo-source windows),
 // :906-979 (IsValidWindow, BuildWindow, Intersect); Geometry.cu:306-311
-// (toLeft).  Open-mesh decisions D8/D9 are documented in the oracle header.
+// (toLeft).  On open meshes (which the reference cannot build) a boundary
+// vertex is never a pseudo-source, and a boundary source also reaches the
+// 1-ring vertex across its incoming boundary edge; a 1-ring vertex reached
+// over an outgoing boundary half-edge e records e itself as its enter edge,
+// and the backtrace takes verts[1](enter) == vertex as a step along e.
 //
 // The policy P provides
 //   mesh:   int twin(e), next(e), prev(e), v0(e), v1(e); double len(e);
@@ -132,7 +136,8 @@ struct Ich {
     SVG_HD void push_win(const Win& w) { p.count(CNT_WIN_PUSH, 1); p.wpush(w); }
     SVG_HD void push_pseudo(const Pseudo& q) { p.count(CNT_PSEUDO_PUSH, 1); p.ppush(q); }
 
-    // ICHDevice::Initialize, vertex source (ICHDevice.cu:527-581) + D9
+    // ICHDevice::Initialize, vertex source (ICHDevice.cu:527-581), plus the
+    // boundary-source rule above
     SVG_HD void initialize()
     {
         const int src = source;
@@ -160,7 +165,7 @@ struct Ich {
             VInfo vi = p.vget(opVert);
             vi.birth = 0;
             vi.dist = p.len(cur);
-            vi.enter = p.twin(cur) != -1 ? p.twin(cur) : cur;     // D9
+            vi.enter = p.twin(cur) != -1 ? p.twin(cur) : cur;     // boundary: e itself
             p.vput(opVert, vi);
             if (p.vflag(opVert) & 1) {                            // angle < 2*PI
                 cur = p.twin(cur);
@@ -176,7 +181,7 @@ struct Ich {
             cur = p.twin(cur);
             if (cur != -1) cur = p.next(cur);
         } while (cur != start && cur != -1);
-        if (cur == -1) {                                          // D9: boundary source
+        if (cur == -1) {                                          // boundary source
             const int inE = p.prev(start);
             const int w = p.v0(inE);
             p.vtouch(w);
@@ -589,7 +594,8 @@ struct Ich {
         return true;
     }
 
-    // ICHDevice::BuildGeodesicPathTo(vertId, ...) (ICHDevice.cu:336-480) + D9.
+    // ICHDevice::BuildGeodesicPathTo(vertId, ...) (ICHDevice.cu:336-480), plus
+    // steps along a boundary enter edge.
     // Returns pathPassVert; fills the two key points (edge -1 -> x = 0).
     SVG_HD bool path_to(int vertId, int& srcEdge, double& srcX, int& dstEdge, double& dstX, int hopCap)
     {
@@ -614,7 +620,7 @@ struct Ich {
                     pass = true;
                 }
                 curVert = nextVert;
-            } else if (p.v1(enter) == curVert) {                       // D9
+            } else if (p.v1(enter) == curVert) {                       // along a boundary enter edge
                 const int nextVert = p.v0(enter);
                 if (!p.vget(nextVert).isSource) {
                     if (dstEdge == -1) { dstEdge = enter; dstX = 0.0; }

@@ -1,4 +1,6 @@
-// Host half-edge builder (Mesh.cu:24-126 semantics, decisions D1/D2/D8).
+// Host half-edge builder (Mesh.cu:24-126 semantics; on open meshes boundary
+// twins are -1, a boundary vertex starts its fan at the boundary and is never a
+// saddle).
 // Compiled with -ffp-contract=off so lengths and angle sums are the same
 // correctly rounded values on every build.
 #include "svg_mesh.h"
@@ -30,7 +32,7 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
     m.nv = nv; m.nf = nf; m.ne = 3 * nf;
     m.xyz.assign(xyz, xyz + 3 * (size_t)nv);
     m.tri.assign(tri, tri + 3 * (size_t)nf);
-    m.twin.assign(m.ne, -1);                                   // D1
+    m.twin.assign(m.ne, -1);                                   // boundary: -1
     m.len.resize(m.ne);
     for (int f = 0; f < nf; ++f) {
         for (int j = 0; j < 3; ++j) {
@@ -73,12 +75,12 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
     std::vector<uint8_t> bnd(nv, 0);
     for (int e = 0; e < m.ne; ++e)
         if (m.twin[e] == -1) { bnd[v0(e)] = 1; bnd[v1(e)] = 1; }
-    for (int e = 0; e < m.ne; ++e)                             // D2
+    for (int e = 0; e < m.ne; ++e)                             // boundary fan start
         if (bnd[v0(e)] && m.twin[prev(e)] == -1) m.firstEdge[v0(e)] = e;
     m.flags.assign(nv, 0);
     for (int v = 0; v < nv; ++v) {
         uint8_t f = 0;
-        if (bnd[v] || m.angle[v] < 2.0 * kPi) f |= VF_NOT_SADDLE;     // D8
+        if (bnd[v] || m.angle[v] < 2.0 * kPi) f |= VF_NOT_SADDLE;     // boundary: never a saddle
         if (!bnd[v] && m.angle[v] > 2.0 * kPi) f |= VF_SADDLE;
         if (bnd[v]) f |= VF_BOUNDARY;
         m.flags[v] = f;

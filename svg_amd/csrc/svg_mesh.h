@@ -4,9 +4,10 @@
 // half-edge 3f+j runs tri[f][j] -> tri[f][(j+1)%3], next/prev stay inside the
 // face, twins pair opposite half-edges, the vertex angle sum adds the corner
 // angle at verts[0] of every half-edge in index order, and firstEdge is the
-// largest-index outgoing half-edge.  Open meshes follow decisions D1 (boundary
-// twin = -1), D2 (boundary firstEdge = the outgoing half-edge whose prev has
-// no twin) and D8 (boundary vertices are never saddles).
+// largest-index outgoing half-edge.  On open meshes (which the reference
+// cannot build) a boundary half-edge has twin -1, a boundary vertex's
+// firstEdge is its outgoing half-edge whose prev has no twin, and boundary
+// vertices are never saddles.
 #pragma once
 
 #include <cstdint>

@@ -98,7 +98,7 @@ def test_flat_grid_angle_sums_and_boundary_first_edge():
     corners = [0, n - 1, n * (n - 1), n * n - 1]
     # corner angle sums: pi/2 except the two split by a diagonal (still pi/2)
     assert np.allclose(ex["angle"][corners], math.pi / 2, atol=1e-13)
-    # D2: a boundary vertex's firstEdge is outgoing and its prev has no twin
+    # a boundary vertex's firstEdge is outgoing and its prev has no twin
     tw = ex["twin"]
     v0 = tri.ravel()
     for v in np.nonzero(~interior)[0]:
