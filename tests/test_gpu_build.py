@@ -56,10 +56,10 @@ def test_build_heightfield64_full_both_modes():
 
 
 def test_build_sphere30k_sample():
-    """noisy_sphere (V=30,722, K=100), batched, oracle on 200 spread sources."""
+    """noisy_sphere (V=30,722, K=100), batched, oracle on 2,000 spread sources."""
     xyz, tri = mg.noisy_sphere()
     K = 100
-    srcs = np.linspace(0, len(xyz) - 1, 200).astype(int)
+    srcs = np.linspace(0, len(xyz) - 1, 2000).astype(int)
     print("case sphere30k", flush=True)
     _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
     m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
@@ -127,12 +127,12 @@ def test_build_torus72k_full_and_autotuned():
 def test_build_terrain750k_sample(K):
     """aniso_terrain (866x866 grid, V=749,956, skinny/obtuse triangles, open
     boundary) at K=100 and K=500 (the reference runs out of memory at 500):
-    every source built in one pass, oracle parity on 48 spread sources,
+    every source built in one pass, oracle parity on 480 spread sources,
     every row sorted and within K."""
     xyz, tri = mg.aniso_terrain()
     print(f"case terrain750k K={K}", flush=True)
     m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
-    srcs = np.linspace(0, len(xyz) - 1, 48).astype(int)
+    srcs = np.linspace(0, len(xyz) - 1, 480).astype(int)
     _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
     tails, h = gpu_rows(g)
     bad = compare_rows((tails, h), orows, otails, srcs, K)
