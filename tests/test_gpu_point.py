@@ -223,7 +223,7 @@ def test_point_queries_torus72k_K500_match_oracle():
     om = O.Mesh(xyz, tri)
     twin = om.export()["twin"]
     rng = np.random.default_rng(41)
-    qs = _pairs(rng, tri, twin, 12)
+    qs = _pairs(rng, tri, twin, 64)
     faces = np.array([(f0, f1) for f0, _, f1, _ in qs], np.int32)
     pts = np.array([(w0 @ xyz[tri[f0]], w1 @ xyz[tri[f1]]) for f0, w0, f1, w1 in qs])
     print("torus72k point batch", flush=True)
