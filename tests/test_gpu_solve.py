@@ -176,3 +176,38 @@ def test_cli_build_sssd_msad(small_graph, tmp_path):
     far = float(out.stdout.split("vertex at")[1].split()[0])
     assert far == d[d < 1e300].max()
     assert run("nonsense", str(obj), "40").returncode != 0
+
+
+def test_solvers_torus72k_K500_match_oracle():
+    """The benchmark's graph (bumpy torus, V = 72,000, K = 500, 35M arcs):
+    A* distances within 1e-9 of the oracle's and MSAD distances bit-equal to
+    its heap Dijkstra (SVG.cu:354-413), on the device-built rows."""
+    xyz, tri = mg.workload("bumpy_torus")
+    m = S.Mesh.from_arrays(xyz, tri)
+    g = S.SVG()
+    g.AssignMesh(m)
+    g.SetParameters(500, 10, 2)
+    g.ConstructSVG()
+    V, K, _ = g.info()
+    tails = np.empty(V, np.int32)
+    adj = np.empty(V * K, np.int32)
+    w = np.empty(V * K, np.float64)
+    S.check(S.lib().svg_graph_to_host(g.g, tails.ctypes.data, adj.ctypes.data, w.ctypes.data,
+                                      None, None, None, None))
+    og = O.Graph.from_arrays(xyz, adj, w, tails, K)
+    del adj, w
+    rng = np.random.default_rng(72)
+    for _ in range(16):
+        s, t = (int(x) for x in rng.choice(V, 2, replace=False))
+        print(f"torus72k sssd {s}->{t}", flush=True)
+        d_gpu = g.SolveSSSD(s, t)
+        d_ref, *_ = og.sssd(s, t)
+        assert abs(d_gpu - d_ref) <= 1e-9 * d_ref, (s, t, d_gpu, d_ref)
+    for ns in (1, 5):
+        src = rng.choice(V, ns, replace=False).astype(np.int32)
+        wt = rng.uniform(0, 0.05, ns) if ns > 1 else np.zeros(1)
+        print(f"torus72k msad ns={ns}", flush=True)
+        dist, sid, _ = g.SolveMSAD(src, wt)
+        rdist, _, rsid, _ = og.msad(src, wt)
+        assert np.array_equal(dist, rdist)
+        assert np.mean(sid == rsid) > 0.99
