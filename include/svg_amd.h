@@ -74,6 +74,7 @@ typedef struct {
     int split_cap;          /* batched: split-table slots per source (0 = auto) */
     int autotune;           /* batched: learn from each build of this mesh (per K and
                                coefficients) the source order, table sizes and block shape */
+    int lanes_per_wave;     /* batched: sources stepped by one 64-wide wave (1..64; 0 = auto) */
 } svg_build_params;
 
 typedef struct {
@@ -93,6 +94,9 @@ typedef struct {
     long long counters[16]; /* event counters (win push/pop, pseudo push/pop, propagate,
                                children, split/vert writes, fan half-edges, pseudo
                                neighbours, -, -, backtrace hops, reached, -, -) */
+    int lanes_per_wave;     /* sources per wave of the propagation kernel */
+    int fallback_plain;     /* 1: a batched build ran every source on the plain path (the
+                               mesh has 2^24-1 or more half-edges) */
 } svg_build_stats;
 
 void svg_build_params_default(svg_build_params *p, int K);

@@ -25,7 +25,8 @@ class BuildParams(C.Structure):
     _fields_ = [("K", C.c_int), ("split_coef", C.c_uint32), ("vert_coef", C.c_uint32),
                 ("mode", C.c_int), ("plain_blocks", C.c_int), ("plain_threads", C.c_int),
                 ("lanes", C.c_int), ("prop_block", C.c_int), ("chunk", C.c_int),
-                ("vert_cap", C.c_int), ("split_cap", C.c_int), ("autotune", C.c_int)]
+                ("vert_cap", C.c_int), ("split_cap", C.c_int), ("autotune", C.c_int),
+                ("lanes_per_wave", C.c_int)]
 
 
 class BuildStats(C.Structure):
@@ -35,7 +36,8 @@ class BuildStats(C.Structure):
                 ("flagged_drop", C.c_int), ("flagged_overflow", C.c_int),
                 ("flagged_error", C.c_int), ("lanes", C.c_int), ("prop_block", C.c_int),
                 ("chunk", C.c_int), ("vert_cap", C.c_int), ("split_cap", C.c_int),
-                ("tuned", C.c_int), ("counters", C.c_longlong * 16)]
+                ("tuned", C.c_int), ("counters", C.c_longlong * 16), ("lanes_per_wave", C.c_int),
+                ("fallback_plain", C.c_int)]
 
 
 class ServerCfg(C.Structure):

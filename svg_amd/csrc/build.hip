@@ -228,10 +228,19 @@ std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& 
             err = "plain build: out of device memory for per-thread scratch";
             break;
         }
-        hipMemset(a.status, 0, sizeof(uint32_t) * (size_t)h.nv);
+        if (hipMemset(a.status, 0, sizeof(uint32_t) * (size_t)h.nv) != hipSuccess) {
+            err = "plain build: clearing the status array failed";
+            break;
+        }
         if (nList > 0) {
-            if (hipMalloc(&dList, sizeof(int32_t) * (size_t)nList) != hipSuccess) { err = "oom"; break; }
-            hipMemcpy(dList, srcList, sizeof(int32_t) * (size_t)nList, hipMemcpyHostToDevice);
+            if (hipMalloc(&dList, sizeof(int32_t) * (size_t)nList) != hipSuccess) {
+                err = "plain build: out of device memory for the source list";
+                break;
+            }
+            if (hipMemcpy(dList, srcList, sizeof(int32_t) * (size_t)nList, hipMemcpyHostToDevice) != hipSuccess) {
+                err = "plain build: source list upload failed";
+                break;
+            }
             a.srcList = dList;
         }
         st.allocSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -249,7 +258,8 @@ std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& 
         hipEventDestroy(e0); hipEventDestroy(e1);
         if (err.empty()) {
             std::vector<uint32_t> stv((size_t)h.nv);
-            hipMemcpy(stv.data(), a.status, sizeof(uint32_t) * (size_t)h.nv, hipMemcpyDeviceToHost);
+            const hipError_t ce = hipMemcpy(stv.data(), a.status, sizeof(uint32_t) * (size_t)h.nv, hipMemcpyDeviceToHost);
+            if (ce != hipSuccess) { err = std::string("plain build: status to host: ") + hipGetErrorString(ce); break; }
             if (st.status.size() != (size_t)h.nv) st.status.assign((size_t)h.nv, 0);
             for (int k = 0; k < a.nsrc; ++k) {
                 const int v = nList > 0 ? srcList[k] : k;
@@ -837,16 +847,19 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
 __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
 {
     extern __shared__ double ldsHeap[];       // (kHeapLds-1) keys then idx per thread
-    // the first lanesPerWave lanes of each wave run sources
+    // the first lanesPerWave lanes of each wave run sources; the LDS heap
+    // levels are laid out for those lanes only
     const int wl = threadIdx.x & 63;
     if (wl >= a.lanesPerWave) return;
-    const int lane = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * a.lanesPerWave + wl;
+    const int lanesPerBlock = (int)(blockDim.x >> 6) * a.lanesPerWave;
+    const int bl = (int)(threadIdx.x >> 6) * a.lanesPerWave + wl;
+    const int lane = (int)blockIdx.x * lanesPerBlock + bl;
     if (lane >= a.lanes) return;
     BatchedPolicy p;
     policy_mesh(p, a);
-    p.lstride = blockDim.x;
-    p.lk = ldsHeap + threadIdx.x;
-    p.li = reinterpret_cast<int*>(ldsHeap + (kHeapLds - 1) * blockDim.x) + threadIdx.x;
+    p.lstride = lanesPerBlock;
+    p.lk = ldsHeap + bl;
+    p.li = reinterpret_cast<int*>(ldsHeap + (kHeapLds - 1) * lanesPerBlock) + bl;
     p.wh = a.wheap + (size_t)lane * kHeapSlots;
     p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
     p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
@@ -985,7 +998,15 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
 {
     const int K = c.K;
     const int V = h.nv;
-    if (h.ne >= (int)kEnterNone) return "batched build supports fewer than 2^24-1 half-edges";
+    if (h.ne >= (int)kEnterNone) {
+        // enter edges are packed into 24 bits of a vertex-table slot: larger
+        // meshes run every source on the plain path (same rows)
+        std::string e = build_plain(h, d, c, g, st, nullptr, 0);
+        st.fallbackPlain = 1;
+        return e;
+    }
+    const int lanesPerWave = c.lanesPerWave > 0 ? c.lanesPerWave : kLanesPerWave;
+    if (lanesPerWave > 64) return "lanes_per_wave must be at most 64";
     const int vrefcap = (int)((unsigned)K * c.vertCoef);
     const int srefcap = (int)((unsigned)K * c.splitCoef);
     unsigned vcap = c.vertCap > 0 ? next_pow2((unsigned)c.vertCap) : next_pow2(2u * (unsigned)vrefcap + 2u);
@@ -1032,10 +1053,17 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     }
     if (vcap < 64) vcap = 64;
     if (scap < 64) scap = 64;
-    if (vcap > 8192) return "vertex table larger than the extraction sort buffer (8192)";
+    // The extraction sorts a source's reached list in LDS (at most 8192
+    // entries): larger tables are capped there, and a source that reaches
+    // more vertices than the capped table holds is flagged and recomputed
+    // on the plain path (large K).
+    if (vcap > 8192) vcap = 8192;
     const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
     size_t freeB = 0, totB = 0;
-    hipMemGetInfo(&freeB, &totB);
+    {
+        const hipError_t me = hipMemGetInfo(&freeB, &totB);
+        if (me != hipSuccess) return std::string("batched build: hipMemGetInfo: ") + hipGetErrorString(me);
+    }
     const size_t perLane = sizeof(BHeapEnt) * kHeapSlots +
                            (sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
     size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
@@ -1053,7 +1081,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         if ((size_t)chunk > maxChunk) chunk = (int)std::max<size_t>(maxChunk, 1);
     }
     if (chunk > V) chunk = V;
-    st.lanes = lanes; st.propBlock = block; st.chunk = chunk; st.vertCap = (int)vcap; st.splitCap = (int)scap;
+    st.lanes = lanes; st.propBlock = block; st.lanesPerWave = lanesPerWave; st.chunk = chunk; st.vertCap = (int)vcap; st.splitCap = (int)scap;
 
     auto t0 = std::chrono::steady_clock::now();
     BArgs a;
@@ -1061,7 +1089,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
     a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
     a.pseudoShare = kPseudoShare;
-    a.lanesPerWave = kLanesPerWave;
+    a.lanesPerWave = lanesPerWave;
     std::string err;
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
@@ -1097,13 +1125,17 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     for (int base = 0; err.empty() && base < V; base += chunk) {
         const int n = std::min(chunk, V - base);
         a.nsrc = n; a.srcBase = base;
-        hipMemsetAsync(a.vtab, 0xFF, sizeof(BSlotV) * vcap * (size_t)n, 0);
-        hipMemsetAsync(a.stab, 0xFF, sizeof(BSlotS) * scap * (size_t)n, 0);
-        hipMemsetAsync(a.work, 0, sizeof(int), 0);
+        {
+            hipError_t ce = hipMemsetAsync(a.vtab, 0xFF, sizeof(BSlotV) * vcap * (size_t)n, 0);
+            if (ce == hipSuccess) ce = hipMemsetAsync(a.stab, 0xFF, sizeof(BSlotS) * scap * (size_t)n, 0);
+            if (ce == hipSuccess) ce = hipMemsetAsync(a.work, 0, sizeof(int), 0);
+            if (ce != hipSuccess) { err = std::string("batched build: table reset: ") + hipGetErrorString(ce); break; }
+        }
         hipEventRecord(e0, 0);
         const int pl = std::min(lanes, n);
         a.lanes = pl;
-        const size_t ldsBytes = (size_t)(kHeapLds - 1) * block * (sizeof(double) + sizeof(int));
+        const size_t ldsBytes = (size_t)(kHeapLds - 1) * (size_t)(block / 64) * (size_t)a.lanesPerWave *
+                                (sizeof(double) + sizeof(int));
         const size_t threads = (size_t)((pl + a.lanesPerWave - 1) / a.lanesPerWave) * 64;
         hipLaunchKernelGGL(batched_prop_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), ldsBytes,
                            0, a);
@@ -1124,8 +1156,13 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         const double pms = elapsed_ms(e0, e1), ems = elapsed_ms(e1, e2);
         st.propSeconds += pms * 1e-3; st.extractSeconds += ems * 1e-3; st.seconds += (pms + ems) * 1e-3;
         hs.resize((size_t)n); hc.resize((size_t)n * kNumBatchedCounters);
-        hipMemcpy(hs.data(), a.status, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost);
-        hipMemcpy(hc.data(), a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)n, hipMemcpyDeviceToHost);
+        {
+            // the redo list is built from these flags: never from a copy that failed
+            hipError_t ce = hipMemcpy(hs.data(), a.status, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost);
+            if (ce == hipSuccess)
+                ce = hipMemcpy(hc.data(), a.cnt, sizeof(int32_t) * kNumBatchedCounters * (size_t)n, hipMemcpyDeviceToHost);
+            if (ce != hipSuccess) { err = std::string("batched build: status to host: ") + hipGetErrorString(ce); break; }
+        }
         for (int i = 0; i < n; ++i) {
             const int src = order.empty() ? base + i : order[(size_t)base + i];
             st.status[src] = hs[i];

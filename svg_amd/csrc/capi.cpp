@@ -167,6 +167,7 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     c.plainBlocks = p->plain_blocks; c.plainThreads = p->plain_threads;
     c.lanes = p->lanes; c.propBlock = p->prop_block; c.chunk = p->chunk;
     c.vertCap = p->vert_cap; c.splitCap = p->split_cap; c.autotune = p->autotune;
+    c.lanesPerWave = p->lanes_per_wave;
     if (c.plainThreads > 1024 || (c.plainThreads % 64) != 0 && c.plainThreads > 0)
         return svg_set_error(SVG_E_ARG, "plain_threads must be a multiple of 64 up to 1024");
     svg_graph_s* g = new (std::nothrow) svg_graph_s();
@@ -194,7 +195,11 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     }
     if (!e.empty()) { free_graph(g->d); delete g; return svg_set_error(SVG_E_HIP, e); }
     std::vector<int32_t> tails((size_t)m->h.nv);
-    hipMemcpy(tails.data(), g->d.tail, sizeof(int32_t) * tails.size(), hipMemcpyDeviceToHost);
+    const hipError_t te = hipMemcpy(tails.data(), g->d.tail, sizeof(int32_t) * tails.size(), hipMemcpyDeviceToHost);
+    if (te != hipSuccess) {
+        free_graph(g->d); delete g;
+        return svg_set_error(SVG_E_HIP, std::string("row lengths to host: ") + hipGetErrorString(te));
+    }
     long long arcs = 0;
     for (int32_t t : tails) arcs += t;
     st.arcs = arcs;
@@ -216,6 +221,7 @@ extern "C" int svg_build_stats_get(svg_graph_t g, svg_build_stats* st)
     st->lanes = s.lanes; st->prop_block = s.propBlock; st->chunk = s.chunk;
     st->vert_cap = s.vertCap; st->split_cap = s.splitCap; st->tuned = s.tuned;
     for (size_t i = 0; i < s.counters.size() && i < 16; ++i) st->counters[i] = s.counters[i];
+    st->lanes_per_wave = s.lanesPerWave; st->fallback_plain = s.fallbackPlain;
     return SVG_OK;
 }
 
@@ -446,14 +452,18 @@ extern "C" int svg_graph_load(svg_mesh_t m, int K, const char* path, svg_graph_t
     g->mesh = m;
     std::string e = alloc_graph(V, K, g->d);
     if (!e.empty()) { free_graph(g->d); delete g; return svg_set_error(SVG_E_MEM, e); }
-    hipMemcpy(g->d.tail, tails.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
-    hipMemcpy(g->d.adj, adj.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
-    hipMemcpy(g->d.w, w.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-    hipMemcpy(g->d.srcEdge, se.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
-    hipMemcpy(g->d.dstEdge, de.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
-    hipMemcpy(g->d.srcX, sx.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-    hipError_t he = hipMemcpy(g->d.dstX, dx.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-    if (he != hipSuccess) { free_graph(g->d); delete g; return svg_set_error(SVG_E_HIP, hipGetErrorString(he)); }
+    // every copy is checked: a handle over partly uploaded rows is never returned
+    hipError_t he = hipMemcpy(g->d.tail, tails.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.adj, adj.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.w, w.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.srcEdge, se.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.dstEdge, de.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.srcX, sx.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(g->d.dstX, dx.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        free_graph(g->d); delete g;
+        return svg_set_error(SVG_E_HIP, std::string(".svg upload: ") + hipGetErrorString(he));
+    }
     g->arcs = arcs;
     g->stats.arcs = arcs;
     *out = g;

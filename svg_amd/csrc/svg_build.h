@@ -35,6 +35,7 @@ struct BuildConfig {
     int chunk = 0;               // sources whose tables are resident at once
     int vertCap = 0;             // per-source vertex table slots (power of two)
     int splitCap = 0;            // per-source split table slots (power of two)
+    int lanesPerWave = 0;        // source lanes per 64-wide wave of the propagation kernel
     int autotune = 0;
     TuneRecord* tune = nullptr;  // autotune: what earlier builds of this mesh recorded
 };
@@ -50,6 +51,8 @@ struct BuildStats {
     int flaggedDrop = 0, flaggedOverflow = 0, flaggedError = 0;
     int lanes = 0, propBlock = 0, chunk = 0, vertCap = 0, splitCap = 0;
     int tuned = 0;               // 1: order, capacities and block came from the autotuner
+    int lanesPerWave = 0;
+    int fallbackPlain = 0;       // 1: the mesh exceeds the batched form's limits; every source ran plain
     std::vector<long long> counters;   // summed event counters (CNT_NUM)
     std::vector<uint32_t> status;      // per-vertex status bits (SF_*)
 };
