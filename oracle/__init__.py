@@ -21,6 +21,7 @@ COUNTERS = [
     "fan_halfedge", "pseudo_neigh", "win_drop", "pseudo_drop", "max_winq", "max_pseudoq",
     "reached", "total_calc", "bt_hops", "arcs", "vert_overflow", "split_overflow",
     "vert_keys", "split_keys", "level_skip", "errors", "iterations",
+    "winq_left", "pseudoq_left", "total_prev",
 ]
 
 ROW_DTYPE = np.dtype([
@@ -78,6 +79,15 @@ def lib():
     L.oracle_selftest_heap.argtypes = [_dp, C.c_int, _ip]
     L.oracle_selftest_hash.argtypes = [C.c_uint, _up, C.c_int, _ip]
     L.oracle_acos_many.argtypes = [_dp, _dp, C.c_int]
+    L.oracle_selftest_winq.argtypes = [_dp, C.c_int, _ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.oracle_selftest_pseudoq.argtypes = [_dp, C.c_int, _ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.oracle_selftest_valid.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, _ip, _dp, C.c_int]
+    L.oracle_selftest_propagate.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, C.c_int, _ip, _dp,
+                                            C.c_double, C.c_double, _ip, _dp, C.POINTER(C.c_int), _lp,
+                                            _dp]
+    L.oracle_ich_trace.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _lp]
     assert L.oracle_num_counters() == len(COUNTERS)
     _lib = L
     return L
@@ -259,6 +269,69 @@ def hash_slots(size: int, keys):
     slots = np.empty(len(keys), np.int32)
     ov = lib().oracle_selftest_hash(size, keys, len(keys), slots)
     return slots, bool(ov)
+
+
+def queue_order(keys, pseudo: bool = False):
+    """Push keys into the window (or pseudo-window) queue exactly as the build
+    sizes it, then pop everything: (ids in pop order, kept, dropped)."""
+    keys = np.ascontiguousarray(keys, dtype=np.float64)
+    order = np.full(len(keys), -1, np.int32)
+    kept, dropped = C.c_int(), C.c_int()
+    f = lib().oracle_selftest_pseudoq if pseudo else lib().oracle_selftest_winq
+    f(keys, len(keys), order, C.byref(kept), C.byref(dropped))
+    return order[: kept.value], kept.value, dropped.value
+
+
+def _preset(vdist):
+    vids = np.ascontiguousarray(list(vdist.keys()), dtype=np.int32)
+    vd = np.ascontiguousarray([vdist[k] for k in vdist], dtype=np.float64)
+    return len(vids), vids, vd
+
+
+def window_valid(mesh: Mesh, edge: int, b0, b1, d0, d1, ps, vdist: dict, is_left: bool) -> bool:
+    """IsValidWindow of one window on `edge` with the given vertex distances
+    (vertex id -> distance; absent vertices are unreached)."""
+    n, vids, vd = _preset(vdist or {0: float(np.finfo(np.float64).max)})
+    r = lib().oracle_selftest_valid(mesh.h, edge, b0, b1, d0, d1, ps, n, vids, vd, int(is_left))
+    if r < 0:
+        raise ValueError("bad edge")
+    return bool(r)
+
+
+def propagate_one(mesh: Mesh, src: int, edge: int, b0, b1, d0, d1, ps, vdist: dict,
+                  split=None):
+    """PropagateWindow of one window on `edge` from vertex source src, with
+    vertex distances preset and the split record of twin(edge) preset to
+    split = (dist, x).  Returns dict: children [(edge, b0, b1)] in pop order,
+    counters, far_dist, split (dist, x) afterwards."""
+    n, vids, vd = _preset(vdist or {src: 0.0})
+    ce = np.zeros(2, np.int32)
+    cb = np.zeros(4, np.float64)
+    nc = C.c_int()
+    cnt = np.zeros(len(COUNTERS), np.int64)
+    out = np.zeros(3, np.float64)
+    big = float(np.finfo(np.float64).max)
+    sd, sx = split if split is not None else (big, big)
+    r = lib().oracle_selftest_propagate(mesh.h, src, edge, b0, b1, d0, d1, ps, n, vids, vd, sd, sx,
+                                        ce, cb, C.byref(nc), cnt, out)
+    if r < 0:
+        raise ValueError("bad edge")
+    kids = [(int(ce[i]), float(cb[2 * i]), float(cb[2 * i + 1])) for i in range(nc.value)]
+    return dict(children=kids, counters=dict(zip(COUNTERS, cnt.tolist())), far_dist=float(out[0]),
+                split=(float(out[1]), float(out[2])))
+
+
+def ich_trace(mesh: Mesh, src: int, K: int = -1, cap: int = 100000):
+    """Execute(K) from src with its choices recorded: (kind[n] 0 window /
+    1 pseudo-window, window-top key[n], pseudo-top key[n], counters)."""
+    kind = np.zeros(cap, np.int32)
+    wk = np.zeros(cap, np.float64)
+    pk = np.zeros(cap, np.float64)
+    cnt = np.zeros(len(COUNTERS), np.int64)
+    n = lib().oracle_ich_trace(mesh.h, src, K, cap, kind, wk, pk, cnt)
+    if n < 0:
+        raise ValueError("bad source")
+    return kind[:n], wk[:n], pk[:n], dict(zip(COUNTERS, cnt.tolist()))
 
 
 def acos(x):

@@ -400,6 +400,8 @@ enum {
     C_FAN_HALFEDGE, C_PSEUDO_NEIGH, C_WIN_DROP, C_PSEUDO_DROP, C_MAX_WINQ, C_MAX_PSEUDOQ,
     C_REACHED, C_TOTAL_CALC, C_BT_HOPS, C_ARCS, C_VERT_OVERFLOW, C_SPLIT_OVERFLOW,
     C_VERT_KEYS, C_SPLIT_KEYS, C_LEVEL_SKIP, C_ERRORS, C_ITER,
+    C_WINQ_LEFT, C_PSEUDOQ_LEFT,      /* queue lengths when Execute returned */
+    C_TOTAL_PREV,                     /* totalCalcVertNum before the last main pop */
     C_NCOUNTERS
 };
 
@@ -426,7 +428,29 @@ typedef struct {
     OWindow stored[O_STORED_WIN];     /* storedWindows */
     int nStored;
     int stalled;                      /* the queue tops were unordered (NaN keys) */
+    /* optional record of Execute's choices (ICHDevice.cu:88, 110): per main
+     * pop the kind taken (0 window, 1 pseudo-window) and both queue tops'
+     * keys (DBL_MAX for an empty queue) */
+    int *trKind; double *trW, *trP; int trCap, trN;
 } OIch;
+
+/* The queues exactly as the build sizes them (SVG.cu:19-22): arrays of
+ * WINPQ_SIZE / PSEUDOWINPQ_SIZE items, one of them the unused slot 0, so
+ * push drops once WINPQ_SIZE - 1 items are queued (PriorityQueue.cuh:53). */
+static int winq_init(OWinQ *q)
+{
+    q->a = (OWinQItem *)malloc(sizeof(OWinQItem) * O_WINPQ_SIZE);
+    q->maxSize = O_WINPQ_SIZE - 1;
+    q->tail = 1; q->drops = 0; q->maxLen = 0;
+    return q->a != NULL;
+}
+static int pseudoq_init(OPseudoQ *q)
+{
+    q->a = (OPseudoQItem *)malloc(sizeof(OPseudoQItem) * O_PSEUDOWINPQ_SIZE);
+    q->maxSize = O_PSEUDOWINPQ_SIZE - 1;
+    q->tail = 1; q->drops = 0; q->maxLen = 0;
+    return q->a != NULL;
+}
 
 static void ich_init(OIch *s, const OMesh *m, int K, unsigned splitCoef, unsigned vertCoef)
 {
@@ -436,10 +460,8 @@ static void ich_init(OIch *s, const OMesh *m, int K, unsigned splitCoef, unsigne
     s->vert.size = (unsigned)K * vertCoef;
     s->split.data = (OSplitItem *)malloc(sizeof(OSplitItem) * (s->split.size + 1));
     s->vert.data = (OVItem *)malloc(sizeof(OVItem) * (s->vert.size + 1));
-    s->winQ.a = (OWinQItem *)malloc(sizeof(OWinQItem) * O_WINPQ_SIZE);
-    s->pseudoQ.a = (OPseudoQItem *)malloc(sizeof(OPseudoQItem) * O_PSEUDOWINPQ_SIZE);
-    s->winQ.maxSize = O_WINPQ_SIZE - 1;
-    s->pseudoQ.maxSize = O_PSEUDOWINPQ_SIZE - 1;
+    winq_init(&s->winQ);
+    pseudoq_init(&s->pseudoQ);
     s->hopCap = 4L * ((long)m->nf + (long)m->nv) + 16;
 }
 
@@ -878,6 +900,16 @@ static void gen_sub_wins(OIch *s, const OPseudo *pw)
     } while (cur != start && cur != -1);
 }
 
+/* Records a main pop of Execute when a trace buffer is attached. */
+static void ich_trace(OIch *s, int kind)
+{
+    if (!s->trKind || s->trN >= s->trCap) return;
+    s->trKind[s->trN] = kind;
+    s->trW[s->trN] = s->winQ.tail != 1 ? s->winQ.a[1].item.minDist : DBL_MAX;
+    s->trP[s->trN] = s->pseudoQ.tail != 1 ? s->pseudoQ.a[1].item.dist : DBL_MAX;
+    s->trN++;
+}
+
 /* ICHDevice::Execute (ICHDevice.cu:67-132) for a vertex source. */
 static void ich_execute(OIch *s, int K)
 {
@@ -885,6 +917,7 @@ static void ich_execute(OIch *s, int K)
     ich_initialize(s);
     while (s->winQ.tail != 1 || s->pseudoQ.tail != 1) {
         s->cnt[C_ITER]++;
+        s->cnt[C_TOTAL_PREV] = s->totalCalcVertNum;
         while (s->winQ.tail != 1 &&
                s->winQ.a[1].item.pseudoSrcId < (unsigned)m->nv &&
                s->winQ.a[1].item.pseudoSrcBirthTime !=
@@ -903,12 +936,14 @@ static void ich_execute(OIch *s, int K)
         int wEmpty = s->winQ.tail == 1, pEmpty = s->pseudoQ.tail == 1;
         if (!wEmpty && (pEmpty || s->winQ.a[1].item.minDist < s->pseudoQ.a[1].item.dist)) {
             OWindow win = s->winQ.a[1].item;
+            ich_trace(s, 0);
             OWinQ_pop(&s->winQ);
             s->cnt[C_WIN_POP]++;
             if (win.level > m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
             propagate_window(s, &win);
         } else if (!pEmpty && (wEmpty || s->winQ.a[1].item.minDist >= s->pseudoQ.a[1].item.dist)) {
             OPseudo pw = s->pseudoQ.a[1].item;
+            ich_trace(s, 1);
             OPseudoQ_pop(&s->pseudoQ);
             s->cnt[C_PSEUDO_POP]++;
             if (pw.level >= (unsigned)m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
@@ -916,6 +951,8 @@ static void ich_execute(OIch *s, int K)
         }
         if (K != -1 && s->totalCalcVertNum >= K) break;
     }
+    s->cnt[C_WINQ_LEFT] = s->winQ.tail - 1;
+    s->cnt[C_PSEUDOQ_LEFT] = s->pseudoQ.tail - 1;
 }
 
 /* ICHDevice::BuildGeodesicPathTo(vertId, ...) (ICHDevice.cu:336-480) + D9. */
@@ -1729,6 +1766,148 @@ int oracle_selftest_hash(unsigned size, const unsigned *keys, int n, int *slot_o
     int ov = t.overflowed;
     free(t.data);
     return ov;
+}
+
+/* The window queue as the build configures it (winq_init): pushes n keys
+ * (item i carries edge id i), then pops everything; order_out receives the
+ * ids in pop order, *kept how many were queued, *dropped how many pushes
+ * were refused. */
+int oracle_selftest_winq(const double *keys, int n, int *order_out, int *kept, int *dropped)
+{
+    OWinQ q;
+    if (!winq_init(&q)) return -1;
+    for (int i = 0; i < n; ++i) {
+        OWindow w; memset(&w, 0, sizeof(w));
+        w.edgeID = (unsigned)i; w.minDist = keys[i];
+        OWinQ_push(&q, &w, keys[i]);
+    }
+    *kept = q.tail - 1;
+    *dropped = (int)q.drops;
+    for (int i = 0; q.tail != 1; ++i) {
+        order_out[i] = (int)q.a[1].item.edgeID;
+        OWinQ_pop(&q);
+    }
+    free(q.a);
+    return 0;
+}
+
+/* Same for the pseudo-window queue. */
+int oracle_selftest_pseudoq(const double *keys, int n, int *order_out, int *kept, int *dropped)
+{
+    OPseudoQ q;
+    if (!pseudoq_init(&q)) return -1;
+    for (int i = 0; i < n; ++i) {
+        OPseudo p; memset(&p, 0, sizeof(p));
+        p.vertID = (unsigned)i; p.dist = keys[i];
+        OPseudoQ_push(&q, &p, keys[i]);
+    }
+    *kept = q.tail - 1;
+    *dropped = (int)q.drops;
+    for (int i = 0; q.tail != 1; ++i) {
+        order_out[i] = (int)q.a[1].item.vertID;
+        OPseudoQ_pop(&q);
+    }
+    free(q.a);
+    return 0;
+}
+
+/* Vertex distances of the ICH state for the selftests below: vids[i] gets
+ * dist vd[i] (DBL_MAX leaves it unreached). */
+static void preset_verts(OIch *s, int nv, const int *vids, const double *vd)
+{
+    for (int i = 0; i < nv; ++i)
+        if (vd[i] != DBL_MAX) vert_w(&s->vert, (unsigned)vids[i])->dist = vd[i];
+}
+
+/* IsValidWindow (ICHDevice.cu:906-955) on one window of edge `edge`:
+ * (b0, b1, d0, d1, ps) as the Window fields, vertex distances preset for
+ * the vertices vids[0..nv). Returns 1 valid, 0 rejected, -1 bad input. */
+int oracle_selftest_valid(const OMesh *m, int edge, double b0, double b1, double d0, double d1,
+                          double ps, int nv, const int *vids, const double *vd, int isLeft)
+{
+    if (edge < 0 || edge >= m->ne) return -1;
+    OIch s;
+    long long cnt[C_NCOUNTERS];
+    ich_init(&s, m, 8, 10, 2);
+    s.cnt = cnt;
+    memset(cnt, 0, sizeof(cnt));
+    ich_clear(&s);
+    preset_verts(&s, nv, vids, vd);
+    OWindow w; memset(&w, 0, sizeof(w));
+    w.edgeID = (unsigned)edge;
+    w.b0 = b0; w.b1 = b1; w.d0 = d0; w.d1 = d1; w.pseudoSrcDist = ps;
+    win_calc_min(&w);
+    int r = is_valid_window(&s, &w, isLeft);
+    ich_destroy(&s);
+    return r;
+}
+
+/* PropagateWindow (ICHDevice.cu:618-726) of one window of edge `edge` from
+ * source vertex `src`: vertex distances preset as above, the split record of
+ * twin(edge) preset to (split_dist, split_x) unless split_dist is DBL_MAX.
+ * Outputs: child_edge[2], child_b[4] (b0, b1 per child) of the windows it
+ * queued in pop order, *nchild their number, counters[C_NCOUNTERS],
+ * out_state[3] = the far vertex's distance and the split record (dist, x)
+ * afterwards.  Returns 0, or -1 on bad input. */
+int oracle_selftest_propagate(const OMesh *m, int src, int edge, double b0, double b1, double d0,
+                              double d1, double ps, int nv, const int *vids, const double *vd,
+                              double split_dist, double split_x, int *child_edge, double *child_b,
+                              int *nchild, long long *counters, double *out_state)
+{
+    if (edge < 0 || edge >= m->ne || m->e[edge].twin == -1) return -1;
+    OIch s;
+    ich_init(&s, m, 8, 10, 2);
+    s.cnt = counters;
+    memset(counters, 0, sizeof(long long) * C_NCOUNTERS);
+    ich_clear(&s);
+    s.sourceVert = (unsigned)src;
+    preset_verts(&s, nv, vids, vd);
+    unsigned e0 = (unsigned)m->e[edge].twin;
+    if (split_dist != DBL_MAX) {
+        OSplit *sp = split_w(&s.split, e0);
+        sp->dist = split_dist; sp->x = split_x; sp->pseudoSrcId = (unsigned)src;
+    }
+    OWindow w; memset(&w, 0, sizeof(w));
+    w.edgeID = (unsigned)edge;
+    w.b0 = b0; w.b1 = b1; w.d0 = d0; w.d1 = d1; w.pseudoSrcDist = ps;
+    w.srcID = (unsigned)src; w.pseudoSrcId = (unsigned)src;
+    win_calc_min(&w);
+    propagate_window(&s, &w);
+    *nchild = 0;
+    while (s.winQ.tail != 1 && *nchild < 2) {
+        child_edge[*nchild] = (int)s.winQ.a[1].item.edgeID;
+        child_b[2 * *nchild] = s.winQ.a[1].item.b0;
+        child_b[2 * *nchild + 1] = s.winQ.a[1].item.b1;
+        ++*nchild;
+        OWinQ_pop(&s.winQ);
+    }
+    out_state[0] = vert_r(&s.vert, (unsigned)m->e[m->e[e0].next].v1)->dist;
+    out_state[1] = split_r(&s.split, e0)->dist;
+    out_state[2] = split_r(&s.split, e0)->x;
+    ich_destroy(&s);
+    return 0;
+}
+
+/* Execute(K) from vertex src with Execute's choices recorded (ich_trace):
+ * kind[i], wkey[i], pkey[i] for the first cap main pops; returns how many
+ * were recorded (-1 bad input).  counters as oracle_ich_single. */
+int oracle_ich_trace(const OMesh *m, int src, int K, int cap, int *kind, double *wkey, double *pkey,
+                     long long *counters)
+{
+    if (src < 0 || src >= m->nv || cap < 0) return -1;
+    OIch s;
+    int tabK = K > 0 ? K : m->nv;
+    ich_init(&s, m, tabK, 10, 2);
+    s.cnt = counters;
+    memset(counters, 0, sizeof(long long) * C_NCOUNTERS);
+    ich_clear(&s);
+    s.trKind = kind; s.trW = wkey; s.trP = pkey; s.trCap = cap; s.trN = 0;
+    s.sourceVert = (unsigned)src;
+    ich_execute(&s, K);
+    counters[C_TOTAL_CALC] = s.totalCalcVertNum;
+    int n = s.trN;
+    ich_destroy(&s);
+    return n;
 }
 
 /* Vectorised o_acos for the tests. */
