@@ -154,8 +154,15 @@ int svg_sssd_point(svg_graph_t g, int f0, const double *p0, int f1, const double
  * graph stages share the solver's slots. */
 int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t *faces, const double *points, double *dist,
                          int32_t *key_edge, double *key_x, int32_t *last_vert, int32_t *route);
-/* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]. */
+/* Many SSSD queries in shared launches: st[2*nq] pairs -> out[nq]
+ * (SVG::SolveSSSD, SVG.cu:241-251, once per pair). */
 int svg_sssd_batch(svg_graph_t g, const int32_t *st, int nq, double *out);
+/* Many MSAD queries in shared launches (SVG::SolveMSAD, SVG.cu:354-367, once
+ * per query): query i has sources src[offsets[i] .. offsets[i+1]) with
+ * weights w[same range] (offsets[nq+1], increasing); dist[nq*V] and
+ * src_id[nq*V] (may be NULL) receive each query's V values in turn. */
+int svg_msad_batch(svg_graph_t g, const int32_t *src, const double *w, const int64_t *offsets, int nq,
+                   double *dist, int32_t *src_id);
 
 /* ------------------------------------------------------- geodesic paths */
 /* The surface polyline of an SVG arc u -> v from the key points its row
@@ -210,12 +217,25 @@ typedef struct {
     /* work of the completed queries, per kind (0 = SSSD, 1 = MSAD): queries,
      * shared rounds they took part in, frontier entries expanded, arcs scanned */
     long long n_kind[2], rounds_kind[2], settled_kind[2], relaxed_kind[2];
+    int slots;              /* request slots admitted by the memory budget */
+    long long slot_bytes;   /* device memory of one slot */
 } svg_server_stats;
 
 int svg_server_start(svg_graph_t g, const svg_server_cfg *cfg, svg_server_t *out);
-/* Enqueue a request; returns a ticket >= 0 or SVG_E_BUSY. */
+/* Enqueue a request; returns a ticket >= 0 or SVG_E_BUSY.  Replaces the
+ * reference's one-query-at-a-time driver loops (kernel.cu:63-92, 336-389). */
 long long svg_server_submit(svg_server_t s, const svg_request *req);
-/* Block until ticket completes; latency in milliseconds from submit. */
+/* Completion callback: called once per request from a server worker thread
+ * (never while a server lock is held) with the ticket, SVG_OK or an error
+ * code, and the latency from submit in milliseconds (-1 on error); the
+ * request's outputs are complete when it runs.  It may submit further
+ * requests but must not wait for tickets or stop the server. */
+typedef void (*svg_done_fn)(long long ticket, int status, double latency_ms, void *user);
+/* svg_server_submit with a completion callback; such a ticket is not
+ * waited for (svg_server_wait rejects it). */
+long long svg_server_submit_cb(svg_server_t s, const svg_request *req, svg_done_fn cb, void *user);
+/* Block until ticket completes; latency in milliseconds from submit.  The
+ * ticket is consumed: a second wait on it is an error. */
 int svg_server_wait(svg_server_t s, long long ticket, double *latency_ms);
 int svg_server_stats_get(svg_server_t s, svg_server_stats *st);
 int svg_server_stop(svg_server_t s);

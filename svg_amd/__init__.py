@@ -273,8 +273,12 @@ class SVG:
 
     def SolveMSAD(self, sources, weights, want_parent: bool = True):   # SVG.cu:354-367
         V, _, _ = self.info()
-        src = np.ascontiguousarray(sources, dtype=np.int32)
-        w = np.ascontiguousarray(weights, dtype=np.float64)
+        src = np.ascontiguousarray(sources, dtype=np.int32).ravel()
+        w = np.ascontiguousarray(weights, dtype=np.float64).ravel()
+        if len(src) == 0 or len(w) != len(src):
+            raise ValueError("SolveMSAD needs one weight per source and at least one source")
+        if np.any(src < 0) or np.any(src >= V):
+            raise ValueError("source vertex out of range")
         dist = np.empty(V, np.float64)
         sid = np.empty(V, np.int32) if want_parent else None
         par = np.empty(V, np.int32) if want_parent else None
@@ -292,6 +296,29 @@ class SVG:
         out = np.empty((max(n.value, 1), 3), np.float64)
         check(lib().svg_geodesic_path(self.g, v, len(v), out.reshape(-1), n.value, C.byref(n)))
         return out[: n.value]
+
+    def SolveMSADBatch(self, queries, want_srcid: bool = True):
+        """Many MSAD queries in shared launches: queries is a list of
+        (sources, weights).  Returns (dist[nq, V], src_id[nq, V] or None)."""
+        V, _, _ = self.info()
+        srcs, ws, offs = [], [], [0]
+        for s_, w_ in queries:
+            s_ = np.asarray(s_, dtype=np.int32).ravel()
+            w_ = np.asarray(w_, dtype=np.float64).ravel()
+            if len(s_) == 0 or len(s_) != len(w_):
+                raise ValueError("every query needs one weight per source and at least one source")
+            if np.any(s_ < 0) or np.any(s_ >= V):
+                raise ValueError("source vertex out of range")
+            srcs.append(s_)
+            ws.append(w_)
+            offs.append(offs[-1] + len(s_))
+        nq = len(srcs)
+        src = np.ascontiguousarray(np.concatenate(srcs) if nq else np.zeros(0, np.int32))
+        w = np.ascontiguousarray(np.concatenate(ws) if nq else np.zeros(0, np.float64))
+        dist = np.empty((max(nq, 1), V), np.float64)
+        sid = np.empty((max(nq, 1), V), np.int32) if want_srcid else None
+        check(lib().svg_msad_batch(self.g, src, w, np.asarray(offs, np.int64), nq, dist.reshape(-1), _ptr(sid)))
+        return dist[:nq], (sid[:nq] if want_srcid else None)
 
     def SolveSSSDBatch(self, pairs):
         st = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1))

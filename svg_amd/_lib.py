@@ -58,7 +58,8 @@ class ServerStats(C.Structure):
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("qps", C.c_double), ("max_batch_seen", C.c_int), ("launches", C.c_longlong),
                 ("n_kind", C.c_longlong * 2), ("rounds_kind", C.c_longlong * 2),
-                ("settled_kind", C.c_longlong * 2), ("relaxed_kind", C.c_longlong * 2)]
+                ("settled_kind", C.c_longlong * 2), ("relaxed_kind", C.c_longlong * 2),
+                ("slots", C.c_int), ("slot_bytes", C.c_longlong)]
 
 
 COUNTER_NAMES = ["win_push", "win_pop", "pseudo_push", "pseudo_pop", "propagate",
@@ -72,9 +73,13 @@ EXPORTS = [
     "svg_mesh_upload", "svg_mesh_info", "svg_mesh_export", "svg_mesh_free",
     "svg_build_params_default", "svg_build", "svg_build_stats_get", "svg_build_status",
     "svg_graph_info", "svg_graph_to_host", "svg_graph_save", "svg_graph_load", "svg_graph_free",
-    "svg_sssd", "svg_msad", "svg_sssd_point", "svg_sssd_point_batch", "svg_sssd_batch", "svg_server_start", "svg_server_submit",
+    "svg_sssd", "svg_msad", "svg_sssd_point", "svg_sssd_point_batch", "svg_sssd_batch", "svg_msad_batch",
+    "svg_server_start", "svg_server_submit", "svg_server_submit_cb",
     "svg_server_wait", "svg_server_stats_get", "svg_server_stop", "svg_server_replay",
 ]
+
+# svg_done_fn: void (*)(long long ticket, int status, double latency_ms, void *user)
+DONE_FN = C.CFUNCTYPE(None, C.c_longlong, C.c_int, C.c_double, C.c_void_p)
 
 _lib = None
 
@@ -116,6 +121,8 @@ def lib():
                            C.POINTER(C.c_int)]
     L.svg_msad.argtypes = [_vp, _ip, _dp, C.c_int, _dp, _vp, _vp]
     L.svg_sssd_batch.argtypes = [_vp, _ip, C.c_int, _dp]
+    L.svg_msad_batch.argtypes = [_vp, _ip, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                 C.c_int, _dp, _vp]
     L.svg_sssd_point_batch.argtypes = [_vp, C.c_int, _ip, _dp, _dp, _ip, _dp, _ip, _ip]
     L.svg_sssd_point.argtypes = [_vp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_double), _ip, _dp,
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int), _vp, C.c_int, C.POINTER(C.c_int)]
@@ -126,6 +133,8 @@ def lib():
         L.svg_server_start.argtypes = [_vp, C.POINTER(ServerCfg), C.POINTER(_vp)]
         L.svg_server_submit.argtypes = [_vp, C.POINTER(Request)]
         L.svg_server_submit.restype = C.c_longlong
+        L.svg_server_submit_cb.argtypes = [_vp, C.POINTER(Request), DONE_FN, C.c_void_p]
+        L.svg_server_submit_cb.restype = C.c_longlong
         L.svg_server_wait.argtypes = [_vp, C.c_longlong, C.POINTER(C.c_double)]
         L.svg_server_stats_get.argtypes = [_vp, C.POINTER(ServerStats)]
         L.svg_server_stop.argtypes = [_vp]

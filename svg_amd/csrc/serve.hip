@@ -22,6 +22,7 @@
 #include <deque>
 #include <memory>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "solve.h"
@@ -42,7 +43,24 @@ struct Ticket {
     int32_t* srcidOut = nullptr;
     Clock::time_point submit, done;
     int state = 0;           // 0 queued, 1 running, 2 done, 3 failed
+    long long id = -1;
+    svg_done_fn cb = nullptr;   // completion callback (the ticket is then never waited for)
+    void* user = nullptr;
 };
+
+// A callback due once the server mutex is released.
+struct DoneCall {
+    svg_done_fn cb;
+    void* user;
+    long long id;
+    int status;
+    double ms;
+};
+
+void run_calls(const std::vector<DoneCall>& calls)
+{
+    for (const DoneCall& c : calls) c.cb(c.id, c.status, c.ms, c.user);
+}
 
 // One slot group: a slot engine, its stream and worker, and the pinned
 // buffers its results leave through.
@@ -71,12 +89,16 @@ struct svg_server_s {
     svg_graph_s* g = nullptr;
     std::vector<std::unique_ptr<Group>> groups;
     int slots = 0;
+    size_t slotBytes = 0;
     int maxBatch = 0;
     size_t queueCap = 0;
     std::mutex mu;
     std::condition_variable cvWork, cvDone;
     std::deque<Ticket*> queue;
-    std::vector<std::unique_ptr<Ticket>> tickets;
+    // Live tickets by id.  A ticket is freed once its result is consumed:
+    // by svg_server_wait (or replay), or right after its callback ran.
+    std::unordered_map<long long, std::unique_ptr<Ticket>> tickets;
+    long long nextTicket = 0;
     bool stopping = false;
     std::string error;
     // stats
@@ -92,36 +114,55 @@ namespace {
 
 enum : int { SLOT_FREE = 0, SLOT_RUNNING = 1, SLOT_FETCHING = 2 };
 
+// Marks T failed (3) or done (2); a ticket with a callback is freed here and
+// its call queued for after the mutex is released.  Caller holds S->mu.
+void settle(svg_server_s* S, Ticket* T, int state, std::vector<DoneCall>& calls)
+{
+    T->state = state;
+    if (!T->cb) return;
+    const double ms = state == 2 ? std::chrono::duration<double, std::milli>(T->done - T->submit).count() : -1.0;
+    calls.push_back(DoneCall{T->cb, T->user, T->id, state == 2 ? SVG_OK : SVG_E_HIP, ms});
+    S->tickets.erase(T->id);
+}
+
 void fail_all(svg_server_s* S, const std::string& e)
 {
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (S->error.empty()) S->error = e;
-    for (auto& G : S->groups)
-        for (Ticket*& T : G->slotTicket)
-            if (T) { T->state = 3; T = nullptr; }
-    for (Ticket* T : S->queue) T->state = 3;
-    S->queue.clear();
-    S->cvDone.notify_all();
+    std::vector<DoneCall> calls;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->error.empty()) S->error = e;
+        for (auto& G : S->groups)
+            for (Ticket*& T : G->slotTicket)
+                if (T) { settle(S, T, 3, calls); T = nullptr; }
+        for (Ticket* T : S->queue) settle(S, T, 3, calls);
+        S->queue.clear();
+        S->cvDone.notify_all();
+    }
+    run_calls(calls);
 }
 
 void finish_ticket(svg_server_s* S, Group& G, int q, const SlotParam& sp)
 {
     solver_release_slot(G.st, q);
     const Clock::time_point now = Clock::now();
-    std::lock_guard<std::mutex> lk(S->mu);
-    Ticket* T = G.slotTicket[q];
-    if (!T) return;                       // failed meanwhile
-    const int kd = T->kind == Q_MSAD ? 1 : 0;
-    S->nKind[kd] += 1;
-    S->roundsKind[kd] += sp.rounds;
-    S->settledKind[kd] += sp.settled;
-    S->relaxedKind[kd] += sp.relaxed;
-    T->done = now;
-    T->state = 2;
-    S->lat.push_back(std::chrono::duration<double, std::milli>(now - T->submit).count());
-    S->lastDone = now;
-    G.slotTicket[q] = nullptr;
-    S->cvDone.notify_all();
+    std::vector<DoneCall> calls;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        Ticket* T = G.slotTicket[q];
+        if (!T) return;                       // failed meanwhile
+        const int kd = T->kind == Q_MSAD ? 1 : 0;
+        S->nKind[kd] += 1;
+        S->roundsKind[kd] += sp.rounds;
+        S->settledKind[kd] += sp.settled;
+        S->relaxedKind[kd] += sp.relaxed;
+        T->done = now;
+        S->lat.push_back(std::chrono::duration<double, std::milli>(now - T->submit).count());
+        S->lastDone = now;
+        G.slotTicket[q] = nullptr;
+        settle(S, T, 2, calls);
+        S->cvDone.notify_all();
+    }
+    run_calls(calls);
 }
 
 // A group takes a queued request only while no other group with a free slot
@@ -285,13 +326,14 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     int slots = cfg && cfg->max_inflight > 0 ? cfg->max_inflight : 64;
     const size_t perSlot = solver_bytes_per_slot(g->d.V);
     size_t freeB = 0, totB = 0;
-    hipMemGetInfo(&freeB, &totB);
+    if (hipMemGetInfo(&freeB, &totB) != hipSuccess) { delete S; return svg_set_error(SVG_E_HIP, "hipMemGetInfo failed"); }
     size_t budget = cfg && cfg->mem_budget > 0 ? (size_t)cfg->mem_budget : freeB / 4;
     const int memSlots = (int)std::min<size_t>(4096, budget / perSlot);
     if (memSlots < 1) { delete S; return svg_set_error(SVG_E_MEM, "memory budget below one request slot"); }
     slots = std::min(slots, memSlots);
     const int nGroups = std::max(1, std::min(cfg && cfg->streams > 0 ? cfg->streams : 4, std::min(slots, 8)));
     S->slots = slots;
+    S->slotBytes = perSlot;
     S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
     S->queueCap = cfg && cfg->queue_cap > 0 ? (size_t)cfg->queue_cap : (size_t)std::max(1024, 64 * slots);
     const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
@@ -333,7 +375,7 @@ extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_se
     return SVG_OK;
 }
 
-extern "C" long long svg_server_submit(svg_server_t S, const svg_request* r)
+static long long submit_impl(svg_server_t S, const svg_request* r, svg_done_fn cb, void* user)
 {
     if (!S || !r) return svg_set_error(SVG_E_ARG, "null argument");
     const int V = S->g->d.V;
@@ -351,27 +393,57 @@ extern "C" long long svg_server_submit(svg_server_t S, const svg_request* r)
     }
     T->distOut = r->dist_out;
     T->srcidOut = r->srcid_out;
+    T->cb = cb;
+    T->user = user;
     T->submit = Clock::now();
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->error.empty()) return svg_set_error(SVG_E_HIP, "server failed: " + S->error);
     if (S->queue.size() >= S->queueCap) { S->refused++; return svg_set_error(SVG_E_BUSY, "queue full"); }
     if (!S->anySubmit) { S->firstSubmit = T->submit; S->anySubmit = true; }
-    const long long id = (long long)S->tickets.size();
+    const long long id = S->nextTicket++;
+    T->id = id;
     S->queue.push_back(T.get());
-    S->tickets.push_back(std::move(T));
+    S->tickets.emplace(id, std::move(T));
     S->cvWork.notify_all();
     return id;
+}
+
+extern "C" long long svg_server_submit(svg_server_t S, const svg_request* r)
+{
+    return submit_impl(S, r, nullptr, nullptr);
+}
+
+extern "C" long long svg_server_submit_cb(svg_server_t S, const svg_request* r, svg_done_fn cb, void* user)
+{
+    if (!cb) return svg_set_error(SVG_E_ARG, "null callback");
+    return submit_impl(S, r, cb, user);
+}
+
+// Waits for a ticket without a callback and frees it; *done receives its
+// completion time.
+static int wait_ticket(svg_server_t S, long long ticket, Clock::time_point* submit, Clock::time_point* done)
+{
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end() || it->second->cb)
+        return svg_set_error(SVG_E_ARG, "unknown ticket (never issued, already waited for, or submitted with a callback)");
+    Ticket* T = it->second.get();
+    S->cvDone.wait(lk, [&] { return T->state >= 2; });
+    const int st = T->state;
+    if (submit) *submit = T->submit;
+    if (done) *done = T->done;
+    S->tickets.erase(ticket);
+    if (st == 3) return svg_set_error(SVG_E_HIP, "request failed: " + S->error);
+    return SVG_OK;
 }
 
 extern "C" int svg_server_wait(svg_server_t S, long long ticket, double* latency_ms)
 {
     if (!S) return svg_set_error(SVG_E_ARG, "null server");
-    std::unique_lock<std::mutex> lk(S->mu);
-    if (ticket < 0 || ticket >= (long long)S->tickets.size()) return svg_set_error(SVG_E_ARG, "unknown ticket");
-    Ticket& T = *S->tickets[ticket];
-    S->cvDone.wait(lk, [&] { return T.state >= 2; });
-    if (T.state == 3) return svg_set_error(SVG_E_HIP, "request failed: " + S->error);
-    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(T.done - T.submit).count();
+    Clock::time_point a, b;
+    const int rc = wait_ticket(S, ticket, &a, &b);
+    if (rc) return rc;
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(b - a).count();
     return SVG_OK;
 }
 
@@ -401,6 +473,8 @@ extern "C" int svg_server_stats_get(svg_server_t S, svg_server_stats* st)
     const double span = S->anySubmit ? std::chrono::duration<double>(S->lastDone - S->firstSubmit).count() : 0.0;
     st->qps = span > 0 ? (double)S->lat.size() / span : 0.0;
     st->max_batch_seen = S->maxBatchSeen;
+    st->slots = S->slots;
+    st->slot_bytes = (long long)S->slotBytes;
     st->launches = 0;
     for (const auto& G : S->groups) st->launches += G->launches.load(std::memory_order_relaxed);
     for (int k = 0; k < 2; ++k) {
@@ -442,12 +516,10 @@ extern "C" int svg_server_replay(svg_server_t S, const svg_request* reqs, const 
     }
     for (int i = 0; i < n; ++i) {
         if (ids[i] < 0) { if (latency_ms) latency_ms[i] = -1.0; continue; }
-        int rc = svg_server_wait(S, ids[i], nullptr);
+        Clock::time_point done;
+        int rc = wait_ticket(S, ids[i], nullptr, &done);
         if (rc) return rc;
-        if (latency_ms) {
-            std::lock_guard<std::mutex> lk(S->mu);
-            latency_ms[i] = std::chrono::duration<double, std::milli>(S->tickets[ids[i]]->done - sched[i]).count();
-        }
+        if (latency_ms) latency_ms[i] = std::chrono::duration<double, std::milli>(done - sched[i]).count();
     }
     return SVG_OK;
 }

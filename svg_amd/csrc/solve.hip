@@ -1455,6 +1455,52 @@ extern "C" int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t* faces,
     return SVG_OK;
 }
 
+extern "C" int svg_msad_batch(svg_graph_t g, const int32_t* src, const double* w, const int64_t* offsets, int nq,
+                              double* dist, int32_t* src_id)
+{
+    if (!g || !offsets || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
+    const size_t V = (size_t)g->d.V;
+    for (int i = 0; i < nq; ++i) {
+        if (offsets[i + 1] <= offsets[i]) return svg_set_error(SVG_E_ARG, "every query needs a source");
+        if (offsets[i] < 0) return svg_set_error(SVG_E_ARG, "negative offset");
+    }
+    if (nq > 0 && (!src || !w)) return svg_set_error(SVG_E_ARG, "null sources or weights");
+    std::lock_guard<std::mutex> lk(g->mu);
+    const int S = std::min(32, std::max(1, nq));
+    SolverState* st = nullptr;
+    int rc = get_solver(g, S, &st);
+    if (rc) return rc;
+    std::vector<int> slotQuery((size_t)st->S, -1);
+    int nextQ = 0, done = 0;
+    while (done < nq) {
+        std::vector<Admission> adm;
+        for (int q = 0; q < st->S && nextQ < nq; ++q) {
+            if (slotQuery[q] != -1) continue;
+            Admission a;
+            a.slot = q; a.kind = Q_MSAD; a.target = -1; a.wantParent = src_id != nullptr;
+            a.src.assign(src + offsets[nextQ], src + offsets[nextQ + 1]);
+            a.w.assign(w + offsets[nextQ], w + offsets[nextQ + 1]);
+            adm.push_back(std::move(a));
+            slotQuery[q] = nextQ++;
+        }
+        std::string ea = solver_admit_many(*st, g->mesh->d, adm);
+        if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
+        int running = 0;
+        std::string e = solver_round(*st, g->d, g->mesh->d, running);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        for (int q = 0; q < st->S; ++q) {
+            if (slotQuery[q] == -1 || !st->spHost[q].done) continue;
+            const size_t o = (size_t)slotQuery[q] * V;
+            e = solver_fetch(*st, q, dist + o, nullptr, src_id ? src_id + o : nullptr, nullptr);
+            if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+            solver_release_slot(*st, q);
+            slotQuery[q] = -1;
+            ++done;
+        }
+    }
+    return SVG_OK;
+}
+
 extern "C" int svg_sssd_batch(svg_graph_t g, const int32_t* stp, int nq, double* out)
 {
     if (!g || !stp || !out || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
