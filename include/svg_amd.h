@@ -129,6 +129,17 @@ int svg_sssd(svg_graph_t g, int s, int t, double *dist_t, int32_t *path, int pat
  * weights; dist[V], src_id[V], parent[V] (src_id / parent may be NULL). */
 int svg_msad(svg_graph_t g, const int32_t *src, const double *w, int ns, double *dist,
              int32_t *src_id, int32_t *parent);
+/* svg_msad without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and answers as svg_msad; its working memory is a fixed 64 MiB
+ * device array that the library holds on each GPU it runs on until the
+ * process ends, which limits it to graphs of about 87,000 vertices (larger
+ * ones return SVG_E_ARG).  It calls no hipMalloc / hipFree and no pinned
+ * allocation.  Calls share the array: the library serialises them within a
+ * process, so calls overlapping on one GPU wait for each other. */
+int svg_msad_noalloc(svg_graph_t g, const int32_t *src, const double *w, int ns, double *dist,
+                     int32_t *src_id, int32_t *parent);
 /* SVG::SolveSSSD(f0, p0, f1, p1) (SVG.cu:253-352): geodesic distance
  * between point p0[3] on face f0 and point p1[3] on face f1 (coordinates in
  * the mesh's frame).  Bounded ICH runs from both points (K of the graph,

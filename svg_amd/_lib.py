@@ -73,7 +73,7 @@ EXPORTS = [
     "svg_mesh_upload", "svg_mesh_info", "svg_mesh_export", "svg_mesh_free",
     "svg_build_params_default", "svg_build", "svg_build_stats_get", "svg_build_status",
     "svg_graph_info", "svg_graph_to_host", "svg_graph_save", "svg_graph_load", "svg_graph_free",
-    "svg_sssd", "svg_msad", "svg_sssd_point", "svg_sssd_point_batch", "svg_sssd_batch", "svg_msad_batch",
+    "svg_sssd", "svg_msad", "svg_msad_noalloc", "svg_sssd_point", "svg_sssd_point_batch", "svg_sssd_batch", "svg_msad_batch",
     "svg_server_start", "svg_server_submit", "svg_server_submit_cb",
     "svg_server_wait", "svg_server_stats_get", "svg_server_stop", "svg_server_replay",
 ]
@@ -120,6 +120,7 @@ def lib():
     L.svg_sssd.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double), _vp, C.c_int,
                            C.POINTER(C.c_int)]
     L.svg_msad.argtypes = [_vp, _ip, _dp, C.c_int, _dp, _vp, _vp]
+    L.svg_msad_noalloc.argtypes = [_vp, _ip, _dp, C.c_int, _dp, _vp, _vp]
     L.svg_sssd_batch.argtypes = [_vp, _ip, C.c_int, _dp]
     L.svg_msad_batch.argtypes = [_vp, _ip, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                  C.c_int, _dp, _vp]

@@ -16,6 +16,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "point.h"
@@ -802,19 +804,21 @@ __global__ void k_mean_arc(const int32_t* tail, const double* w, int V, int K, d
 
 double graph_mean_arc(const DevGraph& g)
 {
+    // the bucket width only schedules work (answers do not depend on it):
+    // a failed step falls back to width 1
     double* ds = nullptr;
     unsigned long long* dc = nullptr;
-    hipMalloc(&ds, sizeof(double));
-    hipMalloc(&dc, sizeof(unsigned long long));
-    hipMemset(ds, 0, sizeof(double));
-    hipMemset(dc, 0, sizeof(unsigned long long));
-    hipLaunchKernelGGL(k_mean_arc, dim3(256), dim3(256), 0, 0, g.tail, g.w, g.V, g.K, ds, dc);
     double s = 0.0;
     unsigned long long c = 0;
-    hipMemcpy(&s, ds, sizeof(double), hipMemcpyDeviceToHost);
-    hipMemcpy(&c, dc, sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    bool ok = hipMalloc(&ds, sizeof(double)) == hipSuccess && hipMalloc(&dc, sizeof(unsigned long long)) == hipSuccess &&
+              hipMemset(ds, 0, sizeof(double)) == hipSuccess && hipMemset(dc, 0, sizeof(unsigned long long)) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_mean_arc, dim3(256), dim3(256), 0, 0, g.tail, g.w, g.V, g.K, ds, dc);
+        ok = hipMemcpy(&s, ds, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(&c, dc, sizeof(unsigned long long), hipMemcpyDeviceToHost) == hipSuccess;
+    }
     hipFree(ds); hipFree(dc);
-    return c ? s / (double)c : 1.0;
+    return ok && c ? s / (double)c : 1.0;
 }
 
 static SolveArgs make_args(SolverState& s, const DevGraph& g, const DevMesh& m)
@@ -1298,6 +1302,202 @@ extern "C" int svg_msad(svg_graph_t g, const int32_t* src, const double* w, int 
     if (rc) return rc;
     e = solver_fetch(*st, 0, dist, nullptr, src_id, parent);
     solver_release_slot(*st, 0);
+    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+    return SVG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MSAD without device allocation (svg_msad_noalloc).  Every device buffer of
+// one single-slot solver -- the per-vertex state, the frontier lists, the
+// round counters, the slot record, the admission staging and the mean-arc
+// accumulators -- is carved out of `scratch`, a file-scope device array of
+// kNoAllocBytes that each GPU holds for the life of the process.  The same
+// kernels run with the same list capacities as a solver made by
+// solver_create for one slot, so the answers are those of svg_msad.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Device address of `scratch` on each device, looked up on the first call
+// made with that device current (the host-side name is not a device
+// address).
+void* g_scratchDev[kNoAllocDevices] = {nullptr};
+std::mutex g_scratchMu;            // calls are serialised: they share the array
+
+struct NoAllocLayout {
+    size_t off[24];
+    size_t total;
+};
+
+// Offsets (256-B aligned) of every buffer for V vertices and ns seeds; the
+// order matches the fields assigned in noalloc_carve.
+NoAllocLayout noalloc_layout(int V, int ns)
+{
+    NoAllocLayout L;
+    const size_t sv = (size_t)V;
+    const size_t frontCap = 2 * sv + 65536, pCap = sv + 65536;
+    const size_t wonCap = std::min<size_t>(sv / 4 + 65536, 0x7ffffff0ull);
+    const size_t intBytes = (((size_t)7 + (size_t)ns + 2) * sizeof(int) + 15) / 16 * 16;
+    const size_t sizes[] = {
+        sizeof(unsigned long long) * sv, sizeof(int32_t) * sv, sizeof(uint32_t) * sv, sizeof(uint32_t) * sv,
+        sizeof(int32_t) * sv, sizeof(float) * sv,                                   // 0-5 per-vertex state
+        sizeof(uint64_t) * frontCap, sizeof(uint64_t) * frontCap,                   // 6-7 deferred lists
+        sizeof(uint64_t) * pCap * kPLists, sizeof(uint64_t) * pCap * kPLists,       // 8-9 push lists
+        sizeof(ExpEnt) * frontCap, sizeof(ExpEnt) * frontCap,                       // 10-11 rows
+        sizeof(SolveCounts),                                                        // 12
+        sizeof(WonEnt) * wonCap * kPLists, sizeof(WonEnt) * wonCap * kPLists,       // 13-14 won lists
+        sizeof(SlotAcc) * kAccCopies, sizeof(SlotAcc) * kAccCopies,                 // 15-16 statistics
+        sizeof(double) * 2, sizeof(int) * 2,                                        // 17-18 thr, doneP
+        sizeof(SlotParam) * 2,                                                      // 19 slot record + flag
+        intBytes + sizeof(double) * (size_t)ns,                                     // 20 admission staging
+        sizeof(double) + sizeof(unsigned long long),                                // 21 mean-arc sum, count
+    };
+    size_t o = 0;
+    for (int i = 0; i < 22; ++i) {
+        L.off[i] = o;
+        o += (sizes[i] + 255) / 256 * 256;
+    }
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+// The state a fresh single-slot solver starts from (solver_create's
+// initial values), written by a kernel: the array keeps the last call's
+// contents, and fills are kernels, not memsets.
+__global__ void k_noalloc_reset(SolveCounts* cnt, SlotAcc* acc0, SlotAcc* acc1, double* thr, int* doneP,
+                                SlotParam* sp, double* msum, unsigned long long* mcnt)
+{
+    int* c = reinterpret_cast<int*>(cnt);
+    const int nc = (int)(sizeof(SolveCounts) / sizeof(int));
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) c[i] = 0;
+    for (int i = threadIdx.x; i < kAccCopies; i += blockDim.x) {
+        SlotAcc x;
+        x.minDeferred = ~0ull; x.minImp = ~0ull; x.settled = 0; x.relaxed = 0; x.expanded = 0; x.pad = 0;
+        acc0[i] = x;
+        acc1[i] = x;
+    }
+    if (threadIdx.x < 2) { thr[threadIdx.x] = 0.0; doneP[threadIdx.x] = 1; }
+    if (threadIdx.x < 2) {
+        SlotParam p;
+        p.kind = threadIdx.x == 0 ? -1 : 0;        // the second record is the overflow flag
+        p.target = 0; p.wantParent = 0; p.done = 1; p.thr = 0.0; p.delta = 0.0; p.minDeferred = ~0ull;
+        p.expanded = 0; p.rounds = 0; p.gen = 0; p.pad = 0; p.relaxed = 0; p.settled = 0; p.result = 0.0;
+        sp[threadIdx.x] = p;
+    }
+    if (threadIdx.x == 0) { *msum = 0.0; *mcnt = 0ull; }
+}
+
+extern "C" int svg_msad_noalloc(svg_graph_t g, const int32_t* src, const double* w, int ns, double* dist,
+                                int32_t* src_id, int32_t* parent)
+{
+    if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
+    const int V = g->d.V;
+    for (int i = 0; i < ns; ++i)
+        if (src[i] < 0 || src[i] >= V) return svg_set_error(SVG_E_ARG, "source out of range");
+    if (g->d.K > 0xFFFF) return svg_set_error(SVG_E_ARG, "K above 65535");
+    const NoAllocLayout L = noalloc_layout(V, ns);
+    if (L.total > kNoAllocBytes)
+        return svg_set_error(SVG_E_ARG, "graph too large for svg_msad_noalloc: it needs " + std::to_string(L.total) +
+                                            " bytes of its " + std::to_string(kNoAllocBytes) +
+                                            "-byte device array (about 87,000 vertices at most)");
+    int rc = svg_ensure_device();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices)
+        return svg_set_error(SVG_E_HIP, "no current device");
+    if (!g_scratchDev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return svg_set_error(SVG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchDev[dev] = p;
+    }
+    char* base = static_cast<char*>(g_scratchDev[dev]);
+    SolverState s;
+    s.V = V; s.K = g->d.K; s.S = 1; s.stream = 0;
+    s.dist = (unsigned long long*)(base + L.off[0]);
+    s.parent = (int32_t*)(base + L.off[1]);
+    s.stamp = (uint32_t*)(base + L.off[2]);
+    s.inq = (uint32_t*)(base + L.off[3]);
+    s.srcid = (int32_t*)(base + L.off[4]);
+    s.hdist = (float*)(base + L.off[5]);
+    s.dl[0] = (uint64_t*)(base + L.off[6]); s.dl[1] = (uint64_t*)(base + L.off[7]);
+    s.pl[0] = (uint64_t*)(base + L.off[8]); s.pl[1] = (uint64_t*)(base + L.off[9]);
+    s.exp[0] = (ExpEnt*)(base + L.off[10]); s.exp[1] = (ExpEnt*)(base + L.off[11]);
+    s.cnt = (SolveCounts*)(base + L.off[12]);
+    s.won[0] = (WonEnt*)(base + L.off[13]); s.won[1] = (WonEnt*)(base + L.off[14]);
+    s.acc[0] = (SlotAcc*)(base + L.off[15]); s.acc[1] = (SlotAcc*)(base + L.off[16]);
+    s.thr = (double*)(base + L.off[17]);
+    s.doneP = (int*)(base + L.off[18]);
+    s.sp = (SlotParam*)(base + L.off[19]);
+    s.ovDev = &s.sp[1].kind;
+    s.frontCap = 2 * (size_t)V + 65536;
+    s.pCap = (size_t)V + 65536;
+    s.wonCap = std::min<size_t>((size_t)V / 4 + 65536, 0x7ffffff0ull);
+    int* stage = (int*)(base + L.off[20]);
+    const size_t intBytes = (((size_t)7 + (size_t)ns + 2) * sizeof(int) + 15) / 16 * 16;
+    double* stageW = (double*)((char*)stage + intBytes);
+    double* msum = (double*)(base + L.off[21]);
+    unsigned long long* mcnt = (unsigned long long*)(msum + 1);
+    // host copy of the slot record and overflow flag (no pinned memory)
+    SlotParam spHost[2];
+    std::memset(spHost, 0, sizeof(spHost));
+    spHost[0].kind = -1; spHost[0].done = 1; spHost[1].kind = 0;
+    s.spHost = spHost;
+
+    hipLaunchKernelGGL(k_noalloc_reset, dim3(1), dim3(256), 0, 0, s.cnt, s.acc[0], s.acc[1], s.thr, s.doneP, s.sp,
+                       msum, mcnt);
+    // bucket width: the graph's mean arc length, as graph_mean_arc computes it
+    hipLaunchKernelGGL(k_mean_arc, dim3(256), dim3(256), 0, 0, g->d.tail, g->d.w, V, g->d.K, msum, mcnt);
+    double msumH = 0.0;
+    unsigned long long mcntH = 0;
+    hipError_t he = hipMemcpy(&msumH, msum, sizeof(double), hipMemcpyDeviceToHost);
+    if (he == hipSuccess) he = hipMemcpy(&mcntH, mcnt, sizeof(mcntH), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc set-up: ") + hipGetErrorString(he));
+    s.delta = s.deltaMsad = mcntH ? msumH / (double)mcntH : 1.0;
+    // admission staging, laid out as solver_admit_many lays it out for one
+    // slot: slot, kind, target, wantParent, gen, off[2], seeds, xoff[2]; weights
+    const int wantParent = (src_id || parent) ? 1 : 0;
+    const int head[7] = {0, Q_MSAD, -1, wantParent, 1, 0, ns};
+    const int tailOff[2] = {0, 0};
+    he = hipMemcpy(stage, head, sizeof(head), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(stage + 7, src, sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(stage + 7 + ns, tailOff, sizeof(tailOff), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(stageW, w, sizeof(double) * (size_t)ns, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc seeds: ") + hipGetErrorString(he));
+    const DevMesh& m = g->mesh->d;
+    const int* dSlots = stage;
+    const int* dKinds = stage + 1;
+    const int* dTargets = stage + 2;
+    const int* dWant = stage + 3;
+    const int* dGens = stage + 4;
+    const int* dOff = stage + 5;
+    const int* dSeeds = stage + 7;
+    const int* dXoff = stage + 7 + ns;
+    hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, ((size_t)V + 255) / 256)), dim3(256), 0, 0,
+                       s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, 1, V);
+    {
+        DevGraph none;
+        SolveArgs args = make_args(s, none, m);
+        hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
+                           dSeeds, stageW, s.delta, s.deltaMsad, dXoff, dXoff, stageW, s.hdist);
+    }
+    spHost[0].kind = Q_MSAD; spHost[0].target = -1; spHost[0].done = 0; spHost[0].wantParent = wantParent;
+    spHost[0].gen = 1;
+    for (;;) {
+        int running = 0;
+        std::string e = solver_round(s, g->d, m, running);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        if (running == 0) break;
+    }
+    std::string e = solver_fetch(s, 0, dist, nullptr, src_id, parent);
     if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
     return SVG_OK;
 }
