@@ -82,10 +82,16 @@ def make_requests(V, n, qps, seed=21, msad_frac=0.3):
 
 
 def serve_replay(g, V, n, qps, inflight, delta_scale=1.0, rounds_per_sync=4, msad_delta_scale=0.0, streams=0,
-                 chain_rows=0, queue_cap=0):
+                 chain_rows=0, queue_cap=0, given=None):
+    """Replays n requests (or `given` = (arrival, reqs)) through the server;
+    queue_cap > 0 submits them all at once (throughput)."""
     import ctypes as C
     from svg_amd import _lib as L
-    arrival, reqs = make_requests(V, n, qps)
+    arrival, reqs = given if given is not None else make_requests(V, n, qps)
+    n = len(reqs)
+    if queue_cap:
+        arrival = np.zeros(n)
+        queue_cap = max(queue_cap, n)
     lib = L.lib()
     cfg = L.ServerCfg(inflight, 0, inflight, delta_scale, rounds_per_sync, msad_delta_scale, streams, chain_rows,
                       queue_cap)
@@ -182,6 +188,103 @@ def point_queries(g, xyz, tri, n, n_single=16):
             "workload": "half near (face index within 40), half uniform; Dirichlet(2,2,2) points"}, faces, pts
 
 
+def copy_bandwidth(mb=1024, reps=5):
+    """Device-to-device copy of mb MiB timed with HIP events (bytes read +
+    written per second): the on-box bandwidth beside the 8 TB/s spec."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    best = None
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None) == 0
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        best = ms.value if best is None else min(best, ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * n / (best * 1e-3) / 1e9
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Algorithmic bytes of one query (model of the reference's solver,
+# SVG.cu:369-413): per settled vertex its distance and row pointer (16 B),
+# per scanned arc the target id, weight and target distance (20 B), per
+# improvement distance + parent + source id (16 B); A* also reads the
+# target-side position for the heuristic per improvement (24 B).
+def query_bytes(kind, st):
+    b = 16 * st[0] + 20 * st[1] + 16 * st[2]
+    return b + (24 * st[2] if kind == 0 else 0)
+
+
+def cpu_query_baselines(og, reqs, cores, single_s=6.0, all_s=10.0):
+    """The oracle's A* / Dijkstra (the reference's host solvers) on the
+    serving requests: one thread per query over a prefix of each kind, then
+    all cores over a prefix of the mixed list without pacing.  Also returns
+    the prefix used for all cores with its algorithmic bytes."""
+    import concurrent.futures as cf
+    out = {}
+    for kind, name in ((0, "sssd"), (1, "msad")):
+        t0 = time.perf_counter()
+        n = 0
+        for k, s, t, src in reqs:
+            if k != kind:
+                continue
+            if kind == 0:
+                og.sssd(s, t)
+            else:
+                og.msad(src, np.zeros(len(src)))
+            n += 1
+            if time.perf_counter() - t0 > single_s and n >= 4:
+                break
+        dt = time.perf_counter() - t0
+        out[name] = {"value": n / dt, "unit": "queries/s", "ms_per_query": dt / n * 1e3, "cores": 1,
+                     "kind": "port", "sample": f"oracle on the first {n} {name.upper()} requests ({dt:.1f} s, one thread)"}
+
+    def one(r):
+        k, s, t, src = r
+        if k == 0:
+            d, _, _, _, st = og.sssd(s, t)
+        else:
+            _, _, _, st = og.msad(src, np.zeros(len(src)))
+        return query_bytes(k, st)
+
+    # all cores: grow the prefix until the run takes about all_s seconds
+    n = min(len(reqs), max(cores * 4, 64))
+    while True:
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(cores) as ex:
+            qb = list(ex.map(one, reqs[:n]))
+        dt = time.perf_counter() - t0
+        if dt >= all_s or n >= len(reqs):
+            break
+        n = min(len(reqs), int(n * max(1.5, all_s / max(dt, 1e-3))))
+    out["all_cores"] = {"value": n / dt, "unit": "queries/s", "cores": cores, "kind": "port",
+                        "sample": f"oracle on the first {n} requests of the mix without pacing ({dt:.1f} s, "
+                                  f"{cores} threads)"}
+    return out, n, float(np.sum(qb))
+
+
 def calib_copy(mb):
     """Device-to-device copy of a known size, to scale hardware byte counters."""
     import ctypes as C
@@ -207,7 +310,7 @@ def main():
     ap.add_argument("--workload", default="bumpy_torus")
     ap.add_argument("--K", type=int, default=0)
     ap.add_argument("--plain", default="sweep", help="sweep | ref | none")
-    ap.add_argument("--requests", type=int, default=20000)
+    ap.add_argument("--requests", type=int, default=100000)
     ap.add_argument("--qps", type=float, default=1000.0)
     ap.add_argument("--inflight", type=int, default=64)
     ap.add_argument("--delta-scale", default="1.0",
@@ -259,9 +362,13 @@ def main():
 
     # the autotuner learns from the warm-up build(s) of this mesh: source
     # order, table sizes, block shape (rows do not depend on it)
+    first_build = None
     for i in range(args.warmup):
         g = build(S.SVG_MODE_BATCHED, autotune=1)
-        log(f"warmup {i}: build {g.stats()['seconds']:.4f} s")
+        st0 = g.stats()
+        if i == 0:
+            first_build = {"seconds": st0["seconds"], "tuned": st0["tuned"], "prop_block": st0["prop_block"]}
+        log(f"warmup {i}: build {st0['seconds']:.4f} s (tuned {st0['tuned']})")
         g.FreeSVGStructure()
     times, stats = [], []
     wall0 = time.perf_counter()
@@ -290,6 +397,9 @@ def main():
         "config": {"workload": args.workload, "V": V, "F": len(tri), "K": K,
                    "mesh_sha256": digest, "mode": "batched", "parallelism": "single GPU"},
         "build_s": build_s, "build_s_all": times, "wall_s_steps": wall,
+        "wall_ms_per_step": wall / args.steps * 1e3,
+        "wall_includes": "per-build hipMalloc / hipFree of the tables, their resets, the stats copies",
+        "first_build_untuned": first_build,
         "batched": {k: st[k] for k in ("prop_seconds", "extract_seconds", "redo_seconds", "redone",
                                        "flagged_drop", "flagged_overflow", "flagged_error", "lanes",
                                        "prop_block", "chunk", "vert_cap", "split_cap", "tuned", "arcs")},
@@ -302,6 +412,14 @@ def main():
                      "extract_achieved_gbs": eb / st["extract_seconds"] / 1e9},
     }
     g_keep = g
+    try:
+        cbw = copy_bandwidth()
+        res["roofline"]["peak_measured"] = cbw
+        res["roofline"]["frac_of_measured"] = prop_gbs / cbw
+        res["roofline"]["peak_measured_source"] = "1 GiB device-to-device hipMemcpy, read + written bytes / event time"
+        log(f"copy bandwidth {cbw:.0f} GB/s")
+    except Exception as e:       # the bench line still prints without it
+        log(f"copy bandwidth not measured: {e}")
 
     # ---- plain form
     if args.plain != "none":
@@ -326,9 +444,10 @@ def main():
                 for rps in [int(x) for x in args.rounds_per_sync.split(",")]:
                     for nst in [int(x) for x in args.streams.split(",")]:
                         for ch in [int(x) for x in args.chain_rows.split(",")]:
-                            sv, _ = serve_replay(g_keep, V, args.requests, 1e9 if args.batch else args.qps,
-                                                 args.inflight, ds, rps, mds, nst, ch,
-                                                 args.requests if args.batch else 0)
+                            sv, rq = serve_replay(g_keep, V, args.requests, 1e9 if args.batch else args.qps,
+                                                  args.inflight, ds, rps, mds, nst, ch,
+                                                  args.requests if args.batch else 0)
+                            serve_reqs = rq
                             runs.append(sv)
                             log(f"serving (delta x{ds}, msad delta x{mds or ds}, {rps} rounds/sync, {nst} streams, "
                                 f"chain {ch}): p50 {sv['p50_ms']:.3f} ms p99 {sv['p99_ms']:.3f} ms "
@@ -362,11 +481,21 @@ def main():
         n = int(min(V, max(n_probe, args.cpu_seconds / max(per, 1e-6))))
         _, _, _, secs = O.svg_build(om, K, s0=0, s1=n, nthreads=cores, counters=False)
         res["cpu_baseline"] = {"value": n / secs, "unit": "sources/s", "cores": cores,
-                               "kind": "port",
+                               "kind": "port", "cpu": cpu_model(),
                                "sample": f"oracle build of sources [0,{n}) of {V} "
                                          f"({secs:.1f} s on {cores} threads)"}
-        if pq_faces is not None:
-            # the oracle's point queries on the same graph, one core, ~3 s
+        # one core, sources spread over the mesh (about a quarter of the time)
+        n1 = max(8, int(args.cpu_seconds / 4 / max(per * cores, 1e-6)))
+        t1 = 0.0
+        for s0 in np.linspace(0, V - 1, n1).astype(int):
+            _, _, _, d1 = O.svg_build(om, K, s0=int(s0), s1=int(s0) + 1, nthreads=1, counters=False)
+            t1 += d1
+        res["cpu_baseline_1core"] = {"value": n1 / t1, "unit": "sources/s", "cores": 1, "kind": "port",
+                                     "sample": f"oracle build of {n1} sources spread over the mesh "
+                                               f"({t1:.1f} s, one thread)"}
+        log(f"cpu build: {n / secs:.0f} sources/s on {cores} threads, {n1 / t1:.0f} on one")
+        og = None
+        if pq_faces is not None or args.requests > 0:
             Vg, Kg, _ = g_keep.info()
             tails = np.empty(Vg, np.int32)
             adj = np.empty(Vg * Kg, np.int32)
@@ -375,6 +504,28 @@ def main():
                                               None, None, None, None))
             og = O.Graph.from_arrays(xyz, adj, w, tails, Kg)
             del adj, w
+        if args.requests > 0:
+            # the reference's host solvers on the serving requests, then the
+            # same prefix on the GPU all at once: query throughput and the
+            # query path's algorithmic bytes per second
+            arrival, reqs, _ = serve_reqs
+            qb, nq, qbytes = cpu_query_baselines(og, reqs, cores)
+            res["query_cpu_baseline"] = qb
+            sv_b, _ = serve_replay(g_keep, V, nq, 1e9, args.inflight, 1.0, 4, 1.0, 4, 4, nq,
+                                   given=(arrival[:nq], reqs[:nq]))
+            gpu_s = nq / sv_b["achieved_qps"]
+            res["query_roofline"] = {
+                "bound": "hbm", "achieved": qbytes / gpu_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": qbytes / gpu_s / 1e9 / HBM_PEAK_GBS, "queries": nq, "seconds": gpu_s,
+                "algorithmic_bytes": qbytes, "gpu_queries_per_s": sv_b["achieved_qps"],
+                "vs_cpu_all_cores": sv_b["achieved_qps"] / qb["all_cores"]["value"],
+                "bytes_model": "16 B per settled vertex + 20 B per scanned arc + 16 B per improvement "
+                               "(+24 B per improvement for A*), counts of the oracle's heap solvers",
+                "time": "wall time of the same requests submitted at once to the server"}
+            log(f"query cpu baseline {qb}; gpu batch of {nq}: {sv_b['achieved_qps']:.0f} queries/s, "
+                f"{qbytes / gpu_s / 1e9:.1f} GB/s algorithmic")
+        if pq_faces is not None:
+            # the oracle's point queries on the same graph, one core, ~3 s
             t0 = time.perf_counter()
             m_done = 0
             while m_done < len(pq_faces) and (time.perf_counter() - t0 < 3.0 or m_done < 4):
