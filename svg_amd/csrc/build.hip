@@ -275,8 +275,8 @@ std::string build_plain(const HostMesh& h, const DevMesh& d, const BuildConfig& 
 // ---------------------------------------------------------------------------
 // batched form
 // ---------------------------------------------------------------------------
-struct BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 B
-struct BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B
+struct alignas(16) BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 B, one load
+struct alignas(16) BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B, two loads
 struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, twin; double pad; };  // 64 B
 static_assert(sizeof(BWinRec) == 64, "window records fill one 64-B half-line");
 struct alignas(8) BPsEnt { double key; int32_t vert, pseudoSrc, lvlb, pad; };   // 24 B
@@ -415,10 +415,18 @@ struct BatchedPolicy {
     }
 
     // --- vertex table
-    __device__ VInfo vresolve(BSlotV s, unsigned h, int v) const
+    // The slot a lookup of face vertex `op` ended on (its own, or the empty
+    // slot where it would be inserted), kept until the table next changes:
+    // propagate writes that vertex right after the lookup without probing
+    // again.  Only vput changes the table, and every vput clears it.
+    int cacheV = -1; unsigned cacheH = 0; bool cacheIns = false;
+    int cacheE = -1; unsigned cacheSH = 0; bool cacheSIns = false;
+    __device__ VInfo vresolve(BSlotV s, unsigned& h, int v, bool& found) const
     {
+        found = false;
         for (;;) {
             if (s.key == v) {
+                found = true;
                 VInfo r;
                 const uint32_t e24 = s.eb & kEnterNone;
                 r.enter = e24 == kEnterNone ? -1 : (int)e24;
@@ -434,8 +442,9 @@ struct BatchedPolicy {
     }
     __device__ VInfo vget(int v) const
     {
-        const unsigned h = vslot((unsigned)v) & vmask;
-        return vresolve(vt[h], h, v);
+        unsigned h = vslot((unsigned)v) & vmask;
+        bool f;
+        return vresolve(vt[h], h, v, f);
     }
     // The source keeps birth time 0 from Initialize on: no relaxation can
     // improve a distance of 0.
@@ -454,70 +463,101 @@ struct BatchedPolicy {
         bcV0 = v; bcB0 = b;
         return b;
     }
-    __device__ SInfo sresolve(unsigned h, int e) const
+    // A probe reads the whole 32-B slot at once (key and record together).
+    __device__ SInfo sresolve(unsigned& h, int e, bool& found) const
     {
+        found = false;
         for (;;) {
-            const int32_t k = st[h].key;
-            if (k == e) {
-                SInfo r; r.dist = st[h].dist; r.x = st[h].x; r.ps = st[h].ps;
+            const BSlotS x = st[h];
+            if (x.key == e) {
+                found = true;
+                SInfo r; r.dist = x.dist; r.x = x.x; r.ps = x.ps;
                 return r;
             }
-            if (k == -1) return sinfo_default();
+            if (x.key == -1) return sinfo_default();
             h = (h + 1) & smask;
         }
     }
-    __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp) const
+    __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp)
     {
-        const unsigned ha = vslot((unsigned)t.a) & vmask;
-        const unsigned hb = vslot((unsigned)t.b) & vmask;
-        const unsigned ho = vslot((unsigned)t.op) & vmask;
+        unsigned ha = vslot((unsigned)t.a) & vmask;
+        unsigned hb = vslot((unsigned)t.b) & vmask;
+        unsigned ho = vslot((unsigned)t.op) & vmask;
+        unsigned hs = hmix((unsigned)t.e0) & smask;
         const BSlotV sa = vt[ha], sb = vt[hb], so = vt[ho];
-        sp = sresolve(hmix((unsigned)t.e0) & smask, t.e0);
-        va = vresolve(sa, ha, t.a);
-        vb = vresolve(sb, hb, t.b);
-        vo = vresolve(so, ho, t.op);
+        bool fa, fb, fo, fs;
+        sp = sresolve(hs, t.e0, fs);
+        va = vresolve(sa, ha, t.a, fa);
+        vb = vresolve(sb, hb, t.b, fb);
+        vo = vresolve(so, ho, t.op, fo);
+        cacheV = t.op; cacheH = ho; cacheIns = !fo;
+        cacheE = t.e0; cacheSH = hs; cacheSIns = !fs;
     }
     __device__ void vput(int v, const VInfo& x)
     {
         if (dead) return;
-        unsigned h = vslot((unsigned)v) & vmask;
-        for (;;) {
-            const int32_t k = vt[h].key;
-            if (k == v) break;
-            if (k == -1) {
-                if (vcount >= vlimit) { flags |= SF_CAPACITY; dead = true; return; }
-                vt[h].key = v;
-                rl[vcount++] = v;
-                if (vcount > vrefcap) flags |= SF_REF_OVERFLOW;
-                break;
+        unsigned h;
+        bool ins;
+        if (v == cacheV) {
+            h = cacheH;
+            ins = cacheIns;
+        } else {
+            h = vslot((unsigned)v) & vmask;
+            for (;;) {
+                const int32_t k = vt[h].key;
+                if (k == v) { ins = false; break; }
+                if (k == -1) { ins = true; break; }
+                h = (h + 1) & vmask;
             }
-            h = (h + 1) & vmask;
         }
-        vt[h].eb = ((uint32_t)x.enter & kEnterNone) | ((uint32_t)(uint8_t)x.birth << 24);
-        vt[h].dist = x.dist;
+        cacheV = -1;
+        if (ins) {
+            if (vcount >= vlimit) { flags |= SF_CAPACITY; dead = true; return; }
+            rl[vcount++] = v;
+            if (vcount > vrefcap) flags |= SF_REF_OVERFLOW;
+        }
+        BSlotV ns;
+        ns.key = v;
+        ns.eb = ((uint32_t)x.enter & kEnterNone) | ((uint32_t)(uint8_t)x.birth << 24);
+        ns.dist = x.dist;
+        vt[h] = ns;
         if (v == bcV0) bcB0 = (int)x.birth;
         if (v == bcV1) bcB1 = (int)x.birth;
     }
     __device__ void vtouch(int) {}
     // --- split table
-    __device__ SInfo sget(int e) const { return sresolve(hmix((unsigned)e) & smask, e); }
+    __device__ SInfo sget(int e) const
+    {
+        unsigned h = hmix((unsigned)e) & smask;
+        bool f;
+        return sresolve(h, e, f);
+    }
     __device__ void sput(int e, const SInfo& x)
     {
         if (dead) return;
-        unsigned h = hmix((unsigned)e) & smask;
-        for (;;) {
-            const int32_t k = st[h].key;
-            if (k == e) break;
-            if (k == -1) {
-                if (scount >= slimit) { flags |= SF_CAPACITY; dead = true; return; }
-                st[h].key = e;
-                ++scount;
-                if (scount > srefcap) flags |= SF_REF_OVERFLOW;
-                break;
+        unsigned h;
+        bool ins;
+        if (e == cacheE) {
+            h = cacheSH;
+            ins = cacheSIns;
+        } else {
+            h = hmix((unsigned)e) & smask;
+            for (;;) {
+                const int32_t k = st[h].key;
+                if (k == e) { ins = false; break; }
+                if (k == -1) { ins = true; break; }
+                h = (h + 1) & smask;
             }
-            h = (h + 1) & smask;
         }
-        st[h].ps = x.ps; st[h].dist = x.dist; st[h].x = x.x;
+        cacheE = -1;
+        if (ins) {
+            if (scount >= slimit) { flags |= SF_CAPACITY; dead = true; return; }
+            ++scount;
+            if (scount > srefcap) flags |= SF_REF_OVERFLOW;
+        }
+        BSlotS ns;
+        ns.key = e; ns.ps = x.ps; ns.dist = x.dist; ns.x = x.x; ns.pad = 0.0;
+        st[h] = ns;
     }
 
     // --- window heap: PriorityQueue.cuh:51-105 semantics on (key, record) entries
@@ -839,6 +879,7 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
     p.slimit = (int)(a.scap - (a.scap >> 2));
     p.vcount = 0; p.scount = 0;
     p.bcV0 = -1; p.bcV1 = -1;
+    p.cacheV = -1; p.cacheE = -1;
     p.flags = 0; p.dead = false;
     p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
     p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;
