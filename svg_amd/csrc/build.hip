@@ -328,26 +328,88 @@ struct BatchedPolicy {
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
 
-    __device__ void hset(int i, const BHeapEnt& e)
+    __device__ __forceinline__ void hset(int i, const BHeapEnt& e)
     {
         if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
-        else wh[bslot(i)] = e;
-        if (i == pfI0) pfE0 = e;
-        if (i == pfI1) pfE1 = e;
+        else {
+            wh[bslot(i)] = e;
+            if (hintA >= 0) chain_update(i, e);
+        }
     }
-    // Parents of the next two insertion slots, read ahead (at the start of a
-    // propagation, while its face and table reads are in flight) so the first
-    // comparison of each of the step's pushes finds its entry in registers;
-    // hset keeps the copies current.
-    int pfI0 = -1, pfI1 = -1;
-    BHeapEnt pfE0, pfE1;
-    __device__ void push_hint()
+    // The global-level ancestors (index >= kHeapLds) of the two slots a
+    // propagation's pushes fill, A = wtail and A + 1, read at the start of
+    // the propagation together with its face and table reads, so that the
+    // sift-ups compare against registers instead of waiting on memory level
+    // by level.  chA[s] holds the entry at A >> (s + 1), chB[s] the one at
+    // (A + 1) >> (s + 1) where that differs from A's ancestor (where they
+    // coincide, chA[s] serves both).  A push into A keeps chA current, so
+    // the push into A + 1 after it still reads current entries; a window
+    // pop, or the push into A + 1, ends the hint (hintA = -1).
+    static constexpr int kChain = 5;          // global ancestors of a slot below 1024
+    int hintA = -1;
+    // chA<s> / chB<s> as named members: constant indices keep them in registers
+    BHeapEnt cA0, cA1, cA2, cA3, cA4, cB0, cB1, cB2, cB3, cB4;
+    template <int S> __device__ __forceinline__ BHeapEnt& chA()
     {
-        const int a = wtail / 2, b = (wtail + 1) / 2;
-        pfI0 = a >= kHeapLds ? a : -1;
-        pfI1 = b >= kHeapLds && b != a ? b : -1;
-        if (pfI0 >= 0) pfE0 = wh[bslot(a)];
-        if (pfI1 >= 0) pfE1 = wh[bslot(b)];
+        if constexpr (S == 0) return cA0;
+        else if constexpr (S == 1) return cA1;
+        else if constexpr (S == 2) return cA2;
+        else if constexpr (S == 3) return cA3;
+        else return cA4;
+    }
+    template <int S> __device__ __forceinline__ BHeapEnt& chB()
+    {
+        if constexpr (S == 0) return cB0;
+        else if constexpr (S == 1) return cB1;
+        else if constexpr (S == 2) return cB2;
+        else if constexpr (S == 3) return cB3;
+        else return cB4;
+    }
+    template <int S> __device__ __forceinline__ void chain_update1(int i, const BHeapEnt& e)
+    {
+        const int pa = hintA >> (S + 1), pb = (hintA + 1) >> (S + 1);
+        BHeapEnt& x = chA<S>();
+        BHeapEnt& y = chB<S>();
+        const bool ma = i == pa, mb = !ma && i == pb;
+        x.key = ma ? e.key : x.key; x.idx = ma ? e.idx : x.idx;
+        y.key = mb ? e.key : y.key; y.idx = mb ? e.idx : y.idx;
+    }
+    // keeps the copies current when slot i (>= kHeapLds) is written
+    __device__ __forceinline__ void chain_update(int i, const BHeapEnt& e)
+    {
+        chain_update1<0>(i, e); chain_update1<1>(i, e); chain_update1<2>(i, e);
+        chain_update1<3>(i, e); chain_update1<4>(i, e);
+    }
+    template <int S> __device__ __forceinline__ void hint1(int A)
+    {
+        const int pa = A >> (S + 1), pb = (A + 1) >> (S + 1);
+        if (pa >= kHeapLds) chA<S>() = wh[bslot(pa)];
+        if (pb >= kHeapLds && pb != pa) chB<S>() = wh[bslot(pb)];
+    }
+    __device__ __forceinline__ void push_hint()
+    {
+        const int A = wtail;
+        hintA = A;
+        hint1<0>(A); hint1<1>(A); hint1<2>(A); hint1<3>(A); hint1<4>(A);
+    }
+    // One step of a hinted sift-up: compare with ancestor slot >> (S + 1).
+    template <int S> __device__ __forceinline__ void chain_step(int slot, bool useA, double key, int& cur, bool& done)
+    {
+        const int pp = slot >> (S + 1);
+        if (!done && pp >= kHeapLds) {
+            const bool inA = useA || pp == (hintA >> (S + 1));
+            const BHeapEnt ea = chA<S>(), eb = chB<S>();
+            BHeapEnt pe;
+            pe.key = inA ? ea.key : eb.key;
+            pe.idx = inA ? ea.idx : eb.idx;
+            pe.pad = 0;
+            if (!(pe.key > key)) {
+                done = true;
+            } else {
+                hset(cur, pe);
+                cur = pp;
+            }
+        }
     }
     // pseudo heap
     BPsEnt* ph; int ptail;
@@ -356,22 +418,22 @@ struct BatchedPolicy {
     bool dead;
     int c_push, c_pop, c_ppush, c_ppop, c_prop, c_child, c_sw, c_vw, c_fan, c_nb, c_hops, c_err;
 
-    __device__ int twin(int e) const { return F[e / 3].tw[e % 3]; }
-    __device__ static int next(int e) { return (e % 3 == 2) ? e - 2 : e + 1; }
-    __device__ static int prev(int e) { return (e % 3 == 0) ? e + 2 : e - 1; }
-    __device__ int v0(int e) const { return F[e / 3].v[e % 3]; }
-    __device__ int v1(int e) const { return F[e / 3].v[(e % 3 == 2) ? 0 : e % 3 + 1]; }
-    __device__ double len(int e) const { return F[e / 3].l[e % 3]; }
-    __device__ int first_edge(int v) const { return firstE[v]; }
-    __device__ uint8_t vflag(int v) const { return vflags[v]; }
-    __device__ double cornerA(int e) const { return cA[e]; }
-    __device__ double cornerB(int e) const { return cB[e]; }
-    __device__ int nv() const { return nv_; }
-    __device__ int nf() const { return nf_; }
-    __device__ bool stop() const { return dead; }
+    __device__ __forceinline__ int twin(int e) const { return F[e / 3].tw[e % 3]; }
+    __device__ __forceinline__ static int next(int e) { return (e % 3 == 2) ? e - 2 : e + 1; }
+    __device__ __forceinline__ static int prev(int e) { return (e % 3 == 0) ? e + 2 : e - 1; }
+    __device__ __forceinline__ int v0(int e) const { return F[e / 3].v[e % 3]; }
+    __device__ __forceinline__ int v1(int e) const { return F[e / 3].v[(e % 3 == 2) ? 0 : e % 3 + 1]; }
+    __device__ __forceinline__ double len(int e) const { return F[e / 3].l[e % 3]; }
+    __device__ __forceinline__ int first_edge(int v) const { return firstE[v]; }
+    __device__ __forceinline__ uint8_t vflag(int v) const { return vflags[v]; }
+    __device__ __forceinline__ double cornerA(int e) const { return cA[e]; }
+    __device__ __forceinline__ double cornerB(int e) const { return cB[e]; }
+    __device__ __forceinline__ int nv() const { return nv_; }
+    __device__ __forceinline__ int nf() const { return nf_; }
+    __device__ __forceinline__ bool stop() const { return dead; }
     // The record is read as three 16-B vectors and rotated with selects, so
     // nothing is indexed dynamically in registers (no scratch).
-    __device__ void face3(int e0, Tri& t) const
+    __device__ __forceinline__ void face3(int e0, Tri& t) const
     {
         const int f = e0 / 3, j = e0 - 3 * f;
         const uint4* p = reinterpret_cast<const uint4*>(F + f);
@@ -394,7 +456,7 @@ struct BatchedPolicy {
         t.tw1 = j0 ? T1 : (j1 ? T2 : T0);
         t.tw2 = j0 ? T2 : (j1 ? T0 : T1);
     }
-    __device__ void count(int which, long long n)
+    __device__ __forceinline__ void count(int which, long long n)
     {
         const int k = (int)n;
         switch (which) {
@@ -421,7 +483,7 @@ struct BatchedPolicy {
     // again.  Only vput changes the table, and every vput clears it.
     int cacheV = -1; unsigned cacheH = 0; bool cacheIns = false;
     int cacheE = -1; unsigned cacheSH = 0; bool cacheSIns = false;
-    __device__ VInfo vresolve(BSlotV s, unsigned& h, int v, bool& found) const
+    __device__ __forceinline__ VInfo vresolve(BSlotV s, unsigned& h, int v, bool& found) const
     {
         found = false;
         for (;;) {
@@ -440,7 +502,7 @@ struct BatchedPolicy {
             s = vt[h];
         }
     }
-    __device__ VInfo vget(int v) const
+    __device__ __forceinline__ VInfo vget(int v) const
     {
         unsigned h = vslot((unsigned)v) & vmask;
         bool f;
@@ -453,7 +515,7 @@ struct BatchedPolicy {
     // window top's pseudo-source, which change rarely); vput keeps them
     // current, the table reset clears them.
     int bcV0 = -1, bcB0 = 0, bcV1 = -1, bcB1 = 0;
-    __device__ int birth_of(int v)
+    __device__ __forceinline__ int birth_of(int v)
     {
         if (v == src) return 0;
         if (v == bcV0) return bcB0;
@@ -464,7 +526,7 @@ struct BatchedPolicy {
         return b;
     }
     // A probe reads the whole 32-B slot at once (key and record together).
-    __device__ SInfo sresolve(unsigned& h, int e, bool& found) const
+    __device__ __forceinline__ SInfo sresolve(unsigned& h, int e, bool& found) const
     {
         found = false;
         for (;;) {
@@ -478,7 +540,7 @@ struct BatchedPolicy {
             h = (h + 1) & smask;
         }
     }
-    __device__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp)
+    __device__ __forceinline__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp)
     {
         unsigned ha = vslot((unsigned)t.a) & vmask;
         unsigned hb = vslot((unsigned)t.b) & vmask;
@@ -493,7 +555,7 @@ struct BatchedPolicy {
         cacheV = t.op; cacheH = ho; cacheIns = !fo;
         cacheE = t.e0; cacheSH = hs; cacheSIns = !fs;
     }
-    __device__ void vput(int v, const VInfo& x)
+    __device__ __forceinline__ void vput(int v, const VInfo& x)
     {
         if (dead) return;
         unsigned h;
@@ -524,15 +586,15 @@ struct BatchedPolicy {
         if (v == bcV0) bcB0 = (int)x.birth;
         if (v == bcV1) bcB1 = (int)x.birth;
     }
-    __device__ void vtouch(int) {}
+    __device__ __forceinline__ void vtouch(int) {}
     // --- split table
-    __device__ SInfo sget(int e) const
+    __device__ __forceinline__ SInfo sget(int e) const
     {
         unsigned h = hmix((unsigned)e) & smask;
         bool f;
         return sresolve(h, e, f);
     }
-    __device__ void sput(int e, const SInfo& x)
+    __device__ __forceinline__ void sput(int e, const SInfo& x)
     {
         if (dead) return;
         unsigned h;
@@ -566,7 +628,7 @@ struct BatchedPolicy {
     // read after an allocation is needed only two allocations later, so
     // the two pushes of one propagation never wait on it.  A slot below
     // the register part exists only while both registers are filled.
-    __device__ int walloc()
+    __device__ __forceinline__ int walloc()
     {
         int idx;
         if (wcache >= 0) {
@@ -578,14 +640,14 @@ struct BatchedPolicy {
         }
         return idx;
     }
-    __device__ void wrelease(int idx)
+    __device__ __forceinline__ void wrelease(int idx)
     {
         if (wcache2 >= 0) wfree[wftop++] = (uint16_t)wcache2;
         wcache2 = wcache;
         wcache = idx;
     }
-    __device__ bool wempty() const { return wtail == 1; }
-    __device__ void wtop(Win& w) const
+    __device__ __forceinline__ bool wempty() const { return wtail == 1; }
+    __device__ __forceinline__ void wtop(Win& w) const
     {
         // Every change of the root refreshes nrec (wpop loads it, wpush
         // copies the new record), so the root's record is always at hand.
@@ -661,8 +723,9 @@ struct BatchedPolicy {
     __device__ __forceinline__ double lkey(int i) const { return lk[(i - 1) * lstride]; }
     __device__ __forceinline__ int lidx(int i) const { return li[(i - 1) * lstride]; }
     __device__ __forceinline__ BHeapEnt gget(int i) const { return wh[bslot(i)]; }
-    __device__ void wpop()
+    __device__ __forceinline__ void wpop()
     {
+        hintA = -1;
         wrelease(rootIdx);
         --wtail;
         // the last entry: read from both parts, kept from the one it is in
@@ -722,7 +785,7 @@ struct BatchedPolicy {
         if (deeper && sift_blocks(cur, top, 2 * cur - 16)) sift_blocks(cur, top, 16 + (2 * cur - 128));
         hset(cur, top);
     }
-    __device__ void wpush(const Win& w)
+    __device__ __forceinline__ void wpush(const Win& w)
     {
         if (wtail - 1 >= kHeapCap) { flags |= SF_WIN_DROP; return; }
         const int idx = walloc();
@@ -732,16 +795,32 @@ struct BatchedPolicy {
         rec.lvlb = (w.level << 8) | (w.birth & 0xFF); rec.twin = w.twin;
         wr[idx] = rec;
         ++wtail;
-        int cur = wtail - 1, p = cur / 2;
+        const int slot = wtail - 1;
+        int cur = slot, p = cur / 2;
         const double key = w.minDist;
         BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
         // global levels of the path (parents with index >= 16), then LDS ones
         bool done = false;
-        while (p >= kHeapLds) {
-            const BHeapEnt pe = p == pfI0 ? pfE0 : (p == pfI1 ? pfE1 : gget(p));
-            if (!(pe.key > key)) { done = true; break; }
-            hset(cur, pe);
-            cur = p; p = cur / 2;
+        const int A = hintA;
+        const bool useA = A >= 0 && slot == A;
+        const bool useB = A >= 0 && slot == A + 1;
+        if (useA || useB) {
+            // step S compares with the ancestor slot >> (S + 1) from its
+            // register copy (hset keeps the copies current as entries move)
+            chain_step<0>(slot, useA, key, cur, done);
+            chain_step<1>(slot, useA, key, cur, done);
+            chain_step<2>(slot, useA, key, cur, done);
+            chain_step<3>(slot, useA, key, cur, done);
+            chain_step<4>(slot, useA, key, cur, done);
+            p = cur / 2;
+            // (every global ancestor of a slot below 1024 is cached)
+        } else {
+            while (p >= kHeapLds) {
+                const BHeapEnt pe = gget(p);
+                if (!(pe.key > key)) { done = true; break; }
+                hset(cur, pe);
+                cur = p; p = cur / 2;
+            }
         }
         if (!done) {
             while (p > 0) {
@@ -753,11 +832,12 @@ struct BatchedPolicy {
             }
         }
         hset(cur, ne);
+        if (useB) hintA = -1;
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
     // --- pseudo heap (inline 24-B entries, root cached in registers)
-    __device__ bool pempty() const { return ptail == 1; }
-    __device__ void ptop(Pseudo& q) const
+    __device__ __forceinline__ bool pempty() const { return ptail == 1; }
+    __device__ __forceinline__ void ptop(Pseudo& q) const
     {
         const BPsEnt e = pRoot;
         q.dist = e.key; q.vert = e.vert; q.pseudoSrc = e.pseudoSrc;
@@ -765,7 +845,7 @@ struct BatchedPolicy {
     }
     // Sift-down two levels per round trip: a node's children and
     // grandchildren (contiguous in the array) are read together.
-    __device__ void ppop()
+    __device__ __forceinline__ void ppop()
     {
         --ptail;
         const BPsEnt top = ph[ptail];
@@ -805,7 +885,7 @@ struct BatchedPolicy {
         ph[cur] = top;
         if (!movedRoot) pRoot = top;
     }
-    __device__ void ppush(const Pseudo& q)
+    __device__ __forceinline__ void ppush(const Pseudo& q)
     {
         if (ptail - 1 >= kHeapCap) { flags |= SF_PSEUDO_DROP; return; }
         ++ptail;
@@ -880,6 +960,7 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
     p.vcount = 0; p.scount = 0;
     p.bcV0 = -1; p.bcV1 = -1;
     p.cacheV = -1; p.cacheE = -1;
+    p.hintA = -1;
     p.flags = 0; p.dead = false;
     p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
     p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;
