@@ -331,10 +331,7 @@ struct BatchedPolicy {
     __device__ __forceinline__ void hset(int i, const BHeapEnt& e)
     {
         if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
-        else {
-            wh[bslot(i)] = e;
-            if (hintA >= 0) chain_update(i, e);
-        }
+        else wh[bslot(i)] = e;
     }
     // The global-level ancestors (index >= kHeapLds) of the two slots a
     // propagation's pushes fill, A = wtail and A + 1, read at the start of
@@ -365,20 +362,20 @@ struct BatchedPolicy {
         else if constexpr (S == 3) return cB3;
         else return cB4;
     }
-    template <int S> __device__ __forceinline__ void chain_update1(int i, const BHeapEnt& e)
+    // After the push into A, the push into A + 1 reads chA where the two
+    // chains share an ancestor, so the push into A keeps chA current: the
+    // copy of ancestor S follows what is written to its slot.
+    template <int S> __device__ __forceinline__ void chain_set1(int pos, const BHeapEnt& e)
     {
-        const int pa = hintA >> (S + 1), pb = (hintA + 1) >> (S + 1);
         BHeapEnt& x = chA<S>();
-        BHeapEnt& y = chB<S>();
-        const bool ma = i == pa, mb = !ma && i == pb;
-        x.key = ma ? e.key : x.key; x.idx = ma ? e.idx : x.idx;
-        y.key = mb ? e.key : y.key; y.idx = mb ? e.idx : y.idx;
+        const bool m = pos == (hintA >> (S + 1));
+        x.key = m ? e.key : x.key;
+        x.idx = m ? e.idx : x.idx;
     }
-    // keeps the copies current when slot i (>= kHeapLds) is written
-    __device__ __forceinline__ void chain_update(int i, const BHeapEnt& e)
+    __device__ __forceinline__ void chain_set(int pos, const BHeapEnt& e)
     {
-        chain_update1<0>(i, e); chain_update1<1>(i, e); chain_update1<2>(i, e);
-        chain_update1<3>(i, e); chain_update1<4>(i, e);
+        chain_set1<0>(pos, e); chain_set1<1>(pos, e); chain_set1<2>(pos, e);
+        chain_set1<3>(pos, e); chain_set1<4>(pos, e);
     }
     template <int S> __device__ __forceinline__ void hint1(int A)
     {
@@ -407,6 +404,14 @@ struct BatchedPolicy {
                 done = true;
             } else {
                 hset(cur, pe);
+                // cur is the slot itself (S = 0) or ancestor S - 1
+                if constexpr (S > 0) {
+                    if (useA) {
+                        BHeapEnt& x = chA<S - 1>();
+                        x.key = pe.key;
+                        x.idx = pe.idx;
+                    }
+                }
                 cur = pp;
             }
         }
@@ -827,10 +832,12 @@ struct BatchedPolicy {
                 const double pk = lkey(p);
                 if (!(pk > key)) break;
                 BHeapEnt pe; pe.key = pk; pe.idx = lidx(p); pe.pad = 0;
+                if (useA && cur >= kHeapLds) chain_set(cur, pe);
                 hset(cur, pe);
                 cur = p; p = cur / 2;
             }
         }
+        if (useA && cur >= kHeapLds) chain_set(cur, ne);
         hset(cur, ne);
         if (useB) hintA = -1;
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
