@@ -383,10 +383,20 @@ struct BatchedPolicy {
         if (pa >= kHeapLds) chA<S>() = wh[bslot(pa)];
         if (pb >= kHeapLds && pb != pa) chB<S>() = wh[bslot(pb)];
     }
+    // the parent of the next pseudo-heap slot, read with the same batch
+    int pHintI = -1;
+    double pHk; int32_t pHv, pHs, pHl;
     __device__ __forceinline__ void push_hint()
     {
         const int A = wtail;
         hintA = A;
+        pHintI = ptail / 2;
+        if (pHintI > 0) {
+            const uint2* q = reinterpret_cast<const uint2*>(ph + pHintI);
+            const uint2 x = q[0], y = q[1], z = q[2];
+            pHk = __hiloint2double((int)x.y, (int)x.x);
+            pHv = (int32_t)y.x; pHs = (int32_t)y.y; pHl = (int32_t)z.x;
+        }
         hint1<0>(A); hint1<1>(A); hint1<2>(A); hint1<3>(A); hint1<4>(A);
     }
     // One step of a hinted sift-up: compare with ancestor slot >> (S + 1).
@@ -854,6 +864,7 @@ struct BatchedPolicy {
     // grandchildren (contiguous in the array) are read together.
     __device__ __forceinline__ void ppop()
     {
+        pHintI = -1;
         --ptail;
         const BPsEnt top = ph[ptail];
         int cur = 1;
@@ -898,7 +909,22 @@ struct BatchedPolicy {
         ++ptail;
         int cur = ptail - 1, p = cur / 2;
         const double key = q.dist;
-        while (p > 0) {
+        const int hinted = pHintI;
+        pHintI = -1;
+        bool done = false;
+        // the first level from the parent read ahead (its own block of code:
+        // no choice between a register copy and a memory read)
+        if (p > 0 && p == hinted) {
+            if (!(pHk > key)) {
+                done = true;
+            } else {
+                BPsEnt pe;
+                pe.key = pHk; pe.vert = pHv; pe.pseudoSrc = pHs; pe.lvlb = pHl; pe.pad = 0;
+                ph[cur] = pe;
+                cur = p; p = cur / 2;
+            }
+        }
+        while (!done && p > 0) {
             const BPsEnt pe = ph[p];
             if (!(pe.key > key)) break;
             ph[cur] = pe;
@@ -968,6 +994,7 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
     p.bcV0 = -1; p.bcV1 = -1;
     p.cacheV = -1; p.cacheE = -1;
     p.hintA = -1;
+    p.pHintI = -1;
     p.flags = 0; p.dead = false;
     p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
     p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;
