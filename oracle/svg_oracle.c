@@ -73,6 +73,12 @@
 
 /* ------------------------------------------------------------------------ */
 /* acos, fdlibm method (D4)                                                  */
+/* The coefficients and reductions follow fdlibm's e_acos.c, whose notice   */
+/* is preserved as its licence asks:                                        */
+/*   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.      */
+/*   Developed at SunPro, a Sun Microsystems, Inc. business.                */
+/*   Permission to use, copy, modify, and distribute this software is       */
+/*   freely granted, provided that this notice is preserved.                */
 /* ------------------------------------------------------------------------ */
 static const double o_pi = 3.14159265358979311600e+00;
 static const double o_pio2_hi = 1.57079632679489655800e+00;

@@ -6,6 +6,13 @@
 // operations only (+ - * / sqrt), so the host and the device produce the same
 // bits when both are compiled with -ffp-contract=off.  Error < 1 ulp against
 // glibc's acos.
+//
+// The coefficients and reductions follow fdlibm's e_acos.c, whose notice is
+// preserved as its licence asks:
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this software is freely
+//   granted, provided that this notice is preserved.
 #pragma once
 
 #include <hip/hip_runtime.h>
