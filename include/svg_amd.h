@@ -75,6 +75,11 @@ typedef struct {
     int autotune;           /* batched: learn from each build of this mesh (per K and
                                coefficients) the source order, table sizes and block shape */
     int lanes_per_wave;     /* batched: sources stepped by one 64-wide wave (1..64; 0 = auto) */
+    int queue_cap;          /* batched: window / pseudo-window queue storage per source lane
+                               (1..1023; 0 = the reference's 1023, or with autotune the peak
+                               recorded by earlier builds).  A source that outgrows a storage
+                               below 1023 is recomputed exactly on the plain path, so the
+                               rows never depend on it. */
 } svg_build_params;
 
 typedef struct {
@@ -95,6 +100,8 @@ typedef struct {
                                children, split/vert writes, fan half-edges, pseudo
                                neighbours, -, -, backtrace hops, reached, -, -) */
     int lanes_per_wave;     /* sources per wave of the propagation kernel */
+    int queue_cap;          /* window queue storage per lane used (1023: the reference's) */
+    int pseudo_queue_cap;   /* pseudo-window queue storage per lane used */
     int fallback_plain;     /* 1: a batched build ran every source on the plain path (the
                                mesh has 2^24-1 or more half-edges) */
 } svg_build_stats;

@@ -323,6 +323,8 @@ struct BatchedPolicy {
     BSlotS* st; unsigned smask; int scount; int srefcap; int slimit;
     // window heap
     BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
+    int wcap, pcap;                  // queue storage per lane (<= kHeapCap); wmax, pmax: peak lengths
+    int wmax, pmax;
     double* lk; int* li; int lstride;   // LDS part of the heap (logical 1..15), lane-major
     int wtail, wftop, wfresh, wcache, wcache2;
     double rootKey; int rootIdx;
@@ -802,7 +804,13 @@ struct BatchedPolicy {
     }
     __device__ __forceinline__ void wpush(const Win& w)
     {
-        if (wtail - 1 >= kHeapCap) { flags |= SF_WIN_DROP; return; }
+        if (wtail - 1 >= wcap) {
+            // at the reference capacity a push is dropped (PriorityQueue.cuh:53); below it the
+            // source outgrew its storage and is recomputed on the plain path
+            if (wcap >= kHeapCap) flags |= SF_WIN_DROP;
+            else { flags |= SF_CAPACITY; dead = true; }
+            return;
+        }
         const int idx = walloc();
         BWinRec rec;
         rec.b0 = w.b0; rec.b1 = w.b1; rec.ps = w.ps; rec.sx = w.sx; rec.sy = w.sy;
@@ -811,6 +819,7 @@ struct BatchedPolicy {
         wr[idx] = rec;
         ++wtail;
         const int slot = wtail - 1;
+        wmax = slot > wmax ? slot : wmax;
         int cur = slot, p = cur / 2;
         const double key = w.minDist;
         BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
@@ -905,8 +914,13 @@ struct BatchedPolicy {
     }
     __device__ __forceinline__ void ppush(const Pseudo& q)
     {
-        if (ptail - 1 >= kHeapCap) { flags |= SF_PSEUDO_DROP; return; }
+        if (ptail - 1 >= pcap) {
+            if (pcap >= kHeapCap) flags |= SF_PSEUDO_DROP;
+            else { flags |= SF_CAPACITY; dead = true; }
+            return;
+        }
         ++ptail;
+        pmax = ptail - 1 > pmax ? ptail - 1 : pmax;
         int cur = ptail - 1, p = cur / 2;
         const double key = q.dist;
         const int hinted = pHintI;
@@ -960,10 +974,15 @@ struct BArgs {
     int hopCap;
     const int32_t* order;       // sources in processing order (null: by id)
     int pseudoShare;            // pseudo events run when nP * pseudoShare >= the wave's stepping lanes
+    int wcap, pcap;             // window / pseudo queue storage per lane (kHeapCap: the reference's)
+    int hslots;                 // window heap slots per lane (blocked layout)
     int lanesPerWave;           // source lanes per 64-wide wave
 };
 
-constexpr int kNumBatchedCounters = 13;
+constexpr int kNumBatchedCounters = 15;
+// Launch shapes the autotuner tries: threads per block, source lanes per wave.
+constexpr int kTuneShapes = 5;
+constexpr int kTuneShape[kTuneShapes][2] = {{64, 32}, {128, 32}, {256, 32}, {128, 28}, {256, 28}};
 // A wave runs its pseudo-source events once at least 1/kPseudoShare of its
 // stepping lanes wait for one.
 constexpr int kPseudoShare = 4;
@@ -1016,10 +1035,11 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     p.lstride = lanesPerBlock;
     p.lk = ldsHeap + bl;
     p.li = reinterpret_cast<int*>(ldsHeap + (kHeapLds - 1) * lanesPerBlock) + bl;
-    p.wh = a.wheap + (size_t)lane * kHeapSlots;
-    p.wr = a.wrec + (size_t)lane * (kHeapCap + 1);
-    p.wfree = a.wfree + (size_t)lane * (kHeapCap + 1);
-    p.ph = a.pheap + (size_t)lane * (kHeapCap + 1);
+    p.wh = a.wheap + (size_t)lane * a.hslots;
+    p.wr = a.wrec + (size_t)lane * (a.wcap + 1);
+    p.wfree = a.wfree + (size_t)lane * (a.wcap + 1);
+    p.ph = a.pheap + (size_t)lane * (a.pcap + 1);
+    p.wcap = a.wcap; p.pcap = a.pcap;
     using I = Ich<BatchedPolicy>;
     I ich(p, 0, a.K);
     Win top;
@@ -1039,6 +1059,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
             policy_tables(p, a, slot, src);
             p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.wcache2 = -1; p.rootKey = 0.0; p.rootIdx = 0;
+            p.wmax = 0; p.pmax = 0;
             p.nrecIdx = -1;
             p.ptail = 1;
             ich.reset(src);
@@ -1062,7 +1083,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
             c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
             c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
-            c[10] = 0; c[11] = p.vcount; c[12] = p.scount;
+            c[10] = 0; c[11] = p.vcount; c[12] = p.scount; c[13] = p.wmax; c[14] = p.pmax;
             have = false;
         }
     }
@@ -1161,33 +1182,50 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         st.fallbackPlain = 1;
         return e;
     }
-    const int lanesPerWave = c.lanesPerWave > 0 ? c.lanesPerWave : kLanesPerWave;
+    int lanesPerWave = c.lanesPerWave > 0 ? c.lanesPerWave : kLanesPerWave;
     if (lanesPerWave > 64) return "lanes_per_wave must be at most 64";
+    if (c.queueCap < 0 || c.queueCap > kHeapCap) return "queue_cap must be in [0, 1023]";
     const int vrefcap = (int)((unsigned)K * c.vertCoef);
     const int srefcap = (int)((unsigned)K * c.splitCoef);
     unsigned vcap = c.vertCap > 0 ? next_pow2((unsigned)c.vertCap) : next_pow2(2u * (unsigned)vrefcap + 2u);
     unsigned scap = c.splitCap > 0 ? next_pow2((unsigned)c.splitCap) : next_pow2(4u * (unsigned)K);
     int block = c.propBlock > 0 ? c.propBlock : 64;
+    // Queue storage per lane: the reference's 1023 items unless set; a
+    // source that outgrows a smaller storage is recomputed exactly on the
+    // plain path (the rows never depend on it).
+    int wcap = c.queueCap > 0 ? c.queueCap : kHeapCap;
+    int pcap = c.queueCap > 0 ? c.queueCap : kHeapCap;
     // Autotuner (per mesh, K and coefficients; results never depend on it):
     // once a build has recorded every source's event count, the cheapest
     // sources are the ones left to start late; when the default tables would
     // not fit at once they are sized to the peak occupancy recorded (a source
     // that still overflows is recomputed exactly on the plain path); the
-    // block shape is the fastest of 64/128/256 threads, each tried once on
-    // successive builds.
+    // queues get the storage their recorded peak needs (with a quarter of
+    // headroom) when no source reached the reference capacity; the launch
+    // shape (threads per block, source lanes per wave) is the fastest of a
+    // few candidates, each tried once on successive builds.
     TuneRecord* tune = c.autotune ? c.tune : nullptr;
     std::vector<int32_t> order;
     int blockSlot = -1;
     if (tune) {
-        static const int kBlocks[3] = {64, 128, 256};
-        if (c.propBlock <= 0) {
+        if (c.propBlock <= 0 && c.lanesPerWave <= 0) {
             int best = -1;
-            for (int i = 0; i < 3 && blockSlot < 0; ++i) if (tune->blockMs[i] <= 0.0) blockSlot = i;
+            for (int i = 0; i < kTuneShapes && blockSlot < 0; ++i) if (tune->blockMs[i] <= 0.0) blockSlot = i;
             if (blockSlot < 0) {
-                for (int i = 0; i < 3; ++i) if (best < 0 || tune->blockMs[i] < tune->blockMs[best]) best = i;
+                for (int i = 0; i < kTuneShapes; ++i) if (best < 0 || tune->blockMs[i] < tune->blockMs[best]) best = i;
                 blockSlot = best;
             }
-            block = kBlocks[blockSlot];
+            block = kTuneShape[blockSlot][0];
+            lanesPerWave = kTuneShape[blockSlot][1];
+        }
+        if (c.queueCap <= 0 && (int)tune->cost.size() == V) {
+            auto capOf = [](int peak) {
+                int q = 1;
+                while (q - 1 < peak + peak / 4 + 2) q <<= 1;
+                return std::min(kHeapCap, q - 1);
+            };
+            if (tune->maxWinQ < kHeapCap) wcap = capOf(tune->maxWinQ);
+            if (tune->maxPseudoQ < kHeapCap) pcap = capOf(tune->maxPseudoQ);
         }
         const int nLanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
         if ((int)tune->cost.size() == V && nLanes < V) {
@@ -1220,8 +1258,10 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         const hipError_t me = hipMemGetInfo(&freeB, &totB);
         if (me != hipSuccess) return std::string("batched build: hipMemGetInfo: ") + hipGetErrorString(me);
     }
-    const size_t perLane = sizeof(BHeapEnt) * kHeapSlots +
-                           (sizeof(BWinRec) + sizeof(uint16_t) + sizeof(BPsEnt)) * (kHeapCap + 1);
+    // heap blocks: 16 (slots below 128) or all 144
+    const int hslots = (wcap < 128 ? 16 : kHeapBlocks) * 8;
+    const size_t perLane = sizeof(BHeapEnt) * (size_t)hslots + (sizeof(BWinRec) + sizeof(uint16_t)) * (size_t)(wcap + 1) +
+                           sizeof(BPsEnt) * (size_t)(pcap + 1);
     size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
     int chunk = c.chunk > 0 ? c.chunk : V;
     const size_t budget = freeB > ((size_t)lanes * perLane + (4ull << 30)) ? freeB - (size_t)lanes * perLane - (4ull << 30) : 0;
@@ -1237,7 +1277,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         if ((size_t)chunk > maxChunk) chunk = (int)std::max<size_t>(maxChunk, 1);
     }
     if (chunk > V) chunk = V;
-    st.lanes = lanes; st.propBlock = block; st.lanesPerWave = lanesPerWave; st.chunk = chunk; st.vertCap = (int)vcap; st.splitCap = (int)scap;
+    st.lanes = lanes; st.propBlock = block; st.lanesPerWave = lanesPerWave; st.queueCap = wcap;
+    st.pseudoCap = pcap; st.chunk = chunk; st.vertCap = (int)vcap; st.splitCap = (int)scap;
 
     auto t0 = std::chrono::steady_clock::now();
     BArgs a;
@@ -1245,15 +1286,16 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
     a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
     a.pseudoShare = kPseudoShare;
+    a.wcap = wcap; a.pcap = pcap; a.hslots = hslots;
     a.lanesPerWave = lanesPerWave;
     std::string err;
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.stab, sizeof(BSlotS) * scap * (size_t)chunk) != hipSuccess ||
-        hipMalloc(&a.wheap, sizeof(BHeapEnt) * kHeapSlots * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.wrec, sizeof(BWinRec) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.wfree, sizeof(uint16_t) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.pheap, sizeof(BPsEnt) * (kHeapCap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wheap, sizeof(BHeapEnt) * (size_t)hslots * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wrec, sizeof(BWinRec) * (size_t)(wcap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wfree, sizeof(uint16_t) * (size_t)(wcap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.pheap, sizeof(BPsEnt) * (size_t)(pcap + 1) * (size_t)lanes) != hipSuccess ||
         hipMalloc(&a.work, sizeof(int)) != hipSuccess ||
         hipMalloc(&a.status, sizeof(uint32_t) * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.vcount, sizeof(int32_t) * (size_t)chunk) != hipSuccess ||
@@ -1268,7 +1310,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         a.order = dOrder;
     }
     std::vector<int32_t> newCost;
-    int newMaxV = 0, newMaxS = 0;
+    int newMaxV = 0, newMaxS = 0, newMaxW = 0, newMaxP = 0;
     if (tune) newCost.assign((size_t)V, 0);
     st.allocSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::vector<int32_t> redo;
@@ -1339,6 +1381,9 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
                 const int grow = (hs[i] & SF_CAPACITY) ? 2 : 1;
                 newMaxV = std::max(newMaxV, cc[11] * grow);
                 newMaxS = std::max(newMaxS, cc[12] * grow);
+                // a source stopped by its queue storage has an unknown peak: use the reference's
+                newMaxW = std::max(newMaxW, (hs[i] & SF_CAPACITY) ? kHeapCap : cc[13]);
+                newMaxP = std::max(newMaxP, (hs[i] & SF_CAPACITY) ? kHeapCap : cc[14]);
             }
         }
     }
@@ -1346,6 +1391,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         tune->cost.swap(newCost);
         tune->maxVert = newMaxV;
         tune->maxSplit = newMaxS;
+        tune->maxWinQ = newMaxW;
+        tune->maxPseudoQ = newMaxP;
         if (blockSlot >= 0) tune->blockMs[blockSlot] = st.propSeconds * 1e3;
     }
     hipFree(dOrder);

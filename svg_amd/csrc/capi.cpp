@@ -168,6 +168,7 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     c.lanes = p->lanes; c.propBlock = p->prop_block; c.chunk = p->chunk;
     c.vertCap = p->vert_cap; c.splitCap = p->split_cap; c.autotune = p->autotune;
     c.lanesPerWave = p->lanes_per_wave;
+    c.queueCap = p->queue_cap;
     if (c.plainThreads > 1024 || (c.plainThreads % 64) != 0 && c.plainThreads > 0)
         return svg_set_error(SVG_E_ARG, "plain_threads must be a multiple of 64 up to 1024");
     svg_graph_s* g = new (std::nothrow) svg_graph_s();
@@ -222,6 +223,7 @@ extern "C" int svg_build_stats_get(svg_graph_t g, svg_build_stats* st)
     st->vert_cap = s.vertCap; st->split_cap = s.splitCap; st->tuned = s.tuned;
     for (size_t i = 0; i < s.counters.size() && i < 16; ++i) st->counters[i] = s.counters[i];
     st->lanes_per_wave = s.lanesPerWave; st->fallback_plain = s.fallbackPlain;
+    st->queue_cap = s.queueCap; st->pseudo_queue_cap = s.pseudoCap;
     return SVG_OK;
 }
 

@@ -18,7 +18,8 @@ struct TuneRecord {
     unsigned splitCoef = 0, vertCoef = 0;
     std::vector<int32_t> cost;     // per source: window + pseudo pops
     int maxVert = 0, maxSplit = 0; // largest table occupancy seen
-    double blockMs[3] = {0.0, 0.0, 0.0};   // propagation time with 64 / 128 / 256 threads
+    int maxWinQ = 1023, maxPseudoQ = 1023;  // longest window / pseudo queue seen
+    double blockMs[8] = {0.0};     // propagation time of each launch shape tried
 };
 
 struct BuildConfig {
@@ -36,6 +37,7 @@ struct BuildConfig {
     int vertCap = 0;             // per-source vertex table slots (power of two)
     int splitCap = 0;            // per-source split table slots (power of two)
     int lanesPerWave = 0;        // source lanes per 64-wide wave of the propagation kernel
+    int queueCap = 0;            // window / pseudo queue storage per lane (0: the reference's 1023)
     int autotune = 0;
     TuneRecord* tune = nullptr;  // autotune: what earlier builds of this mesh recorded
 };
@@ -52,6 +54,7 @@ struct BuildStats {
     int lanes = 0, propBlock = 0, chunk = 0, vertCap = 0, splitCap = 0;
     int tuned = 0;               // 1: order, capacities and block came from the autotuner
     int lanesPerWave = 0;
+    int queueCap = 0, pseudoCap = 0;
     int fallbackPlain = 0;       // 1: the mesh exceeds the batched form's limits; every source ran plain
     std::vector<long long> counters;   // summed event counters (CNT_NUM)
     std::vector<uint32_t> status;      // per-vertex status bits (SF_*)

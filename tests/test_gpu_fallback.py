@@ -134,6 +134,51 @@ def test_large_k_caps_the_vertex_table():
         g.FreeSVGStructure()
 
 
+@pytest.mark.parametrize("qcap", [31, 100])
+def test_small_queue_storage_is_recomputed_exactly(qcap):
+    # Queue storage below the reference's 1023 items: sources whose queues
+    # outgrow it are flagged and recomputed on the plain path, so the rows
+    # equal the oracle's, which queues up to 1023.
+    xyz, tri = TORUS()
+    K = 80
+    print(f"case queue_cap {qcap}", flush=True)
+    rows, tails, c = _oracle(xyz, tri, K)
+    _, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED, queue_cap=qcap)
+    st, stt = g.stats(), g.status()
+    assert st["queue_cap"] == qcap and st["pseudo_queue_cap"] == qcap
+    over = (c["max_winq"] > qcap) | (c["max_pseudoq"] > qcap)
+    assert over.any()
+    assert np.all(stt[over] & SF_CAPACITY) and np.all(stt[over] & SF_REDONE)
+    _assert_rows(g, rows, tails, K)
+
+
+def test_autotuned_queue_storage_and_shape():
+    # The autotuner sizes the queues to the recorded peak (with headroom)
+    # and tries the launch shapes one per build; the rows stay the oracle's.
+    xyz, tri = SPHERE()
+    K = 60
+    print("case autotuned queue storage", flush=True)
+    rows, tails, c = _oracle(xyz, tri, K)
+    m = S.Mesh.from_arrays(xyz, tri)
+    seen = set()
+    for it in range(7):
+        g = S.SVG()
+        g.AssignMesh(m)
+        g.SetParameters(K, 10, 2)
+        g.ConstructSVG(mode=S.SVG_MODE_BATCHED, autotune=1)
+        st = g.stats()
+        seen.add((st["prop_block"], st["lanes_per_wave"]))
+        if it == 0:
+            assert st["queue_cap"] == 1023 and st["tuned"] == 0
+        else:
+            assert st["tuned"] == 1
+            assert max(c["max_winq"]) <= st["queue_cap"] < 1023
+            assert max(c["max_pseudoq"]) <= st["pseudo_queue_cap"] < 1023
+        _assert_rows(g, rows, tails, K)
+        g.FreeSVGStructure()
+    assert len(seen) == 5
+
+
 @pytest.mark.parametrize("lpw", [1, 7, 16, 28, 64])
 def test_lanes_per_wave_do_not_change_rows(lpw):
     xyz, tri = TORUS()
