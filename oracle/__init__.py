@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsvg_oracle.so")
+# SVG_ORACLE_LIB names another build of the oracle (a sanitizer build).
+LIB_PATH = os.environ.get("SVG_ORACLE_LIB") or os.path.join(HERE, "libsvg_oracle.so")
 
 COUNTERS = [
     "win_push", "win_pop", "win_stale", "pseudo_push", "pseudo_pop", "pseudo_stale",

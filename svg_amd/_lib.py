@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsvg_amd.so")
+# SVG_AMD_LIB names another build of the same library (a sanitizer build
+# of the host code, for instance); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("SVG_AMD_LIB") or os.path.join(HERE, "libsvg_amd.so")
 
 SVG_MODE_PLAIN = 0
 SVG_MODE_BATCHED = 1
