@@ -402,7 +402,9 @@ def main():
         "first_build_untuned": first_build,
         "batched": {k: st[k] for k in ("prop_seconds", "extract_seconds", "redo_seconds", "redone",
                                        "flagged_drop", "flagged_overflow", "flagged_error", "lanes",
-                                       "prop_block", "chunk", "vert_cap", "split_cap", "tuned", "arcs")},
+                                       "prop_block", "lanes_per_wave", "queue_cap", "pseudo_queue_cap",
+                                       "chunk", "vert_cap", "split_cap", "tuned", "arcs")},
+        "windows_per_s": c["win_pop"] / st["prop_seconds"],
         "counters": c,
         "roofline": {"bound": "hbm", "kernel": "batched_prop_kernel",
                      "achieved": prop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
