@@ -279,7 +279,7 @@ struct alignas(16) BSlotV { int32_t key; uint32_t eb; double dist; };     // 16 
 struct alignas(16) BSlotS { int32_t key; int32_t ps; double dist; double x; double pad; };  // 32 B, two loads
 struct BWinRec { double b0, b1, ps, sx, sy; int32_t edge, pseudoSrc, lvlb, twin; double pad; };  // 64 B
 static_assert(sizeof(BWinRec) == 64, "window records fill one 64-B half-line");
-struct alignas(8) BPsEnt { double key; int32_t vert, pseudoSrc, lvlb, pad; };   // 24 B
+struct BPsRec { int32_t vert, pseudoSrc, lvlb, pad; };        // 16 B pseudo-window record
 struct BHeapEnt { double key; int32_t idx; int32_t pad; };     // 16 B
 
 constexpr uint32_t kEnterNone = 0xFFFFFFu;
@@ -344,25 +344,23 @@ struct BatchedPolicy {
     // coincide, chA[s] serves both).  A push into A keeps chA current, so
     // the push into A + 1 after it still reads current entries; a window
     // pop, or the push into A + 1, ends the hint (hintA = -1).
-    static constexpr int kChain = 5;          // global ancestors of a slot below 1024
+    static constexpr int kChain = 4;          // ancestors kept (a slot below 512 has no more global ones)
     int hintA = -1;
     // chA<s> / chB<s> as named members: constant indices keep them in registers
-    BHeapEnt cA0, cA1, cA2, cA3, cA4, cB0, cB1, cB2, cB3, cB4;
+    BHeapEnt cA0, cA1, cA2, cA3, cB0, cB1, cB2, cB3;
     template <int S> __device__ __forceinline__ BHeapEnt& chA()
     {
         if constexpr (S == 0) return cA0;
         else if constexpr (S == 1) return cA1;
         else if constexpr (S == 2) return cA2;
-        else if constexpr (S == 3) return cA3;
-        else return cA4;
+        else return cA3;
     }
     template <int S> __device__ __forceinline__ BHeapEnt& chB()
     {
         if constexpr (S == 0) return cB0;
         else if constexpr (S == 1) return cB1;
         else if constexpr (S == 2) return cB2;
-        else if constexpr (S == 3) return cB3;
-        else return cB4;
+        else return cB3;
     }
     // After the push into A, the push into A + 1 reads chA where the two
     // chains share an ancestor, so the push into A keeps chA current: the
@@ -377,7 +375,7 @@ struct BatchedPolicy {
     __device__ __forceinline__ void chain_set(int pos, const BHeapEnt& e)
     {
         chain_set1<0>(pos, e); chain_set1<1>(pos, e); chain_set1<2>(pos, e);
-        chain_set1<3>(pos, e); chain_set1<4>(pos, e);
+        chain_set1<3>(pos, e);
     }
     template <int S> __device__ __forceinline__ void hint1(int A)
     {
@@ -385,21 +383,21 @@ struct BatchedPolicy {
         if (pa >= kHeapLds) chA<S>() = wh[bslot(pa)];
         if (pb >= kHeapLds && pb != pa) chB<S>() = wh[bslot(pb)];
     }
-    // the parent of the next pseudo-heap slot, read with the same batch
+    // the parent of the next pseudo-heap slot when it lies in the global
+    // part, read with the same batch
     int pHintI = -1;
-    double pHk; int32_t pHv, pHs, pHl;
+    double pHk; int32_t pHi;
     __device__ __forceinline__ void push_hint()
     {
         const int A = wtail;
         hintA = A;
         pHintI = ptail / 2;
-        if (pHintI > 0) {
-            const uint2* q = reinterpret_cast<const uint2*>(ph + pHintI);
-            const uint2 x = q[0], y = q[1], z = q[2];
-            pHk = __hiloint2double((int)x.y, (int)x.x);
-            pHv = (int32_t)y.x; pHs = (int32_t)y.y; pHl = (int32_t)z.x;
+        if (pHintI >= kHeapLds) {
+            const uint4 v = *reinterpret_cast<const uint4*>(pgh() + bslot(pHintI));
+            pHk = __hiloint2double((int)v.y, (int)v.x);
+            pHi = (int32_t)v.z;
         }
-        hint1<0>(A); hint1<1>(A); hint1<2>(A); hint1<3>(A); hint1<4>(A);
+        hint1<0>(A); hint1<1>(A); hint1<2>(A); hint1<3>(A);
     }
     // One step of a hinted sift-up: compare with ancestor slot >> (S + 1).
     template <int S> __device__ __forceinline__ void chain_step(int slot, bool useA, double key, int& cur, bool& done)
@@ -428,9 +426,20 @@ struct BatchedPolicy {
             }
         }
     }
-    // pseudo heap
-    BPsEnt* ph; int ptail;
-    BPsEnt pRoot;
+    // pseudo heap: the window heap's layout (logical entries 1..15 in LDS,
+    // three global levels per 128-B block, 16-B (key, record) entries) over
+    // a pool of 16-B records with its own free stack
+    // (its storage follows the window heap's in each lane's arrays, at
+    // offsets that are the same for every lane)
+    int phOff, prOff, pfOff, plOff;
+    __device__ __forceinline__ BHeapEnt* pgh() const { return wh + phOff; }
+    __device__ __forceinline__ BPsRec* prr() const { return reinterpret_cast<BPsRec*>(wr + prOff); }
+    __device__ __forceinline__ uint16_t* pfr() const { return wfree + pfOff; }
+    __device__ __forceinline__ double* plk() const { return lk + plOff; }
+    __device__ __forceinline__ int* pli() const { return li + plOff; }
+    int ptail, pftop, pfresh, pcache, pcache2;
+    double pRootKey; int pRootIdx;
+    BPsRec pnrec;                    // record of the pseudo root
     uint32_t flags;
     bool dead;
     int c_push, c_pop, c_ppush, c_ppop, c_prop, c_child, c_sw, c_vw, c_fan, c_nb, c_hops, c_err;
@@ -798,8 +807,10 @@ struct BatchedPolicy {
             }
         }
         // global levels: children of nodes 8..15 root blocks 0..15, children
-        // of nodes 64..127 root blocks 16..143
-        if (deeper && sift_blocks(cur, top, 2 * cur - 16)) sift_blocks(cur, top, 16 + (2 * cur - 128));
+        // of nodes 64..127 root blocks 16..143 (read only when the node has
+        // a child: below 128 slots the lane holds blocks 0..15 only)
+        if (deeper && sift_blocks(cur, top, 2 * cur - 16) && 2 * cur < wtail)
+            sift_blocks(cur, top, 16 + (2 * cur - 128));
         hset(cur, top);
     }
     __device__ __forceinline__ void wpush(const Win& w)
@@ -835,9 +846,15 @@ struct BatchedPolicy {
             chain_step<1>(slot, useA, key, cur, done);
             chain_step<2>(slot, useA, key, cur, done);
             chain_step<3>(slot, useA, key, cur, done);
-            chain_step<4>(slot, useA, key, cur, done);
             p = cur / 2;
-            // (every global ancestor of a slot below 1024 is cached)
+            // a slot of 512 or more has one global ancestor above the kept ones
+            while (!done && p >= kHeapLds) {
+                const BHeapEnt pe = gget(p);
+                if (!(pe.key > key)) { done = true; break; }
+                if (useA) chain_set(cur, pe);
+                hset(cur, pe);
+                cur = p; p = cur / 2;
+            }
         } else {
             while (p >= kHeapLds) {
                 const BHeapEnt pe = gget(p);
@@ -861,57 +878,145 @@ struct BatchedPolicy {
         if (useB) hintA = -1;
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
-    // --- pseudo heap (inline 24-B entries, root cached in registers)
+    // --- pseudo heap: the window heap's operations on the pseudo storage
+    __device__ __forceinline__ int palloc()
+    {
+        int idx;
+        if (pcache >= 0) {
+            idx = pcache;
+            pcache = pcache2;
+            pcache2 = pftop > 0 ? (int)pfr()[--pftop] : -1;
+        } else {
+            idx = pfresh++;
+        }
+        return idx;
+    }
+    __device__ __forceinline__ void prelease(int idx)
+    {
+        if (pcache2 >= 0) pfr()[pftop++] = (uint16_t)pcache2;
+        pcache2 = pcache;
+        pcache = idx;
+    }
+    __device__ __forceinline__ double plkey(int i) const { return plk()[(i - 1) * lstride]; }
+    __device__ __forceinline__ int plidx(int i) const { return pli()[(i - 1) * lstride]; }
+    __device__ __forceinline__ BHeapEnt pgget(int i) const { return pgh()[bslot(i)]; }
+    __device__ __forceinline__ void phset(int i, const BHeapEnt& e)
+    {
+        if (i < kHeapLds) { plk()[(i - 1) * lstride] = e.key; pli()[(i - 1) * lstride] = e.idx; }
+        else pgh()[bslot(i)] = e;
+    }
+    __device__ __forceinline__ void pload_block(int blk, double (&k)[8], int (&ix)[8]) const
+    {
+        const uint4* q = reinterpret_cast<const uint4*>(pgh() + (size_t)blk * 8);
+#pragma unroll
+        for (int s = 1; s < 8; ++s) {
+            const uint4 v = q[s];
+            k[s] = __hiloint2double((int)v.y, (int)v.x);
+            ix[s] = (int)v.z;
+        }
+    }
+    // sift_blocks on the pseudo heap
+    __device__ __forceinline__ bool psift_blocks(int& cur, const BHeapEnt& top, int blkOfLeft)
+    {
+        double ka[8], kb[8];
+        int ia[8], ib[8];
+        pload_block(blkOfLeft, ka, ia);
+        pload_block(blkOfLeft + 1, kb, ib);
+        int l = 2 * cur, r = l + 1;
+        if (l >= ptail) return false;
+        bool right = r < ptail && !(ka[1] < kb[1]);
+        double ck = right ? kb[1] : ka[1];
+        int ci = right ? ib[1] : ia[1];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
+        cur = right ? r : l;
+        double x[8];
+        int xi[8];
+#pragma unroll
+        for (int s = 2; s < 8; ++s) { x[s] = right ? kb[s] : ka[s]; xi[s] = right ? ib[s] : ia[s]; }
+        l = 2 * cur; r = l + 1;
+        if (l >= ptail) return false;
+        right = r < ptail && !(x[2] < x[3]);
+        ck = right ? x[3] : x[2];
+        ci = right ? xi[3] : xi[2];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
+        cur = right ? r : l;
+        const double y0 = right ? x[6] : x[4], y1 = right ? x[7] : x[5];
+        const int j0 = right ? xi[6] : xi[4], j1 = right ? xi[7] : xi[5];
+        l = 2 * cur; r = l + 1;
+        if (l >= ptail) return false;
+        right = r < ptail && !(y0 < y1);
+        ck = right ? y1 : y0;
+        ci = right ? j1 : j0;
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
+        cur = right ? r : l;
+        return true;
+    }
     __device__ __forceinline__ bool pempty() const { return ptail == 1; }
     __device__ __forceinline__ void ptop(Pseudo& q) const
     {
-        const BPsEnt e = pRoot;
-        q.dist = e.key; q.vert = e.vert; q.pseudoSrc = e.pseudoSrc;
-        q.birth = (int)(int8_t)(e.lvlb & 0xFF); q.level = e.lvlb >> 8;
+        q.dist = pRootKey; q.vert = pnrec.vert; q.pseudoSrc = pnrec.pseudoSrc;
+        q.birth = (int)(int8_t)(pnrec.lvlb & 0xFF); q.level = pnrec.lvlb >> 8;
     }
-    // Sift-down two levels per round trip: a node's children and
-    // grandchildren (contiguous in the array) are read together.
+    // PriorityQueue.cuh:75-105 on the pseudo heap (as wpop)
     __device__ __forceinline__ void ppop()
     {
         pHintI = -1;
+        prelease(pRootIdx);
         --ptail;
-        const BPsEnt top = ph[ptail];
-        int cur = 1;
-        bool movedRoot = false;
-        auto sel = [](bool c, const BPsEnt& a, const BPsEnt& b) {
-            BPsEnt r;
-            r.key = c ? a.key : b.key; r.vert = c ? a.vert : b.vert;
-            r.pseudoSrc = c ? a.pseudoSrc : b.pseudoSrc; r.lvlb = c ? a.lvlb : b.lvlb; r.pad = 0;
-            return r;
-        };
-        for (;;) {
-            const int l = 2 * cur, gl = 2 * l;
-            if (l >= ptail) break;
-            BPsEnt c0 = ph[l], c1 = c0, g0 = c0, g1 = c0, g2 = c0, g3 = c0;
-            if (l + 1 < ptail) c1 = ph[l + 1];
-            if (gl < ptail) g0 = ph[gl];
-            if (gl + 1 < ptail) g1 = ph[gl + 1];
-            if (gl + 2 < ptail) g2 = ph[gl + 2];
-            if (gl + 3 < ptail) g3 = ph[gl + 3];
-            // the left child only if it is strictly smaller (PriorityQueue.cuh:85-88)
-            const bool right = l + 1 < ptail && !(c0.key < c1.key);
-            const BPsEnt c = sel(right, c1, c0);
-            if (!(c.key < top.key)) break;
-            ph[cur] = c;
-            if (cur == 1) { pRoot = c; movedRoot = true; }
-            cur = l + (right ? 1 : 0);
-            const int l2 = 2 * cur;
-            if (l2 >= ptail) break;
-            const BPsEnt d0 = sel(right, g2, g0), d1 = sel(right, g3, g1);
-            const bool r2 = l2 + 1 < ptail && !(d0.key < d1.key);
-            const BPsEnt d = sel(r2, d1, d0);
-            if (!(d.key < top.key)) break;
-            ph[cur] = d;
-            cur = l2 + (r2 ? 1 : 0);
+        double tk;
+        int ti;
+        {
+            const int il = ptail < kHeapLds ? ptail : kHeapLds - 1;
+            const BHeapEnt g = pgget(ptail < kHeapLds ? kHeapLds : ptail);
+            const double lkk = plkey(il);
+            const int lii = plidx(il);
+            tk = ptail < kHeapLds ? lkk : g.key;
+            ti = ptail < kHeapLds ? lii : g.idx;
         }
-        ph[cur] = top;
-        if (!movedRoot) pRoot = top;
+        BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
+        int cur = 1;
+        int mc = -1;
+        double mk = 0.0;
+        int mi = 0;
+        {
+            const int l = 2, r = 3;
+            if (r < ptail) {
+                const double kl = plkey(l), kr = plkey(r);
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? plidx(l) : plidx(r);
+            } else if (l < ptail) {
+                mc = l; mk = plkey(l); mi = plidx(l);
+            }
+        }
+        const bool rootMoves = mc != -1 && mk < top.key;
+        pRootKey = rootMoves ? mk : top.key;
+        pRootIdx = rootMoves ? mi : top.idx;
+        if (ptail > 1) pnrec = prr()[pRootIdx];
+        bool deeper = false;
+        while (mc != -1 && mk < top.key) {
+            plk()[(cur - 1) * lstride] = mk;
+            pli()[(cur - 1) * lstride] = mi;
+            cur = mc;
+            if (cur >= 8) { deeper = true; break; }
+            const int l = 2 * cur, r = l + 1;
+            if (r < ptail) {
+                const double kl = plkey(l), kr = plkey(r);
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? plidx(l) : plidx(r);
+            } else if (l < ptail) {
+                mc = l; mk = plkey(l); mi = plidx(l);
+            } else {
+                mc = -1;
+            }
+        }
+        if (deeper && psift_blocks(cur, top, 2 * cur - 16) && 2 * cur < ptail)
+            psift_blocks(cur, top, 16 + (2 * cur - 128));
+        phset(cur, top);
     }
+    // PriorityQueue.cuh:51-66 on the pseudo heap (as wpush)
     __device__ __forceinline__ void ppush(const Pseudo& q)
     {
         if (ptail - 1 >= pcap) {
@@ -919,35 +1024,48 @@ struct BatchedPolicy {
             else { flags |= SF_CAPACITY; dead = true; }
             return;
         }
+        const int idx = palloc();
+        BPsRec rec;
+        rec.vert = q.vert; rec.pseudoSrc = q.pseudoSrc;
+        rec.lvlb = (q.level << 8) | (q.birth & 0xFF); rec.pad = 0;
+        prr()[idx] = rec;
         ++ptail;
-        pmax = ptail - 1 > pmax ? ptail - 1 : pmax;
-        int cur = ptail - 1, p = cur / 2;
+        const int slot = ptail - 1;
+        pmax = slot > pmax ? slot : pmax;
+        int cur = slot, p = cur / 2;
         const double key = q.dist;
+        BHeapEnt ne; ne.key = key; ne.idx = idx; ne.pad = 0;
         const int hinted = pHintI;
         pHintI = -1;
         bool done = false;
-        // the first level from the parent read ahead (its own block of code:
-        // no choice between a register copy and a memory read)
-        if (p > 0 && p == hinted) {
+        // the first global level from the parent read ahead (its own block
+        // of code: no choice between a register copy and a memory read)
+        if (p >= kHeapLds && p == hinted) {
             if (!(pHk > key)) {
                 done = true;
             } else {
-                BPsEnt pe;
-                pe.key = pHk; pe.vert = pHv; pe.pseudoSrc = pHs; pe.lvlb = pHl; pe.pad = 0;
-                ph[cur] = pe;
+                BHeapEnt pe; pe.key = pHk; pe.idx = pHi; pe.pad = 0;
+                phset(cur, pe);
                 cur = p; p = cur / 2;
             }
         }
-        while (!done && p > 0) {
-            const BPsEnt pe = ph[p];
-            if (!(pe.key > key)) break;
-            ph[cur] = pe;
+        while (!done && p >= kHeapLds) {
+            const BHeapEnt pe = pgget(p);
+            if (!(pe.key > key)) { done = true; break; }
+            phset(cur, pe);
             cur = p; p = cur / 2;
         }
-        BPsEnt e; e.key = key; e.vert = q.vert; e.pseudoSrc = q.pseudoSrc;
-        e.lvlb = (q.level << 8) | (q.birth & 0xFF); e.pad = 0;
-        ph[cur] = e;
-        if (cur == 1) pRoot = e;
+        if (!done) {
+            while (p > 0) {
+                const double pk = plkey(p);
+                if (!(pk > key)) break;
+                BHeapEnt pe; pe.key = pk; pe.idx = plidx(p); pe.pad = 0;
+                phset(cur, pe);
+                cur = p; p = cur / 2;
+            }
+        }
+        phset(cur, ne);
+        if (cur == 1) { pRootKey = key; pRootIdx = idx; pnrec = rec; }
     }
 };
 
@@ -963,10 +1081,9 @@ struct BArgs {
     BSlotV* vtab;               // chunk * vcap
     int32_t* rlist;             // chunk * vcap
     BSlotS* stab;               // chunk * scap
-    BHeapEnt* wheap;            // lanes * kHeapSlots (blocked layout)
-    BWinRec* wrec;              // lanes * 1024
-    uint16_t* wfree;            // lanes * 1024
-    BPsEnt* pheap;              // lanes * 1024
+    BHeapEnt* wheap;            // lanes * (hslots + phslots): window heap, then pseudo heap (blocked layout)
+    BWinRec* wrec;              // lanes * (wcap + 1 + precUnits): window records, then pseudo records
+    uint16_t* wfree;            // lanes * (wcap + pcap + 2): window free stack, then pseudo free stack
     int* work;                  // work counter
     uint32_t* status;           // chunk
     int32_t* vcount;            // chunk
@@ -976,6 +1093,8 @@ struct BArgs {
     int pseudoShare;            // pseudo events run when nP * pseudoShare >= the wave's stepping lanes
     int wcap, pcap;             // window / pseudo queue storage per lane (kHeapCap: the reference's)
     int hslots;                 // window heap slots per lane (blocked layout)
+    int phslots;                // pseudo heap slots per lane, after the window heap's
+    int precUnits;              // 64-B units of pseudo records per lane, after the window records
     int lanesPerWave;           // source lanes per 64-wide wave
 };
 
@@ -1021,7 +1140,9 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
 
 __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
 {
-    extern __shared__ double ldsHeap[];       // (kHeapLds-1) keys then idx per thread
+    // (kHeapLds-1) window keys, then as many pseudo keys, per source lane;
+    // then the window and the pseudo indices
+    extern __shared__ double ldsHeap[];
     // the first lanesPerWave lanes of each wave run sources; the LDS heap
     // levels are laid out for those lanes only
     const int wl = threadIdx.x & 63;
@@ -1034,11 +1155,12 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     policy_mesh(p, a);
     p.lstride = lanesPerBlock;
     p.lk = ldsHeap + bl;
-    p.li = reinterpret_cast<int*>(ldsHeap + (kHeapLds - 1) * lanesPerBlock) + bl;
-    p.wh = a.wheap + (size_t)lane * a.hslots;
-    p.wr = a.wrec + (size_t)lane * (a.wcap + 1);
-    p.wfree = a.wfree + (size_t)lane * (a.wcap + 1);
-    p.ph = a.pheap + (size_t)lane * (a.pcap + 1);
+    p.li = reinterpret_cast<int*>(ldsHeap + 2 * (kHeapLds - 1) * lanesPerBlock) + bl;
+    p.plOff = (kHeapLds - 1) * lanesPerBlock;
+    p.wh = a.wheap + (size_t)lane * (a.hslots + a.phslots);
+    p.wr = a.wrec + (size_t)lane * (a.wcap + 1 + a.precUnits);
+    p.wfree = a.wfree + (size_t)lane * (a.wcap + a.pcap + 2);
+    p.phOff = a.hslots; p.prOff = a.wcap + 1; p.pfOff = a.wcap + 1;
     p.wcap = a.wcap; p.pcap = a.pcap;
     using I = Ich<BatchedPolicy>;
     I ich(p, 0, a.K);
@@ -1061,7 +1183,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.wcache2 = -1; p.rootKey = 0.0; p.rootIdx = 0;
             p.wmax = 0; p.pmax = 0;
             p.nrecIdx = -1;
-            p.ptail = 1;
+            p.ptail = 1; p.pftop = 0; p.pfresh = 0; p.pcache = -1; p.pcache2 = -1;
             ich.reset(src);
             ich.initialize();
             ev = ich.next_event(top, ptop);
@@ -1170,6 +1292,13 @@ static unsigned next_pow2(unsigned x)
     return p;
 }
 
+// LDS of one propagation block: heap levels 1..15 of both queues
+// (8-B key + 4-B index each) for every source lane of the block.
+static size_t lds_bytes(int block, int lanesPerWave)
+{
+    return (size_t)2 * (kHeapLds - 1) * (size_t)(block / 64) * (size_t)lanesPerWave * (sizeof(double) + sizeof(int));
+}
+
 std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig& c, DevGraph& g,
                           BuildStats& st)
 {
@@ -1247,6 +1376,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     }
     if (vcap < 64) vcap = 64;
     if (scap < 64) scap = 64;
+    // one block's LDS stays within 64 KiB
+    while (block > 64 && lds_bytes(block, lanesPerWave) > (64u << 10)) block /= 2;
     // The extraction sorts a source's reached list in LDS (at most 8192
     // entries): larger tables are capped there, and a source that reaches
     // more vertices than the capped table holds is flagged and recomputed
@@ -1260,8 +1391,10 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     }
     // heap blocks: 16 (slots below 128) or all 144
     const int hslots = (wcap < 128 ? 16 : kHeapBlocks) * 8;
-    const size_t perLane = sizeof(BHeapEnt) * (size_t)hslots + (sizeof(BWinRec) + sizeof(uint16_t)) * (size_t)(wcap + 1) +
-                           sizeof(BPsEnt) * (size_t)(pcap + 1);
+    const int phslots = (pcap < 128 ? 16 : kHeapBlocks) * 8;
+    const int precUnits = (int)((sizeof(BPsRec) * (size_t)(pcap + 1) + sizeof(BWinRec) - 1) / sizeof(BWinRec));
+    const size_t perLane = sizeof(BHeapEnt) * (size_t)(hslots + phslots) +
+                           sizeof(BWinRec) * (size_t)(wcap + 1 + precUnits) + sizeof(uint16_t) * (size_t)(wcap + pcap + 2);
     size_t perSrc = sizeof(BSlotV) * vcap + sizeof(int32_t) * vcap + sizeof(BSlotS) * scap + 64;
     int chunk = c.chunk > 0 ? c.chunk : V;
     const size_t budget = freeB > ((size_t)lanes * perLane + (4ull << 30)) ? freeB - (size_t)lanes * perLane - (4ull << 30) : 0;
@@ -1286,16 +1419,15 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     a.m = d; a.g = g; a.K = K; a.lanes = lanes; a.vcap = vcap; a.scap = scap;
     a.vrefcap = vrefcap; a.srefcap = srefcap; a.hopCap = hop_cap(h);
     a.pseudoShare = kPseudoShare;
-    a.wcap = wcap; a.pcap = pcap; a.hslots = hslots;
+    a.wcap = wcap; a.pcap = pcap; a.hslots = hslots; a.phslots = phslots; a.precUnits = precUnits;
     a.lanesPerWave = lanesPerWave;
     std::string err;
     if (hipMalloc(&a.vtab, sizeof(BSlotV) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.rlist, sizeof(int32_t) * vcap * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.stab, sizeof(BSlotS) * scap * (size_t)chunk) != hipSuccess ||
-        hipMalloc(&a.wheap, sizeof(BHeapEnt) * (size_t)hslots * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.wrec, sizeof(BWinRec) * (size_t)(wcap + 1) * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.wfree, sizeof(uint16_t) * (size_t)(wcap + 1) * (size_t)lanes) != hipSuccess ||
-        hipMalloc(&a.pheap, sizeof(BPsEnt) * (size_t)(pcap + 1) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wheap, sizeof(BHeapEnt) * (size_t)(hslots + phslots) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wrec, sizeof(BWinRec) * (size_t)(wcap + 1 + precUnits) * (size_t)lanes) != hipSuccess ||
+        hipMalloc(&a.wfree, sizeof(uint16_t) * (size_t)(wcap + pcap + 2) * (size_t)lanes) != hipSuccess ||
         hipMalloc(&a.work, sizeof(int)) != hipSuccess ||
         hipMalloc(&a.status, sizeof(uint32_t) * (size_t)chunk) != hipSuccess ||
         hipMalloc(&a.vcount, sizeof(int32_t) * (size_t)chunk) != hipSuccess ||
@@ -1332,8 +1464,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         hipEventRecord(e0, 0);
         const int pl = std::min(lanes, n);
         a.lanes = pl;
-        const size_t ldsBytes = (size_t)(kHeapLds - 1) * (size_t)(block / 64) * (size_t)a.lanesPerWave *
-                                (sizeof(double) + sizeof(int));
+        const size_t ldsBytes = lds_bytes(block, a.lanesPerWave);
         const size_t threads = (size_t)((pl + a.lanesPerWave - 1) / a.lanesPerWave) * 64;
         hipLaunchKernelGGL(batched_prop_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), ldsBytes,
                            0, a);
@@ -1398,7 +1529,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     hipFree(dOrder);
     hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
     hipFree(a.vtab); hipFree(a.rlist); hipFree(a.stab); hipFree(a.wheap); hipFree(a.wrec);
-    hipFree(a.wfree); hipFree(a.pheap); hipFree(a.work); hipFree(a.status); hipFree(a.vcount); hipFree(a.cnt);
+    hipFree(a.wfree); hipFree(a.work); hipFree(a.status); hipFree(a.vcount); hipFree(a.cnt);
     if (!err.empty()) return err;
     if (!redo.empty()) {
         // Exact recomputation on the reference-layout path (its own tables
