@@ -311,6 +311,7 @@ __device__ __forceinline__ unsigned hmix(unsigned k) { return k * 2654435761u; }
 __device__ __forceinline__ unsigned vslot(unsigned v) { return v ^ (v >> 11) ^ (v >> 22); }
 
 struct BatchedPolicy {
+    static constexpr bool kUnifiedPop = true;
     const FaceRec* __restrict__ F;
     const double* __restrict__ cA;
     const double* __restrict__ cB;
@@ -877,6 +878,135 @@ struct BatchedPolicy {
         hset(cur, ne);
         if (useB) hintA = -1;
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
+    }
+    // --- one pop for either queue.  Both queues share a layout, so lanes
+    // that pop their window queue and lanes that pop their pseudo queue run
+    // the same instructions at once; `isP` picks the queue's storage (its
+    // offsets), tail, root copies, record and free stack.
+    __device__ __forceinline__ void load_block_at(int base, double (&k)[8], int (&ix)[8]) const
+    {
+        const uint4* q = reinterpret_cast<const uint4*>(wh + base);
+#pragma unroll
+        for (int s = 1; s < 8; ++s) {
+            const uint4 v = q[s];
+            k[s] = __hiloint2double((int)v.y, (int)v.x);
+            ix[s] = (int)v.z;
+        }
+    }
+    __device__ __forceinline__ void hset_at(int i, const BHeapEnt& e, int goff, int loff)
+    {
+        if (i < kHeapLds) { lk[loff + (i - 1) * lstride] = e.key; li[loff + (i - 1) * lstride] = e.idx; }
+        else wh[goff + bslot(i)] = e;
+    }
+    __device__ __forceinline__ bool sift_blocks_at(int& cur, const BHeapEnt& top, int blkOfLeft, int goff, int loff,
+                                                   int tail)
+    {
+        double ka[8], kb[8];
+        int ia[8], ib[8];
+        load_block_at(goff + blkOfLeft * 8, ka, ia);
+        load_block_at(goff + (blkOfLeft + 1) * 8, kb, ib);
+        int l = 2 * cur, r = l + 1;
+        if (l >= tail) return false;
+        bool right = r < tail && !(ka[1] < kb[1]);
+        double ck = right ? kb[1] : ka[1];
+        int ci = right ? ib[1] : ia[1];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
+        cur = right ? r : l;
+        double x[8];
+        int xi[8];
+#pragma unroll
+        for (int s = 2; s < 8; ++s) { x[s] = right ? kb[s] : ka[s]; xi[s] = right ? ib[s] : ia[s]; }
+        l = 2 * cur; r = l + 1;
+        if (l >= tail) return false;
+        right = r < tail && !(x[2] < x[3]);
+        ck = right ? x[3] : x[2];
+        ci = right ? xi[3] : xi[2];
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
+        cur = right ? r : l;
+        const double y0 = right ? x[6] : x[4], y1 = right ? x[7] : x[5];
+        const int j0 = right ? xi[6] : xi[4], j1 = right ? xi[7] : xi[5];
+        l = 2 * cur; r = l + 1;
+        if (l >= tail) return false;
+        right = r < tail && !(y0 < y1);
+        ck = right ? y1 : y0;
+        ci = right ? j1 : j0;
+        if (!(ck < top.key)) return false;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
+        cur = right ? r : l;
+        return true;
+    }
+    // PriorityQueue.cuh:75-105 on the window queue (isP false) or the
+    // pseudo queue (isP true)
+    __device__ __forceinline__ void pop_either(bool isP)
+    {
+        if (isP) { pHintI = -1; prelease(pRootIdx); }
+        else { hintA = -1; wrelease(rootIdx); }
+        const int goff = isP ? phOff : 0;
+        const int loff = isP ? plOff : 0;
+        const int tail = (isP ? ptail : wtail) - 1;
+        if (isP) ptail = tail; else wtail = tail;
+        double tk;
+        int ti;
+        {
+            const int il = tail < kHeapLds ? tail : kHeapLds - 1;
+            const BHeapEnt g = wh[goff + bslot(tail < kHeapLds ? kHeapLds : tail)];
+            const double lkk = lk[loff + (il - 1) * lstride];
+            const int lii = li[loff + (il - 1) * lstride];
+            tk = tail < kHeapLds ? lkk : g.key;
+            ti = tail < kHeapLds ? lii : g.idx;
+        }
+        BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
+        int cur = 1;
+        int mc = -1;
+        double mk = 0.0;
+        int mi = 0;
+        {
+            const int l = 2, r = 3;
+            if (r < tail) {
+                const double kl = lk[loff + (l - 1) * lstride], kr = lk[loff + (r - 1) * lstride];
+                const int il2 = li[loff + (l - 1) * lstride], ir2 = li[loff + (r - 1) * lstride];
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? il2 : ir2;
+            } else if (l < tail) {
+                mc = l; mk = lk[loff + (l - 1) * lstride]; mi = li[loff + (l - 1) * lstride];
+            }
+        }
+        // the new root is known from the first level: its record is loaded
+        // now and arrives while the sift continues
+        const bool rootMoves = mc != -1 && mk < top.key;
+        const double nk = rootMoves ? mk : top.key;
+        const int ni = rootMoves ? mi : top.idx;
+        if (isP) {
+            pRootKey = nk; pRootIdx = ni;
+            if (tail > 1) pnrec = prr()[ni];
+        } else {
+            rootKey = nk; rootIdx = ni;
+            if (tail > 1) { nrec = wr[ni]; nrecIdx = ni; }
+            else nrecIdx = -1;
+        }
+        bool deeper = false;
+        while (mc != -1 && mk < top.key) {
+            lk[loff + (cur - 1) * lstride] = mk;
+            li[loff + (cur - 1) * lstride] = mi;
+            cur = mc;
+            if (cur >= 8) { deeper = true; break; }
+            const int l = 2 * cur, r = l + 1;
+            if (r < tail) {
+                const double kl = lk[loff + (l - 1) * lstride], kr = lk[loff + (r - 1) * lstride];
+                const int il2 = li[loff + (l - 1) * lstride], ir2 = li[loff + (r - 1) * lstride];
+                const bool left = kl < kr;
+                mc = left ? l : r; mk = left ? kl : kr; mi = left ? il2 : ir2;
+            } else if (l < tail) {
+                mc = l; mk = lk[loff + (l - 1) * lstride]; mi = li[loff + (l - 1) * lstride];
+            } else {
+                mc = -1;
+            }
+        }
+        if (deeper && sift_blocks_at(cur, top, 2 * cur - 16, goff, loff, tail) && 2 * cur < tail)
+            sift_blocks_at(cur, top, 16 + (2 * cur - 128), goff, loff, tail);
+        hset_at(cur, top, goff, loff);
     }
     // --- pseudo heap: the window heap's operations on the pseudo storage
     __device__ __forceinline__ int palloc()

@@ -529,6 +529,29 @@ struct Ich {
     SVG_HD bool do_event(int ev, const Win& top, const Pseudo& ptop)
     {
         const int nf = p.nf();
+        if constexpr (P::kUnifiedPop) {
+            // one pop for both queues (the policy's queues share a layout),
+            // so lanes popping either queue run it together
+            if (ev < EV_STALE_W || ev > EV_PSEUDO) return false;
+            const bool isP = ev == EV_STALE_P || ev == EV_PSEUDO;
+            const bool prop = ev == EV_WIN && !(top.level > nf) && top.twin != -1;
+            Tri t;
+            if (prop) p.face3(top.twin, t);
+            p.pop_either(isP);
+            if (isP) p.count(CNT_PSEUDO_POP, 1);
+            else p.count(CNT_WIN_POP, 1);
+            if (ev == EV_WIN) {
+                if (top.level > nf) return true;
+                propagate(top, prop, t);
+            } else if (ev == EV_PSEUDO) {
+                if (ptop.level >= nf) return true;
+                gen_sub_wins(ptop);
+            } else {
+                return true;
+            }
+            if (K != -1 && total >= K) return false;
+            return !p.stop();
+        }
         if (ev == EV_STALE_W || ev == EV_WIN) {
             // the far face of a window that will propagate is read before the
             // pop, so the read overlaps the heap's sift-down
