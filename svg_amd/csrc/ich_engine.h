@@ -426,31 +426,36 @@ struct Ich {
             if (startEdge == -1 || ++guard > 4096) { p.count(CNT_ERRORS, 1); break; }
         } while (startEdge != endEdge);
 
+        // The mesh data of the next fan edge is read while this neighbour is
+        // looked up (the mesh never changes, so the order of the table
+        // reads and writes is the reference's).
         const int start = p.first_edge(pw.vert);
         int cur = start;
+        int opVert = p.v1(cur);
+        double len = p.len(cur);
+        int tw = p.twin(cur);
         do {
-            const int opVert = p.v1(cur);
+            const int nxt = tw != -1 ? p.next(tw) : -1;
+            int opVert2 = 0, tw2 = -1;
+            double len2 = 0.0;
+            if (nxt != -1 && nxt != start) { opVert2 = p.v1(nxt); len2 = p.len(nxt); tw2 = p.twin(nxt); }
             p.count(CNT_PSEUDO_NEIGH, 1);
             VInfo ov = p.vget(opVert);
-            if ((p.vflag(opVert) & 1) || ov.dist < pw.dist + p.len(cur)) {
-                cur = p.twin(cur);
-                if (cur != -1) cur = p.next(cur);
-                continue;
+            if (!((p.vflag(opVert) & 1) || ov.dist < pw.dist + len)) {
+                if (ov.dist == DBL_MAX) ++total;
+                ov.dist = pw.dist + len;
+                ov.birth = (int8_t)(ov.birth + 1);
+                ov.enter = tw;
+                p.vput(opVert, ov);
+                p.count(CNT_VERT_WRITE, 1);
+                Pseudo c;
+                c.vert = opVert; c.dist = ov.dist;
+                c.pseudoSrc = pw.vert;
+                c.birth = ov.birth;
+                c.level = pw.level;
+                push_pseudo(c);
             }
-            if (ov.dist == DBL_MAX) ++total;
-            ov.dist = pw.dist + p.len(cur);
-            ov.birth = (int8_t)(ov.birth + 1);
-            ov.enter = p.twin(cur);
-            p.vput(opVert, ov);
-            p.count(CNT_VERT_WRITE, 1);
-            Pseudo c;
-            c.vert = opVert; c.dist = ov.dist;
-            c.pseudoSrc = pw.vert;
-            c.birth = ov.birth;
-            c.level = pw.level;
-            push_pseudo(c);
-            cur = p.twin(cur);
-            if (cur != -1) cur = p.next(cur);
+            cur = nxt; opVert = opVert2; len = len2; tw = tw2;
         } while (cur != start && cur != -1);
     }
 
