@@ -304,11 +304,15 @@ __device__ __forceinline__ int bslot(int i)           // i >= 16
     return block * 8 + ((1 << r) | (i & ((1 << r) - 1)));
 }
 
-__device__ __forceinline__ unsigned hmix(unsigned k) { return k * 2654435761u; }
-// Vertex-table slot: XOR-folding the high bits into the low ones keeps every
-// aligned group of eight consecutive ids inside one 128-B line of slots, so
-// the vertices of one face and of neighbouring faces share lines.
-__device__ __forceinline__ unsigned vslot(unsigned v) { return v ^ (v >> 11) ^ (v >> 22); }
+// Multiplicative hashing (Knuth): the slot comes from bits 11 and up of
+// k * 2654435761.  (The low bits of the product are a permutation of the
+// low bits of k, so runs of ids, as a mesh region's vertices and edges
+// are, would land on runs of slots and merge into long probe clusters.)
+__device__ __forceinline__ unsigned hmix(unsigned k) { return (k * 2654435761u) >> 11; }
+// Vertex-table slot: each aligned group of eight consecutive ids keeps one
+// 128-B line of slots (vertices of one face and of neighbouring faces share
+// lines), and the groups are scattered by the hash.
+__device__ __forceinline__ unsigned vslot(unsigned v) { return (hmix(v >> 3) << 3) | (v & 7u); }
 
 struct BatchedPolicy {
     static constexpr bool kUnifiedPop = true;
