@@ -315,7 +315,6 @@ __device__ __forceinline__ unsigned hmix(unsigned k) { return (k * 2654435761u) 
 __device__ __forceinline__ unsigned vslot(unsigned v) { return (hmix(v >> 3) << 3) | (v & 7u); }
 
 struct BatchedPolicy {
-    static constexpr bool kUnifiedPop = true;
     const FaceRec* __restrict__ F;
     const double* __restrict__ cA;
     const double* __restrict__ cB;
@@ -334,6 +333,10 @@ struct BatchedPolicy {
     int wtail, wftop, wfresh, wcache, wcache2;
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
+    // The last entry of each heap (slot wtail - 1, ptail - 1) in registers:
+    // a pop starts from it without reading memory.  Every push and pop keeps
+    // it current.
+    double wLk, pLk; int wLi, pLi;
 
     __device__ __forceinline__ void hset(int i, const BHeapEnt& e)
     {
@@ -419,6 +422,7 @@ struct BatchedPolicy {
                 done = true;
             } else {
                 hset(cur, pe);
+                if (cur == slot) { wLk = pe.key; wLi = pe.idx; }
                 // cur is the slot itself (S = 0) or ancestor S - 1
                 if constexpr (S > 0) {
                     if (useA) {
@@ -571,15 +575,59 @@ struct BatchedPolicy {
             h = (h + 1) & smask;
         }
     }
-    __device__ __forceinline__ void lookup_face(const Tri& t, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp)
+    // The table lookups of a propagation (its far face's three vertices and
+    // the split record of the edge it enters) take their first probe before
+    // the window is popped: the pop never touches the tables, so the probes
+    // travel with the pop's own reads.  lookup_issue reads the four slots
+    // (the face must be known), lookup_finish resolves them after the pop,
+    // continuing a probe past a slot held by another key.
+    struct LookPre {
+        int ka, kb, ko; uint32_t ea, eb, eo; double da, db, dd;
+        int ks, ps; double sd, sx;
+        unsigned ha, hb, ho, hs;
+    };
+    __device__ __forceinline__ void ld_slot(unsigned h, int& k, uint32_t& e, double& d) const
     {
-        unsigned ha = vslot((unsigned)t.a) & vmask;
-        unsigned hb = vslot((unsigned)t.b) & vmask;
-        unsigned ho = vslot((unsigned)t.op) & vmask;
-        unsigned hs = hmix((unsigned)t.e0) & smask;
-        const BSlotV sa = vt[ha], sb = vt[hb], so = vt[ho];
+        const uint4 v = *reinterpret_cast<const uint4*>(vt + h);
+        k = (int)v.x; e = v.y; d = __hiloint2double((int)v.w, (int)v.z);
+    }
+    __device__ __forceinline__ void lookup_issue(const Tri& t, LookPre& q) const
+    {
+        q.ha = vslot((unsigned)t.a) & vmask;
+        q.hb = vslot((unsigned)t.b) & vmask;
+        q.ho = vslot((unsigned)t.op) & vmask;
+        q.hs = hmix((unsigned)t.e0) & smask;
+        ld_slot(q.ha, q.ka, q.ea, q.da); ld_slot(q.hb, q.kb, q.eb, q.db); ld_slot(q.ho, q.ko, q.eo, q.dd);
+        {
+            const uint4 v = *reinterpret_cast<const uint4*>(st + q.hs);
+            const double2 w = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(st + q.hs) + 16);
+            q.ks = (int)v.x; q.ps = (int)v.y; q.sd = __hiloint2double((int)v.w, (int)v.z); q.sx = w.x;
+        }
+    }
+    __device__ __forceinline__ SInfo sresolve_from(BSlotS x, unsigned& h, int e, bool& found) const
+    {
+        found = false;
+        for (;;) {
+            if (x.key == e) {
+                found = true;
+                SInfo r; r.dist = x.dist; r.x = x.x; r.ps = x.ps;
+                return r;
+            }
+            if (x.key == -1) return sinfo_default();
+            h = (h + 1) & smask;
+            x = st[h];
+        }
+    }
+    __device__ __forceinline__ void lookup_finish(const Tri& t, LookPre& q, VInfo& va, VInfo& vb, VInfo& vo, SInfo& sp)
+    {
         bool fa, fb, fo, fs;
-        sp = sresolve(hs, t.e0, fs);
+        unsigned ha = q.ha, hb = q.hb, ho = q.ho, hs = q.hs;
+        BSlotS ss; ss.key = q.ks; ss.ps = q.ps; ss.dist = q.sd; ss.x = q.sx; ss.pad = 0.0;
+        sp = sresolve_from(ss, hs, t.e0, fs);
+        BSlotV sa, sb, so;
+        sa.key = q.ka; sa.eb = q.ea; sa.dist = q.da;
+        sb.key = q.kb; sb.eb = q.eb; sb.dist = q.db;
+        so.key = q.ko; so.eb = q.eo; so.dist = q.dd;
         va = vresolve(sa, ha, t.a, fa);
         vb = vresolve(sb, hb, t.b, fb);
         vo = vresolve(so, ho, t.op, fo);
@@ -680,7 +728,7 @@ struct BatchedPolicy {
     __device__ __forceinline__ bool wempty() const { return wtail == 1; }
     __device__ __forceinline__ void wtop(Win& w) const
     {
-        // Every change of the root refreshes nrec (wpop loads it, wpush
+        // Every change of the root refreshes nrec (pop_either loads it, wpush
         // copies the new record), so the root's record is always at hand.
         const BWinRec& r = nrec;
         w.b0 = r.b0; w.b1 = r.b1; w.ps = r.ps; w.sx = r.sx; w.sy = r.sy;
@@ -690,134 +738,12 @@ struct BatchedPolicy {
         w.birth = (int)(int8_t)(r.lvlb & 0xFF);
         w.level = r.lvlb >> 8;
     }
-    // One 3-level block of the global heap part (slots 1..7 of its 8), read
-    // with seven independent 16-B loads.
-    __device__ __forceinline__ void load_block(int blk, double (&k)[8], int (&ix)[8]) const
-    {
-        const uint4* q = reinterpret_cast<const uint4*>(wh + (size_t)blk * 8);
-#pragma unroll
-        for (int s = 1; s < 8; ++s) {
-            const uint4 v = q[s];
-            k[s] = __hiloint2double((int)v.y, (int)v.x);
-            ix[s] = (int)v.z;
-        }
-    }
-    // Sift-down through the three global levels below `cur`, whose children
-    // are the roots of two adjacent blocks: both blocks are read at once, so
-    // the three levels cost one round trip to memory instead of three.
-    // Returns true when the sift continues below the third level.
-    __device__ __forceinline__ bool sift_blocks(int& cur, const BHeapEnt& top, int blkOfLeft)
-    {
-        double ka[8], kb[8];
-        int ia[8], ib[8];
-        load_block(blkOfLeft, ka, ia);
-        load_block(blkOfLeft + 1, kb, ib);
-        // level 1 of the blocks: the two roots
-        int l = 2 * cur, r = l + 1;
-        if (l >= wtail) return false;
-        bool right = r < wtail && !(ka[1] < kb[1]);
-        double ck = right ? kb[1] : ka[1];
-        int ci = right ? ib[1] : ia[1];
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
-        cur = right ? r : l;
-        // the chosen block's entries, by select (no dynamic register index)
-        double x[8];
-        int xi[8];
-#pragma unroll
-        for (int s = 2; s < 8; ++s) { x[s] = right ? kb[s] : ka[s]; xi[s] = right ? ib[s] : ia[s]; }
-        // level 2: slots 2, 3
-        l = 2 * cur; r = l + 1;
-        if (l >= wtail) return false;
-        right = r < wtail && !(x[2] < x[3]);
-        ck = right ? x[3] : x[2];
-        ci = right ? xi[3] : xi[2];
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
-        cur = right ? r : l;
-        // level 3: slots 4, 5 (under slot 2) or 6, 7 (under slot 3)
-        const double y0 = right ? x[6] : x[4], y1 = right ? x[7] : x[5];
-        const int j0 = right ? xi[6] : xi[4], j1 = right ? xi[7] : xi[5];
-        l = 2 * cur; r = l + 1;
-        if (l >= wtail) return false;
-        right = r < wtail && !(y0 < y1);
-        ck = right ? y1 : y0;
-        ci = right ? j1 : j0;
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset(cur, m); }
-        cur = right ? r : l;
-        return true;
-    }
     // LDS-only and global-only reads of heap entries: with one accessor
     // for both, the compiler merges the two paths into generic (flat)
     // loads, whose waits also wait for every global load in flight.
     __device__ __forceinline__ double lkey(int i) const { return lk[(i - 1) * lstride]; }
     __device__ __forceinline__ int lidx(int i) const { return li[(i - 1) * lstride]; }
     __device__ __forceinline__ BHeapEnt gget(int i) const { return wh[bslot(i)]; }
-    __device__ __forceinline__ void wpop()
-    {
-        hintA = -1;
-        wrelease(rootIdx);
-        --wtail;
-        // the last entry: read from both parts, kept from the one it is in
-        double tk;
-        int ti;
-        {
-            const int il = wtail < kHeapLds ? wtail : kHeapLds - 1;
-            const BHeapEnt g = gget(wtail < kHeapLds ? kHeapLds : wtail);
-            const double lkk = lkey(il);
-            const int lii = lidx(il);
-            tk = wtail < kHeapLds ? lkk : g.key;
-            ti = wtail < kHeapLds ? lii : g.idx;
-        }
-        BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
-        // levels in LDS: children of nodes 1..7 are entries 2..15
-        int cur = 1;
-        int mc = -1;
-        double mk = 0.0;
-        int mi = 0;
-        {
-            const int l = 2, r = 3;
-            if (r < wtail) {
-                const double kl = lkey(l), kr = lkey(r);
-                const bool left = kl < kr;
-                mc = left ? l : r; mk = left ? kl : kr; mi = left ? lidx(l) : lidx(r);
-            } else if (l < wtail) {
-                mc = l; mk = lkey(l); mi = lidx(l);
-            }
-        }
-        // The new root is known from the first level: its record is loaded
-        // now and arrives while the sift continues (the sift writes only
-        // heap entries, never records).
-        const bool rootMoves = mc != -1 && mk < top.key;
-        rootKey = rootMoves ? mk : top.key;
-        rootIdx = rootMoves ? mi : top.idx;
-        if (wtail > 1) { nrec = wr[rootIdx]; nrecIdx = rootIdx; }
-        else nrecIdx = -1;
-        bool deeper = false;
-        while (mc != -1 && mk < top.key) {
-            lk[(cur - 1) * lstride] = mk;
-            li[(cur - 1) * lstride] = mi;
-            cur = mc;
-            if (cur >= 8) { deeper = true; break; }
-            const int l = 2 * cur, r = l + 1;
-            if (r < wtail) {
-                const double kl = lkey(l), kr = lkey(r);
-                const bool left = kl < kr;
-                mc = left ? l : r; mk = left ? kl : kr; mi = left ? lidx(l) : lidx(r);
-            } else if (l < wtail) {
-                mc = l; mk = lkey(l); mi = lidx(l);
-            } else {
-                mc = -1;
-            }
-        }
-        // global levels: children of nodes 8..15 root blocks 0..15, children
-        // of nodes 64..127 root blocks 16..143 (read only when the node has
-        // a child: below 128 slots the lane holds blocks 0..15 only)
-        if (deeper && sift_blocks(cur, top, 2 * cur - 16) && 2 * cur < wtail)
-            sift_blocks(cur, top, 16 + (2 * cur - 128));
-        hset(cur, top);
-    }
     __device__ __forceinline__ void wpush(const Win& w)
     {
         if (wtail - 1 >= wcap) {
@@ -858,6 +784,7 @@ struct BatchedPolicy {
                 if (!(pe.key > key)) { done = true; break; }
                 if (useA) chain_set(cur, pe);
                 hset(cur, pe);
+                if (cur == slot) { wLk = pe.key; wLi = pe.idx; }
                 cur = p; p = cur / 2;
             }
         } else {
@@ -865,6 +792,7 @@ struct BatchedPolicy {
                 const BHeapEnt pe = gget(p);
                 if (!(pe.key > key)) { done = true; break; }
                 hset(cur, pe);
+                if (cur == slot) { wLk = pe.key; wLi = pe.idx; }
                 cur = p; p = cur / 2;
             }
         }
@@ -875,11 +803,13 @@ struct BatchedPolicy {
                 BHeapEnt pe; pe.key = pk; pe.idx = lidx(p); pe.pad = 0;
                 if (useA && cur >= kHeapLds) chain_set(cur, pe);
                 hset(cur, pe);
+                if (cur == slot) { wLk = pe.key; wLi = pe.idx; }
                 cur = p; p = cur / 2;
             }
         }
         if (useA && cur >= kHeapLds) chain_set(cur, ne);
         hset(cur, ne);
+        if (cur == slot) { wLk = key; wLi = idx; }
         if (useB) hintA = -1;
         if (cur == 1) { rootKey = key; rootIdx = idx; nrec = rec; nrecIdx = idx; }
     }
@@ -951,15 +881,23 @@ struct BatchedPolicy {
         const int loff = isP ? plOff : 0;
         const int tail = (isP ? ptail : wtail) - 1;
         if (isP) ptail = tail; else wtail = tail;
-        double tk;
-        int ti;
+        const double tk = isP ? pLk : wLk;
+        const int ti = isP ? pLi : wLi;
+        // The popped heap's last entry (slot `tail`) comes from its register
+        // copy.  The entry at slot tail - 1 becomes the last one: it has no
+        // children below the new tail, so the sift-down either ends there
+        // (and leaves `top` in it) or does not touch it.  Its value is read
+        // now, with the pop's other reads, and chosen after the sift.
+        const int nl = tail - 1;
+        double nlk;
+        int nli;
         {
-            const int il = tail < kHeapLds ? tail : kHeapLds - 1;
-            const BHeapEnt g = wh[goff + bslot(tail < kHeapLds ? kHeapLds : tail)];
+            const int il = nl < kHeapLds ? (nl > 0 ? nl : 1) : kHeapLds - 1;
+            const BHeapEnt g = wh[goff + bslot(nl < kHeapLds ? kHeapLds : nl)];
             const double lkk = lk[loff + (il - 1) * lstride];
             const int lii = li[loff + (il - 1) * lstride];
-            tk = tail < kHeapLds ? lkk : g.key;
-            ti = tail < kHeapLds ? lii : g.idx;
+            nlk = nl < kHeapLds ? lkk : g.key;
+            nli = nl < kHeapLds ? lii : g.idx;
         }
         BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
         int cur = 1;
@@ -1011,6 +949,9 @@ struct BatchedPolicy {
         if (deeper && sift_blocks_at(cur, top, 2 * cur - 16, goff, loff, tail) && 2 * cur < tail)
             sift_blocks_at(cur, top, 16 + (2 * cur - 128), goff, loff, tail);
         hset_at(cur, top, goff, loff);
+        const double nk2 = cur == nl ? top.key : nlk;
+        const int ni2 = cur == nl ? top.idx : nli;
+        if (isP) { pLk = nk2; pLi = ni2; } else { wLk = nk2; wLi = ni2; }
     }
     // --- pseudo heap: the window heap's operations on the pseudo storage
     __device__ __forceinline__ int palloc()
@@ -1039,118 +980,13 @@ struct BatchedPolicy {
         if (i < kHeapLds) { plk()[(i - 1) * lstride] = e.key; pli()[(i - 1) * lstride] = e.idx; }
         else pgh()[bslot(i)] = e;
     }
-    __device__ __forceinline__ void pload_block(int blk, double (&k)[8], int (&ix)[8]) const
-    {
-        const uint4* q = reinterpret_cast<const uint4*>(pgh() + (size_t)blk * 8);
-#pragma unroll
-        for (int s = 1; s < 8; ++s) {
-            const uint4 v = q[s];
-            k[s] = __hiloint2double((int)v.y, (int)v.x);
-            ix[s] = (int)v.z;
-        }
-    }
-    // sift_blocks on the pseudo heap
-    __device__ __forceinline__ bool psift_blocks(int& cur, const BHeapEnt& top, int blkOfLeft)
-    {
-        double ka[8], kb[8];
-        int ia[8], ib[8];
-        pload_block(blkOfLeft, ka, ia);
-        pload_block(blkOfLeft + 1, kb, ib);
-        int l = 2 * cur, r = l + 1;
-        if (l >= ptail) return false;
-        bool right = r < ptail && !(ka[1] < kb[1]);
-        double ck = right ? kb[1] : ka[1];
-        int ci = right ? ib[1] : ia[1];
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
-        cur = right ? r : l;
-        double x[8];
-        int xi[8];
-#pragma unroll
-        for (int s = 2; s < 8; ++s) { x[s] = right ? kb[s] : ka[s]; xi[s] = right ? ib[s] : ia[s]; }
-        l = 2 * cur; r = l + 1;
-        if (l >= ptail) return false;
-        right = r < ptail && !(x[2] < x[3]);
-        ck = right ? x[3] : x[2];
-        ci = right ? xi[3] : xi[2];
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
-        cur = right ? r : l;
-        const double y0 = right ? x[6] : x[4], y1 = right ? x[7] : x[5];
-        const int j0 = right ? xi[6] : xi[4], j1 = right ? xi[7] : xi[5];
-        l = 2 * cur; r = l + 1;
-        if (l >= ptail) return false;
-        right = r < ptail && !(y0 < y1);
-        ck = right ? y1 : y0;
-        ci = right ? j1 : j0;
-        if (!(ck < top.key)) return false;
-        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; phset(cur, m); }
-        cur = right ? r : l;
-        return true;
-    }
     __device__ __forceinline__ bool pempty() const { return ptail == 1; }
     __device__ __forceinline__ void ptop(Pseudo& q) const
     {
         q.dist = pRootKey; q.vert = pnrec.vert; q.pseudoSrc = pnrec.pseudoSrc;
         q.birth = (int)(int8_t)(pnrec.lvlb & 0xFF); q.level = pnrec.lvlb >> 8;
     }
-    // PriorityQueue.cuh:75-105 on the pseudo heap (as wpop)
-    __device__ __forceinline__ void ppop()
-    {
-        pHintI = -1;
-        prelease(pRootIdx);
-        --ptail;
-        double tk;
-        int ti;
-        {
-            const int il = ptail < kHeapLds ? ptail : kHeapLds - 1;
-            const BHeapEnt g = pgget(ptail < kHeapLds ? kHeapLds : ptail);
-            const double lkk = plkey(il);
-            const int lii = plidx(il);
-            tk = ptail < kHeapLds ? lkk : g.key;
-            ti = ptail < kHeapLds ? lii : g.idx;
-        }
-        BHeapEnt top; top.key = tk; top.idx = ti; top.pad = 0;
-        int cur = 1;
-        int mc = -1;
-        double mk = 0.0;
-        int mi = 0;
-        {
-            const int l = 2, r = 3;
-            if (r < ptail) {
-                const double kl = plkey(l), kr = plkey(r);
-                const bool left = kl < kr;
-                mc = left ? l : r; mk = left ? kl : kr; mi = left ? plidx(l) : plidx(r);
-            } else if (l < ptail) {
-                mc = l; mk = plkey(l); mi = plidx(l);
-            }
-        }
-        const bool rootMoves = mc != -1 && mk < top.key;
-        pRootKey = rootMoves ? mk : top.key;
-        pRootIdx = rootMoves ? mi : top.idx;
-        if (ptail > 1) pnrec = prr()[pRootIdx];
-        bool deeper = false;
-        while (mc != -1 && mk < top.key) {
-            plk()[(cur - 1) * lstride] = mk;
-            pli()[(cur - 1) * lstride] = mi;
-            cur = mc;
-            if (cur >= 8) { deeper = true; break; }
-            const int l = 2 * cur, r = l + 1;
-            if (r < ptail) {
-                const double kl = plkey(l), kr = plkey(r);
-                const bool left = kl < kr;
-                mc = left ? l : r; mk = left ? kl : kr; mi = left ? plidx(l) : plidx(r);
-            } else if (l < ptail) {
-                mc = l; mk = plkey(l); mi = plidx(l);
-            } else {
-                mc = -1;
-            }
-        }
-        if (deeper && psift_blocks(cur, top, 2 * cur - 16) && 2 * cur < ptail)
-            psift_blocks(cur, top, 16 + (2 * cur - 128));
-        phset(cur, top);
-    }
-    // PriorityQueue.cuh:51-66 on the pseudo heap (as wpush)
+    // PriorityQueue.cuh:51-66 on the pseudo heap (as wpush); pops go through pop_either
     __device__ __forceinline__ void ppush(const Pseudo& q)
     {
         if (ptail - 1 >= pcap) {
@@ -1180,6 +1016,7 @@ struct BatchedPolicy {
             } else {
                 BHeapEnt pe; pe.key = pHk; pe.idx = pHi; pe.pad = 0;
                 phset(cur, pe);
+                pLk = pe.key; pLi = pe.idx;          // cur == slot here
                 cur = p; p = cur / 2;
             }
         }
@@ -1187,6 +1024,7 @@ struct BatchedPolicy {
             const BHeapEnt pe = pgget(p);
             if (!(pe.key > key)) { done = true; break; }
             phset(cur, pe);
+            if (cur == slot) { pLk = pe.key; pLi = pe.idx; }
             cur = p; p = cur / 2;
         }
         if (!done) {
@@ -1195,10 +1033,12 @@ struct BatchedPolicy {
                 if (!(pk > key)) break;
                 BHeapEnt pe; pe.key = pk; pe.idx = plidx(p); pe.pad = 0;
                 phset(cur, pe);
+                if (cur == slot) { pLk = pe.key; pLi = pe.idx; }
                 cur = p; p = cur / 2;
             }
         }
         phset(cur, ne);
+        if (cur == slot) { pLk = key; pLi = idx; }
         if (cur == 1) { pRootKey = key; pRootIdx = idx; pnrec = rec; }
     }
 };

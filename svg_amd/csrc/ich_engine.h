@@ -26,6 +26,11 @@
 //   heaps:  bool wempty(); void wtop(Win&); void wpop(); void wpush(const Win&);
 //           bool pempty(); void ptop(Pseudo&); void ppop(); void ppush(const Pseudo&)
 //   counters: void count(int which, long long n)
+// The stepped form (next_event / do_event) instead pops through
+// void pop_either(bool pseudo) and takes the propagation's lookups in two
+// halves: a type LookPre, void lookup_issue(Tri, LookPre&) before the pop and
+// void lookup_finish(Tri, LookPre&, VInfo& a, VInfo& b, VInfo& op, SInfo&)
+// after it; void push_hint() is called before a propagation's pushes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -250,14 +255,31 @@ struct Ich {
     {
         const int e0 = win.twin;
         if (e0 == -1) return;
-        p.count(CNT_PROPAGATE, 1);
-        p.push_hint();
         Tri t;
         if (faceRead) t = face;
         else p.face3(e0, t);
         VInfo va, vb, vo;
         SInfo sp;
+        p.push_hint();
         p.lookup_face(t, va, vb, vo, sp);
+        propagate_body(win, t, va, vb, vo, sp);
+    }
+    // the table lookups of the far face were issued before the pop
+    template <class Pre>
+    SVG_HD void propagate_pre(const Win& win, const Tri& t, Pre& pre)
+    {
+        VInfo va, vb, vo;
+        SInfo sp;
+        p.push_hint();
+        p.lookup_finish(t, pre, va, vb, vo, sp);
+        propagate_body(win, t, va, vb, vo, sp);
+    }
+    SVG_HD void propagate_body(const Win& win, const Tri& t, const VInfo& va, const VInfo& vb, const VInfo& vo0,
+                               const SInfo& sp)
+    {
+        const int e0 = win.twin;
+        p.count(CNT_PROPAGATE, 1);
+        VInfo vo = vo0;
         const double l0 = t.l0, l1 = t.l1, l2 = t.l2;
         const double sx = win.sx, sy = win.sy;
         const double leftx = win.b0, rightx = win.b1;
@@ -529,51 +551,31 @@ struct Ich {
         return haveTop || havePtop ? EV_STUCK : EV_DONE;
     }
     // Performs `ev`; false once the source is finished (K reached or stopped).
-    // A stale pop and a main pop of the same heap share one pop call site,
-    // so lanes of a wave doing either run it together.
+    // The policy pops either queue through one call (pop_either: the queues
+    // share a layout), so lanes of a wave popping a window, a pseudo-window
+    // or a stale entry of either run the pop together.  The far face of a
+    // window that will propagate is read, and its table lookups issued
+    // (lookup_issue), before the pop, so those reads overlap the sift-down.
     SVG_HD bool do_event(int ev, const Win& top, const Pseudo& ptop)
     {
         const int nf = p.nf();
-        if constexpr (P::kUnifiedPop) {
-            // one pop for both queues (the policy's queues share a layout),
-            // so lanes popping either queue run it together
-            if (ev < EV_STALE_W || ev > EV_PSEUDO) return false;
-            const bool isP = ev == EV_STALE_P || ev == EV_PSEUDO;
-            const bool prop = ev == EV_WIN && !(top.level > nf) && top.twin != -1;
-            Tri t;
-            if (prop) p.face3(top.twin, t);
-            p.pop_either(isP);
-            if (isP) p.count(CNT_PSEUDO_POP, 1);
-            else p.count(CNT_WIN_POP, 1);
-            if (ev == EV_WIN) {
-                if (top.level > nf) return true;
-                propagate(top, prop, t);
-            } else if (ev == EV_PSEUDO) {
-                if (ptop.level >= nf) return true;
-                gen_sub_wins(ptop);
-            } else {
-                return true;
-            }
-            if (K != -1 && total >= K) return false;
-            return !p.stop();
-        }
-        if (ev == EV_STALE_W || ev == EV_WIN) {
-            // the far face of a window that will propagate is read before the
-            // pop, so the read overlaps the heap's sift-down
-            const bool prop = ev == EV_WIN && !(top.level > nf) && top.twin != -1;
-            Tri t;
-            if (prop) p.face3(top.twin, t);
-            p.wpop();
-            p.count(CNT_WIN_POP, 1);
-            if (ev == EV_STALE_W || top.level > nf) return true;
-            propagate(top, prop, t);
-        } else if (ev == EV_STALE_P || ev == EV_PSEUDO) {
-            p.ppop();
-            p.count(CNT_PSEUDO_POP, 1);
-            if (ev == EV_STALE_P || ptop.level >= nf) return true;
+        if (ev < EV_STALE_W || ev > EV_PSEUDO) return false;
+        const bool isP = ev == EV_STALE_P || ev == EV_PSEUDO;
+        const bool prop = ev == EV_WIN && !(top.level > nf) && top.twin != -1;
+        Tri t;
+        typename P::LookPre pre;
+        if (prop) { p.face3(top.twin, t); p.lookup_issue(t, pre); }
+        p.pop_either(isP);
+        if (isP) p.count(CNT_PSEUDO_POP, 1);
+        else p.count(CNT_WIN_POP, 1);
+        if (ev == EV_WIN) {
+            if (!prop) return true;
+            propagate_pre(top, t, pre);
+        } else if (ev == EV_PSEUDO) {
+            if (ptop.level >= nf) return true;
             gen_sub_wins(ptop);
         } else {
-            return false;
+            return true;
         }
         if (K != -1 && total >= K) return false;
         return !p.stop();

@@ -42,7 +42,6 @@ struct PVertItem {
 };
 
 struct PlainPolicy {
-    static constexpr bool kUnifiedPop = false;   // the two queues are popped by their own code
     const EdgeAoS* __restrict__ E;
     const VertAoS* __restrict__ Vt;
     const double* __restrict__ cA;
