@@ -75,6 +75,42 @@ std::string upload_mesh(const HostMesh& h, DevMesh& d, bool aos)
         HIP_TRY(hipMemcpy(d.firstEdge, h.firstEdge.data(), sizeof(int32_t) * (size_t)h.nv, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.flags, h.flags.data(), (size_t)h.nv, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.pos, pos.data(), sizeof(double) * 4 * (size_t)h.nv, hipMemcpyHostToDevice));
+        // fans (only when every half-edge lies on the fan its origin's
+        // firstEdge starts: a manifold mesh)
+        std::vector<FanRec> fan((size_t)h.ne);
+        std::vector<int32_t> fidx((size_t)h.ne, -1), fbase((size_t)h.nv + 1, 0);
+        bool manifold = true;
+        int fp = 0;
+        auto nxt = [](int e) { return (e % 3 == 2) ? e - 2 : e + 1; };
+        auto prv = [](int e) { return (e % 3 == 0) ? e + 2 : e - 1; };
+        for (int v = 0; v < h.nv && manifold; ++v) {
+            fbase[v] = fp;
+            const int start = h.firstEdge[v];
+            int e = start;
+            while (e != -1) {
+                if (fp >= h.ne || fidx[e] != -1 || h.tri[e] != v) { manifold = false; break; }
+                fidx[e] = fp;
+                FanRec& r = fan[fp++];
+                const int tw = h.twin[e];
+                r.cA = h.cornerA[e];
+                r.cB = tw != -1 ? h.cornerB[tw] : 0.0;
+                r.lo = h.len[e]; r.lnx = h.len[nxt(e)]; r.lpv = h.len[prv(e)];
+                r.o = e; r.twn = h.twin[nxt(e)];
+                r.nb = h.tri[nxt(e)];
+                r.tw = tw; r.nbFlag = h.flags[r.nb]; r.pad = 0;
+                e = tw != -1 ? nxt(tw) : -1;
+                if (e == start) break;
+            }
+        }
+        fbase[h.nv] = fp;
+        if (manifold && fp == h.ne) {
+            HIP_TRY(hipMalloc(&d.fan, sizeof(FanRec) * (size_t)h.ne));
+            HIP_TRY(hipMalloc(&d.fanIdx, sizeof(int32_t) * (size_t)h.ne));
+            HIP_TRY(hipMalloc(&d.fanBase, sizeof(int32_t) * ((size_t)h.nv + 1)));
+            HIP_TRY(hipMemcpy(d.fan, fan.data(), sizeof(FanRec) * (size_t)h.ne, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d.fanIdx, fidx.data(), sizeof(int32_t) * (size_t)h.ne, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d.fanBase, fbase.data(), sizeof(int32_t) * ((size_t)h.nv + 1), hipMemcpyHostToDevice));
+        }
     }
     if (aos && !d.edges) {
         std::vector<EdgeAoS> ea((size_t)h.ne);
@@ -108,6 +144,7 @@ void free_dev_mesh(DevMesh& d)
 {
     hipFree(d.face); hipFree(d.cornerA); hipFree(d.cornerB); hipFree(d.firstEdge);
     hipFree(d.flags); hipFree(d.pos); hipFree(d.edges); hipFree(d.verts);
+    hipFree(d.fan); hipFree(d.fanIdx); hipFree(d.fanBase);
     d = DevMesh();
 }
 
@@ -315,7 +352,11 @@ __device__ __forceinline__ unsigned hmix(unsigned k) { return (k * 2654435761u) 
 __device__ __forceinline__ unsigned vslot(unsigned v) { return (hmix(v >> 3) << 3) | (v & 7u); }
 
 struct BatchedPolicy {
+    static constexpr bool kFanRecords = true;
     const FaceRec* __restrict__ F;
+    const FanRec* __restrict__ fanR;
+    const int32_t* __restrict__ fanIdx;
+    const int32_t* __restrict__ fanBase;
     const double* __restrict__ cA;
     const double* __restrict__ cB;
     const int32_t* __restrict__ firstE;
@@ -451,7 +492,12 @@ struct BatchedPolicy {
     BPsRec pnrec;                    // record of the pseudo root
     uint32_t flags;
     bool dead;
-    int c_push, c_pop, c_ppush, c_ppop, c_prop, c_child, c_sw, c_vw, c_fan, c_nb, c_hops, c_err;
+    // The ten event counters of the propagation live in LDS (kLdsCounters
+    // words per lane, lane-contiguous with an odd stride: no bank
+    // conflicts), added to without waiting; the backtrace hop and error
+    // counts stay in registers (the extraction kernel has no LDS counters).
+    int* cl;
+    int c_hops, c_err;
 
     __device__ __forceinline__ int twin(int e) const { return F[e / 3].tw[e % 3]; }
     __device__ __forceinline__ static int next(int e) { return (e % 3 == 2) ? e - 2 : e + 1; }
@@ -495,16 +541,16 @@ struct BatchedPolicy {
     {
         const int k = (int)n;
         switch (which) {
-            case CNT_WIN_PUSH: c_push += k; break;
-            case CNT_WIN_POP: c_pop += k; break;
-            case CNT_PSEUDO_PUSH: c_ppush += k; break;
-            case CNT_PSEUDO_POP: c_ppop += k; break;
-            case CNT_PROPAGATE: c_prop += k; break;
-            case CNT_CHILD_BUILT: c_child += k; break;
-            case CNT_SPLIT_WRITE: c_sw += k; break;
-            case CNT_VERT_WRITE: c_vw += k; break;
-            case CNT_FAN_HALFEDGE: c_fan += k; break;
-            case CNT_PSEUDO_NEIGH: c_nb += k; break;
+            case CNT_WIN_PUSH: atomicAdd(cl + 0, k); break;
+            case CNT_WIN_POP: atomicAdd(cl + 1, k); break;
+            case CNT_PSEUDO_PUSH: atomicAdd(cl + 2, k); break;
+            case CNT_PSEUDO_POP: atomicAdd(cl + 3, k); break;
+            case CNT_PROPAGATE: atomicAdd(cl + 4, k); break;
+            case CNT_CHILD_BUILT: atomicAdd(cl + 5, k); break;
+            case CNT_SPLIT_WRITE: atomicAdd(cl + 6, k); break;
+            case CNT_VERT_WRITE: atomicAdd(cl + 7, k); break;
+            case CNT_FAN_HALFEDGE: atomicAdd(cl + 8, k); break;
+            case CNT_PSEUDO_NEIGH: atomicAdd(cl + 9, k); break;
             case CNT_BT_HOPS: c_hops += k; break;
             case CNT_ERRORS: c_err += k; flags |= SF_ERROR; break;
             default: break;
@@ -633,6 +679,44 @@ struct BatchedPolicy {
         vo = vresolve(so, ho, t.op, fo);
         cacheV = t.op; cacheH = ho; cacheIns = !fo;
         cacheE = t.e0; cacheSH = hs; cacheSIns = !fs;
+    }
+    // --- fans (FanRec): a pseudo-source's sweeps, windows and neighbours
+    __device__ __forceinline__ bool fan_ok() const { return fanR != nullptr; }
+    __device__ __forceinline__ void fan_base(int v, int& base, int& d) const
+    {
+        base = fanBase[v];
+        d = fanBase[v + 1] - base;
+    }
+    __device__ __forceinline__ int fan_index(int e) const { return fanIdx[e]; }
+    __device__ __forceinline__ double fan_cA(int j) const { return fanR[j].cA; }
+    __device__ __forceinline__ double fan_cB(int j) const { return fanR[j].cB; }
+    __device__ __forceinline__ void fan_window(int j, double& lo, double& lnx, double& lpv, int& o, int& twn) const
+    {
+        const double2 x = *reinterpret_cast<const double2*>(&fanR[j].lo);
+        const uint4 y = *reinterpret_cast<const uint4*>(&fanR[j].lpv);
+        lo = x.x; lnx = x.y;
+        lpv = __hiloint2double((int)y.y, (int)y.x);
+        o = (int)y.z; twn = (int)y.w;
+    }
+    __device__ __forceinline__ void fan_neigh(int j, int& nb, int& tw, int& flag, double& lo) const
+    {
+        const uint4 y = *reinterpret_cast<const uint4*>(&fanR[j].nb);
+        nb = (int)y.x; tw = (int)y.y; flag = (int)y.z;
+        lo = fanR[j].lo;
+    }
+    // a vertex lookup in two halves: the first probe's read, then its resolution
+    struct VProbe { unsigned h; int k; uint32_t e; double d; };
+    __device__ __forceinline__ void vprobe_issue(int v, VProbe& q) const
+    {
+        q.h = vslot((unsigned)v) & vmask;
+        ld_slot(q.h, q.k, q.e, q.d);
+    }
+    __device__ __forceinline__ VInfo vprobe_finish(int v, const VProbe& q) const
+    {
+        BSlotV s; s.key = q.k; s.eb = q.e; s.dist = q.d;
+        unsigned h = q.h;
+        bool f;
+        return vresolve(s, h, v, f);
     }
     __device__ __forceinline__ void vput(int v, const VInfo& x)
     {
@@ -1073,6 +1157,8 @@ struct BArgs {
 };
 
 constexpr int kNumBatchedCounters = 15;
+constexpr int kLdsCounters = 10;            // event counters 0..9 of a propagation lane, in LDS
+constexpr int kLdsCounterStride = 11;       // words per lane (odd: lanes hit distinct banks)
 // Launch shapes the autotuner tries: threads per block, source lanes per wave.
 constexpr int kTuneShapes = 5;
 constexpr int kTuneShape[kTuneShapes][2] = {{64, 32}, {128, 32}, {256, 32}, {128, 28}, {256, 28}};
@@ -1087,7 +1173,7 @@ constexpr int kLanesPerWave = 32;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
-    p.F = a.m.face; p.cA = a.m.cornerA; p.cB = a.m.cornerB; p.firstE = a.m.firstEdge;
+    p.F = a.m.face; p.fanR = a.m.fan; p.fanIdx = a.m.fanIdx; p.fanBase = a.m.fanBase; p.cA = a.m.cornerA; p.cB = a.m.cornerB; p.firstE = a.m.firstEdge;
     p.vflags = a.m.flags; p.nv_ = a.m.nv; p.nf_ = a.m.nf;
 }
 
@@ -1108,8 +1194,7 @@ __device__ __forceinline__ void policy_tables(BatchedPolicy& p, const BArgs& a, 
     p.hintA = -1;
     p.pHintI = -1;
     p.flags = 0; p.dead = false;
-    p.c_push = p.c_pop = p.c_ppush = p.c_ppop = p.c_prop = p.c_child = 0;
-    p.c_sw = p.c_vw = p.c_fan = p.c_nb = p.c_hops = p.c_err = 0;
+    p.c_hops = p.c_err = 0;
 }
 
 __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
@@ -1131,6 +1216,7 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     p.lk = ldsHeap + bl;
     p.li = reinterpret_cast<int*>(ldsHeap + 2 * (kHeapLds - 1) * lanesPerBlock) + bl;
     p.plOff = (kHeapLds - 1) * lanesPerBlock;
+    p.cl = reinterpret_cast<int*>(ldsHeap + 3 * (kHeapLds - 1) * lanesPerBlock) + bl * kLdsCounterStride;
     p.wh = a.wheap + (size_t)lane * (a.hslots + a.phslots);
     p.wr = a.wrec + (size_t)lane * (a.wcap + 1 + a.precUnits);
     p.wfree = a.wfree + (size_t)lane * (a.wcap + a.pcap + 2);
@@ -1154,6 +1240,8 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             if (slot >= a.nsrc) break;
             const int src = a.order ? a.order[a.srcBase + slot] : a.srcBase + slot;
             policy_tables(p, a, slot, src);
+#pragma unroll
+            for (int i = 0; i < kLdsCounters; ++i) p.cl[i] = 0;
             p.wtail = 1; p.wftop = 0; p.wfresh = 0; p.wcache = -1; p.wcache2 = -1; p.rootKey = 0.0; p.rootIdx = 0;
             p.wmax = 0; p.pmax = 0;
             p.nrecIdx = -1;
@@ -1177,8 +1265,8 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
             a.status[slot] = p.flags;
             a.vcount[slot] = p.vcount;
             int32_t* c = a.cnt + (size_t)slot * kNumBatchedCounters;
-            c[0] = p.c_push; c[1] = p.c_pop; c[2] = p.c_ppush; c[3] = p.c_ppop; c[4] = p.c_prop;
-            c[5] = p.c_child; c[6] = p.c_sw; c[7] = p.c_vw; c[8] = p.c_fan; c[9] = p.c_nb;
+#pragma unroll
+            for (int i = 0; i < kLdsCounters; ++i) c[i] = p.cl[i];
             c[10] = 0; c[11] = p.vcount; c[12] = p.scount; c[13] = p.wmax; c[14] = p.pmax;
             have = false;
         }
@@ -1270,7 +1358,8 @@ static unsigned next_pow2(unsigned x)
 // (8-B key + 4-B index each) for every source lane of the block.
 static size_t lds_bytes(int block, int lanesPerWave)
 {
-    return (size_t)2 * (kHeapLds - 1) * (size_t)(block / 64) * (size_t)lanesPerWave * (sizeof(double) + sizeof(int));
+    const size_t lanes = (size_t)(block / 64) * (size_t)lanesPerWave;
+    return (size_t)2 * (kHeapLds - 1) * lanes * (sizeof(double) + sizeof(int)) + lanes * kLdsCounterStride * sizeof(int);
 }
 
 std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig& c, DevGraph& g,

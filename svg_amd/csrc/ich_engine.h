@@ -31,6 +31,10 @@
 // halves: a type LookPre, void lookup_issue(Tri, LookPre&) before the pop and
 // void lookup_finish(Tri, LookPre&, VInfo& a, VInfo& b, VInfo& op, SInfo&)
 // after it; void push_hint() is called before a propagation's pushes.
+// A policy with kFanRecords also provides the fan of each vertex by
+// position (fan_ok, fan_base, fan_index, fan_cA, fan_cB, fan_window,
+// fan_neigh) and a vertex lookup in two halves (VProbe, vprobe_issue,
+// vprobe_finish), which gen_sub_wins uses for interior vertices.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -357,10 +361,11 @@ struct Ich {
         if (hasR) { rw.twin = t.tw2; push_win(rw); }
     }
 
-    // ICHDevice::GenSubWinsForPseudoSrcFromWindow (ICHDevice.cu:795-858)
-    SVG_HD void sweep_from_window(const Pseudo& pw, int e0, int& startEdge, int& endEdge)
+    // The two starting angles of GenSubWinsForPseudoSrcFromWindow
+    // (ICHDevice.cu:795-825): the pseudo-source is verts[1](next(e0)), the
+    // window's split point lies on e0.
+    SVG_HD void window_sweep_angles(int e0, double& angle0, double& angle1)
     {
-        (void)pw;
         const int e1 = p.next(e0);
         const int e2 = p.next(e1);
         const double l0 = p.len(e0), l1 = p.len(e1), l2 = p.len(e2);
@@ -374,7 +379,17 @@ struct Ich {
         double a1 = (dx * (v1x - v2x) + dy * (v1y - v2y)) / dlen / l2;
         if (a0 > 1.0) a0 = 1.0; else if (a0 < -1.0) a0 = -1.0;
         if (a1 > 1.0) a1 = 1.0; else if (a1 < -1.0) a1 = -1.0;
-        double angle0 = svg_acos(a0), angle1 = svg_acos(a1);
+        angle0 = svg_acos(a0); angle1 = svg_acos(a1);
+    }
+
+    // ICHDevice::GenSubWinsForPseudoSrcFromWindow (ICHDevice.cu:795-858)
+    SVG_HD void sweep_from_window(const Pseudo& pw, int e0, int& startEdge, int& endEdge)
+    {
+        (void)pw;
+        const int e1 = p.next(e0);
+        const int e2 = p.next(e1);
+        double angle0, angle1;
+        window_sweep_angles(e0, angle0, angle1);
         startEdge = -1; endEdge = -1;
         int cur = p.twin(e1);
         while (angle0 < kPI && cur != -1) {
@@ -413,9 +428,147 @@ struct Ich {
         if (cur != -1) endEdge = p.twin(p.prev(p.twin(cur)));
     }
 
+    // ICHDevice::GenSubWinsForPseudoSrc (ICHDevice.cu:728-904) over the fan
+    // records of a policy that keeps them (kFanRecords): the same sweeps,
+    // windows and neighbour updates, in the same order and arithmetic, with
+    // the fan addressed by position instead of walked one dependent read at
+    // a time.  Position j - 1 (cyclically) is twin(prev(o_j)), the step of
+    // the first sweep and of the window loop; the second sweep's incoming
+    // half-edges twin(o_j) advance with j + 1.  Only interior vertices are
+    // pseudo-sources, so every fan here is a full cycle.
+    SVG_HD void gen_sub_wins_fan(const Pseudo& pw)
+    {
+        const int vert = pw.vert;
+        const VInfo vi = p.vget(vert);
+        int base, d;
+        p.fan_base(vert, base, d);
+        const int last = base + d - 1;
+        int jS = base, jE = base;         // first window position, stop position
+        const int enter = vi.enter;
+        if (!(enter == -1 && vi.birth != -1)) {
+            int jA, jB;                   // where the two sweeps start
+            double angle0 = 0.0, angle1 = 0.0;
+            if (enter >= 0 && p.v0(enter) == vert) {                    // from a pseudo-source
+                jA = p.fan_index(enter);
+                jB = jA;
+            } else if (enter >= 0 && p.v1(p.next(enter)) == vert) {     // from a window
+                window_sweep_angles(enter, angle0, angle1);
+                jB = p.fan_index(p.next(p.next(enter)));
+                jA = jB == base ? last : jB - 1;
+            } else {
+                p.count(CNT_ERRORS, 1);       // unreachable: only interior saddles are pseudo-sources
+                return;
+            }
+            // first sweep: cornerA at jA, jA - 1, ... until the angle reaches
+            // pi; four positions are read at once (those past the stop unused)
+            int j = jA;
+            for (;;) {
+                const int j1 = j == base ? last : j - 1;
+                const int j2 = j1 == base ? last : j1 - 1;
+                const int j3 = j2 == base ? last : j2 - 1;
+                const double c0 = p.fan_cA(j), c1 = p.fan_cA(j1), c2 = p.fan_cA(j2), c3 = p.fan_cA(j3);
+                if (!(angle0 < kPI)) break;
+                angle0 += c0; p.count(CNT_FAN_HALFEDGE, 1); j = j1;
+                if (!(angle0 < kPI)) break;
+                angle0 += c1; p.count(CNT_FAN_HALFEDGE, 1); j = j2;
+                if (!(angle0 < kPI)) break;
+                angle0 += c2; p.count(CNT_FAN_HALFEDGE, 1); j = j3;
+                if (!(angle0 < kPI)) break;
+                angle0 += c3; p.count(CNT_FAN_HALFEDGE, 1); j = j3 == base ? last : j3 - 1;
+            }
+            jS = j == last ? base : j + 1;                 // next(twin(o_j)) = o_{j+1}
+            // second sweep: cornerB of twin(o_j) at jB, jB + 1, ...
+            j = jB;
+            for (;;) {
+                const int j1 = j == last ? base : j + 1;
+                const int j2 = j1 == last ? base : j1 + 1;
+                const int j3 = j2 == last ? base : j2 + 1;
+                const double c0 = p.fan_cB(j), c1 = p.fan_cB(j1), c2 = p.fan_cB(j2), c3 = p.fan_cB(j3);
+                if (!(angle1 < kPI)) break;
+                angle1 += c0; p.count(CNT_FAN_HALFEDGE, 1); j = j1;
+                if (!(angle1 < kPI)) break;
+                angle1 += c1; p.count(CNT_FAN_HALFEDGE, 1); j = j2;
+                if (!(angle1 < kPI)) break;
+                angle1 += c2; p.count(CNT_FAN_HALFEDGE, 1); j = j3;
+                if (!(angle1 < kPI)) break;
+                angle1 += c3; p.count(CNT_FAN_HALFEDGE, 1); j = j3 == last ? base : j3 + 1;
+            }
+            jE = j == base ? last : j - 1;                 // twin(prev(twin(twin(o_j)))) = o_{j-1}
+        }
+        // windows over the edges opposite the fan positions jS, jS - 1, ...
+        // down to (not including) jE; the next record is read before a push,
+        // and the heap ancestors of the first two push slots (and the pseudo
+        // heap's next parent) are read ahead as for a propagation
+        {
+            p.push_hint();
+            int j = jS, guard = 0;
+            double lo, lnx, lpv;
+            int o, twn;
+            p.fan_window(j, lo, lnx, lpv, o, twn);
+            for (;;) {
+                const int jn = j == base ? last : j - 1;
+                double lo2, lnx2, lpv2;
+                int o2, twn2;
+                p.fan_window(jn, lo2, lnx2, lpv2, o2, twn2);
+                Win w;
+                w.edge = p.next(o);
+                w.b0 = 0.0; w.b1 = lnx;
+                w.d0 = lo;
+                w.d1 = lpv;
+                w.ps = pw.dist;
+                win_finish(w);
+                w.pseudoSrc = vert;
+                w.birth = pw.birth;
+                w.level = pw.level + 1;
+                w.twin = twn;
+                push_win(w);
+                j = jn;
+                if (++guard > 4096) { p.count(CNT_ERRORS, 1); break; }
+                if (j == jE) break;
+                lo = lo2; lnx = lnx2; lpv = lpv2; o = o2; twn = twn2;
+            }
+        }
+        // neighbours in fan order from firstEdge, their first probes issued
+        // two at a time (the neighbours are distinct vertices, and a probe
+        // that meets another key continues on the current table)
+        for (int k0 = 0; k0 < d; k0 += 2) {
+            int nb[2], tw[2], fl[2];
+            double ln[2];
+            typename P::VProbe q[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) p.fan_neigh(base + (k0 + i < d ? k0 + i : d - 1), nb[i], tw[i], fl[i], ln[i]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) p.vprobe_issue(nb[i], q[i]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (k0 + i < d) {
+                    p.count(CNT_PSEUDO_NEIGH, 1);
+                    VInfo ov = p.vprobe_finish(nb[i], q[i]);
+                    if (!((fl[i] & 1) || ov.dist < pw.dist + ln[i])) {
+                        if (ov.dist == DBL_MAX) ++total;
+                        ov.dist = pw.dist + ln[i];
+                        ov.birth = (int8_t)(ov.birth + 1);
+                        ov.enter = tw[i];
+                        p.vput(nb[i], ov);
+                        p.count(CNT_VERT_WRITE, 1);
+                        Pseudo c;
+                        c.vert = nb[i]; c.dist = ov.dist;
+                        c.pseudoSrc = vert;
+                        c.birth = ov.birth;
+                        c.level = pw.level;
+                        push_pseudo(c);
+                    }
+                }
+            }
+        }
+    }
+
     // ICHDevice::GenSubWinsForPseudoSrc (ICHDevice.cu:728-793)
     SVG_HD void gen_sub_wins(const Pseudo& pw)
     {
+        if constexpr (P::kFanRecords) {
+            if (p.fan_ok() && !(p.vflag(pw.vert) & 4)) { gen_sub_wins_fan(pw); return; }
+        }
         int startEdge = -1, endEdge = -1;
         const VInfo vi = p.vget(pw.vert);
         if (vi.enter == -1 && vi.birth != -1) {

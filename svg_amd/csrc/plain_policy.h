@@ -42,6 +42,7 @@ struct PVertItem {
 };
 
 struct PlainPolicy {
+    static constexpr bool kFanRecords = false;
     const EdgeAoS* __restrict__ E;
     const VertAoS* __restrict__ Vt;
     const double* __restrict__ cA;

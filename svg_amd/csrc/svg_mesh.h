@@ -33,6 +33,28 @@ struct alignas(64) FaceRec {
 };
 static_assert(sizeof(FaceRec) == 64, "FaceRec must be 64 bytes");
 
+// The fan of a vertex v: its outgoing half-edges o_0 = firstEdge[v],
+// o_{k+1} = next(twin(o_k)) (the order in which a pseudo-source visits its
+// neighbours, ICHDevice.cu:775-791), one 64-byte record per position.  A
+// pseudo-source's sweeps and windows (ICHDevice.cu:728-904) read these
+// records by position instead of walking the fan one dependent read at a
+// time: position k - 1 is twin(prev(o_k)) and the incoming half-edge
+// twin(o_k) is followed by twin(o_{k+1}).
+struct alignas(64) FanRec {
+    double cA;        // cornerA(o): corner at v in face(o)
+    double cB;        // cornerB(twin(o)): corner at v in face(twin(o)); 0 without a twin
+    double lo;        // len(o)
+    double lnx;       // len(next(o)): the edge opposite v in face(o)
+    double lpv;       // len(prev(o))
+    int32_t o;        // the half-edge
+    int32_t twn;      // twin(next(o))
+    int32_t nb;       // verts[1](o): the neighbour
+    int32_t tw;       // twin(o)
+    int32_t nbFlag;   // vertex flags of the neighbour
+    int32_t pad;
+};
+static_assert(sizeof(FanRec) == 64, "FanRec must be 64 bytes");
+
 // Reference-layout records for the plain kernel (Mesh.cuh:6-23).
 struct EdgeAoS {
     int32_t v0, v1, twin, next, prev, face;
@@ -79,6 +101,9 @@ struct DevMesh {
     uint8_t* flags = nullptr;       // nv
     double* pos = nullptr;          // 4*nv (x, y, z, 0) for the A* heuristic
     EdgeAoS* edges = nullptr;       // ne, reference layout (plain kernel)
+    FanRec* fan = nullptr;          // ne: the fans of all vertices, vertex by vertex (null: not manifold)
+    int32_t* fanIdx = nullptr;      // ne: position of half-edge e in the fan of verts[0](e)
+    int32_t* fanBase = nullptr;     // nv + 1: first fan position of each vertex
     VertAoS* verts = nullptr;       // nv, reference layout (plain kernel)
 };
 
