@@ -370,17 +370,20 @@ def main():
             first_build = {"seconds": st0["seconds"], "tuned": st0["tuned"], "prop_block": st0["prop_block"]}
         log(f"warmup {i}: build {st0['seconds']:.4f} s (tuned {st0['tuned']})")
         g.FreeSVGStructure()
-    times, stats = [], []
+    times, stats, walls = [], [], []
     wall0 = time.perf_counter()
     for i in range(args.steps):
+        t0 = time.perf_counter()
         g = build(S.SVG_MODE_BATCHED, autotune=1)
         st = g.stats()
+        if i < args.steps - 1:
+            g.FreeSVGStructure()
+        walls.append(time.perf_counter() - t0)
         times.append(st["seconds"])
         stats.append(st)
         log(f"step {i}: build {st['seconds']:.4f} s (prop {st['prop_seconds']:.4f}, "
-            f"extract {st['extract_seconds']:.4f}, redo {st['redo_seconds']:.4f}), arcs {st['arcs']}")
-        if i < args.steps - 1:
-            g.FreeSVGStructure()
+            f"extract {st['extract_seconds']:.4f}, redo {st['redo_seconds']:.4f}), arcs {st['arcs']}, "
+            f"wall {walls[-1]:.4f} s")
     wall = time.perf_counter() - wall0
     st = stats[-1]
     c = st["counters"]
@@ -397,7 +400,8 @@ def main():
         "config": {"workload": args.workload, "V": V, "F": len(tri), "K": K,
                    "mesh_sha256": digest, "mode": "batched", "parallelism": "single GPU"},
         "build_s": build_s, "build_s_all": times, "wall_s_steps": wall,
-        "wall_ms_per_step": wall / args.steps * 1e3,
+        "wall_ms_per_step": wall / args.steps * 1e3, "wall_s_all": walls,
+        "wall_ms_per_step_median": float(np.median(walls)) * 1e3,
         "wall_includes": "per-build hipMalloc / hipFree of the tables, their resets, the stats copies",
         "first_build_untuned": first_build,
         "batched": {k: st[k] for k in ("prop_seconds", "extract_seconds", "redo_seconds", "redone",
