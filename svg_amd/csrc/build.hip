@@ -1362,6 +1362,21 @@ static size_t lds_bytes(int block, int lanesPerWave)
     return (size_t)2 * (kHeapLds - 1) * lanes * (sizeof(double) + sizeof(int)) + lanes * kLdsCounterStride * sizeof(int);
 }
 
+// Source lanes the device holds at once for a launch shape: resident
+// blocks per CU (registers, LDS) x CUs x lanes per block.  More lanes than
+// this would only start as resident waves end, with the work already taken.
+static int resident_lanes(int block, int lanesPerWave)
+{
+    int perCU = 0, dev = 0, nCU = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, batched_prop_kernel, block,
+                                                     lds_bytes(block, lanesPerWave)) != hipSuccess ||
+        hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || perCU <= 0 ||
+        nCU <= 0)
+        return 65536;
+    return perCU * nCU * (block / 64) * lanesPerWave;
+}
+
 std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig& c, DevGraph& g,
                           BuildStats& st)
 {
@@ -1388,8 +1403,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     int wcap = c.queueCap > 0 ? c.queueCap : kHeapCap;
     int pcap = c.queueCap > 0 ? c.queueCap : kHeapCap;
     // Autotuner (per mesh, K and coefficients; results never depend on it):
-    // once a build has recorded every source's event count, the cheapest
-    // sources are the ones left to start late; when the default tables would
+    // once a build has recorded every source's event count, sources start
+    // from the costliest down; when the default tables would
     // not fit at once they are sized to the peak occupancy recorded (a source
     // that still overflows is recomputed exactly on the plain path); the
     // queues get the storage their recorded peak needs (with a quarter of
@@ -1419,34 +1434,28 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
             if (tune->maxWinQ < kHeapCap) wcap = capOf(tune->maxWinQ);
             if (tune->maxPseudoQ < kHeapCap) pcap = capOf(tune->maxPseudoQ);
         }
-        const int nLanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
-        if ((int)tune->cost.size() == V && nLanes < V) {
-            // The first nLanes sources start together; the others start as
-            // lanes free up.  The cheapest V - nLanes go last, and each group
-            // keeps id order, so the lanes of a wave stay on nearby sources
-            // (their mesh data shares cache lines).
-            std::vector<int32_t> byCost((size_t)V);
-            for (int i = 0; i < V; ++i) byCost[i] = i;
-            std::nth_element(byCost.begin(), byCost.begin() + (V - nLanes), byCost.end(),
-                             [&](int x, int y) { return tune->cost[x] < tune->cost[y] || (tune->cost[x] == tune->cost[y] && x < y); });
-            std::vector<char> late((size_t)V, 0);
-            for (int i = 0; i < V - nLanes; ++i) late[byCost[i]] = 1;
-            order.reserve((size_t)V);
-            for (int i = 0; i < V; ++i) if (!late[i]) order.push_back(i);
-            for (int i = 0; i < V; ++i) if (late[i]) order.push_back(i);
-        }
-        if ((int)tune->cost.size() == V) st.tuned = 1;
     }
-    if (vcap < 64) vcap = 64;
-    if (scap < 64) scap = 64;
     // one block's LDS stays within 64 KiB
     while (block > 64 && lds_bytes(block, lanesPerWave) > (64u << 10)) block /= 2;
+    const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, resident_lanes(block, lanesPerWave));
+    if (tune && (int)tune->cost.size() == V && lanes < V) {
+        // Sources are taken from the costliest down (longest processing
+        // time first): a lane runs about 1.25 sources on the torus, so a
+        // costly source left to start late would set the end of the launch.
+        std::vector<int32_t> byCost((size_t)V);
+        for (int i = 0; i < V; ++i) byCost[i] = i;
+        auto dearer = [&](int x, int y) { return tune->cost[x] > tune->cost[y] || (tune->cost[x] == tune->cost[y] && x < y); };
+        std::sort(byCost.begin(), byCost.end(), dearer);
+        order = byCost;
+    }
+    if (tune && (int)tune->cost.size() == V) st.tuned = 1;
+    if (vcap < 64) vcap = 64;
+    if (scap < 64) scap = 64;
     // The extraction sorts a source's reached list in LDS (at most 8192
     // entries): larger tables are capped there, and a source that reaches
     // more vertices than the capped table holds is flagged and recomputed
     // on the plain path (large K).
     if (vcap > 8192) vcap = 8192;
-    const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, 65536);
     size_t freeB = 0, totB = 0;
     {
         const hipError_t me = hipMemGetInfo(&freeB, &totB);
