@@ -2,7 +2,8 @@
 oracle at the same settings: batched table capacity overflow (recomputed on
 the plain path), reference-table overflow with its aliasing overflow slot
 (MyHashTable.cuh:104-109), chunked builds, large K beyond the batched
-vertex-table cap, every lanes-per-wave shape, and the serving loop's memory
+vertex-table cap, every lanes-per-wave shape, a mesh with a non-manifold
+vertex (fans walked, not read by position), and the serving loop's memory
 admission.  Bit-exact like the other build tests."""
 import ctypes as C
 
@@ -278,3 +279,27 @@ def test_server_completion_callback():
         assert seen[t][0] == 0 and seen[t][1] >= 0
         assert np.array_equal(dout, want)
         assert seen[t][2] == want[0]
+
+
+def _pinched_spheres():
+    # Two spheres that share one vertex: its outgoing half-edges form two
+    # fans, so the mesh is not manifold there and the batched build walks
+    # the fans (ICHDevice.cu:728-904) instead of reading them by position.
+    xyz, tri = SPHERE()
+    V = len(xyz)
+    p0 = xyz[0]
+    n = p0 / np.linalg.norm(p0)
+    xyz2 = xyz - 2.0 * np.outer((xyz - p0) @ n, n)     # mirror image through the tangent plane at vertex 0
+    remap = np.concatenate([[0], np.arange(1, V) + V - 1])
+    tri2 = remap[tri][:, ::-1]                         # the mirror flips the orientation
+    return np.vstack([xyz, xyz2[1:]]), np.vstack([tri, tri2]).astype(np.int32)
+
+
+@pytest.mark.parametrize("mode", [S.SVG_MODE_BATCHED, S.SVG_MODE_PLAIN])
+def test_non_manifold_vertex_rows_are_the_oracles(mode):
+    xyz, tri = _pinched_spheres()
+    K = 40
+    print(f"case pinched spheres mode {mode}", flush=True)
+    rows, tails, _ = _oracle(xyz, tri, K)
+    _, g = _build(xyz, tri, K, mode)
+    _assert_rows(g, rows, tails, K)
