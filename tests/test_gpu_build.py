@@ -55,16 +55,32 @@ def test_build_heightfield64_full_both_modes():
         assert not bad, (mode, bad[:10])
 
 
-def test_build_sphere30k_sample():
-    """noisy_sphere (V=30,722, K=100), batched, oracle on 2,000 spread sources."""
+def test_build_sphere30k_full():
+    """noisy_sphere (V=30,722, K=100), batched: every row bit-equal to the
+    oracle's."""
     xyz, tri = mg.noisy_sphere()
     K = 100
-    srcs = np.linspace(0, len(xyz) - 1, 2000).astype(int)
-    print("case sphere30k", flush=True)
-    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
+    print("case sphere30k (all sources)", flush=True)
+    orows, otails, _, secs = O.svg_build(O.Mesh(xyz, tri), K)
+    print(f"oracle build {secs:.1f} s", flush=True)
     m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
-    bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
-    assert not bad, bad[:10]
+    tails, h = gpu_rows(g)
+    _compare_all(tails, h, orows, otails, K)
+
+
+def _oracle_blocks(xyz, tri, K, nblocks, per_block):
+    """Oracle rows for nblocks runs of per_block consecutive sources spread
+    over the mesh, each run built on every host thread."""
+    om = O.Mesh(xyz, tri)
+    V = len(xyz)
+    starts = np.linspace(0, V - per_block, nblocks).astype(int)
+    srcs, rows, tails = [], [], []
+    for s0 in starts:
+        r, t, _, _ = O.svg_build(om, K, s0=int(s0), s1=int(s0) + per_block)
+        srcs.extend(range(int(s0), int(s0) + per_block))
+        rows.append(r)
+        tails.append(t)
+    return srcs, np.concatenate(rows), np.concatenate(tails)
 
 
 def _compare_all(tails, h, orows, otails, K):
@@ -127,13 +143,13 @@ def test_build_torus72k_full_and_autotuned():
 def test_build_terrain750k_sample(K):
     """aniso_terrain (866x866 grid, V=749,956, skinny/obtuse triangles, open
     boundary) at K=100 and K=500 (the reference runs out of memory at 500):
-    every source built in one pass, oracle parity on 480 spread sources,
-    every row sorted and within K."""
+    every source built in one pass, oracle parity on 4,800 sources (16 runs
+    of 300 consecutive ones spread over the terrain, boundary rows
+    included), every row sorted and within K."""
     xyz, tri = mg.aniso_terrain()
     print(f"case terrain750k K={K}", flush=True)
     m, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED)
-    srcs = np.linspace(0, len(xyz) - 1, 480).astype(int)
-    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K, sources=srcs)
+    srcs, orows, otails = _oracle_blocks(xyz, tri, K, 16, 300)
     tails, h = gpu_rows(g)
     bad = compare_rows((tails, h), orows, otails, srcs, K)
     assert not bad, bad[:10]
