@@ -431,16 +431,20 @@ def main():
     if args.plain != "none":
         grids = [(64, 128)] if args.plain == "ref" else [(64, 128), (256, 128), (1125, 64), (576, 128)]
         best = None
+        per_grid = {}
         for (b, t) in grids:
             gp = build(S.SVG_MODE_PLAIN, plain_blocks=b, plain_threads=t)
             s = gp.stats()["seconds"]
+            per_grid[f"{b}x{t}"] = s
             log(f"plain grid {b}x{t}: {s:.3f} s")
             if best is None or s < best[0]:
                 best = (s, b, t)
             gp.FreeSVGStructure()
         res["plain"] = {"best_seconds": best[0], "grid": [best[1], best[2]],
-                        "grids_tried": [list(x) for x in grids]}
+                        "grids_tried": [list(x) for x in grids], "seconds_per_grid": per_grid}
         res["speedup_batched_vs_plain"] = best[0] / build_s
+        # the reference launches its plain kernel on 64 blocks of 128 threads (SVG.cuh:10-11, SVG.cu:163)
+        res["speedup_batched_vs_plain_reference_grid"] = per_grid["64x128"] / build_s
 
     # ---- serving
     if args.requests > 0:
