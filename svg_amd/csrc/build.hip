@@ -321,24 +321,30 @@ struct BHeapEnt { double key; int32_t idx; int32_t pad; };     // 16 B
 
 constexpr uint32_t kEnterNone = 0xFFFFFFu;
 
-// Window-heap placement: logical binary-heap index i (1-based) keeps the
-// reference's semantics; only where entries live changes.  Levels 0-3
-// (i < 16) sit in LDS, lane-major so a wave's lanes hit distinct banks;
-// deeper levels sit in HBM with every three levels below a band root in one
-// 128-B block of eight 16-B entries (levels 4-6: 16 blocks, 7-9: 128 blocks),
-// so a sift-down through a full heap touches about two memory lines.
-constexpr int kHeapLds = 16;                          // logical 1..15 in LDS
-constexpr int kHeapBlocks = 16 + 128;
-constexpr int kHeapSlots = kHeapBlocks * 8;
+// Heap placement: logical binary-heap index i (1-based) keeps the
+// reference's semantics; only where entries live changes.  Levels 0-4
+// (i < 32) sit in LDS, lane-major so a wave's lanes hit distinct banks (8-B
+// keys and 16-bit record indices in separate arrays); levels 5-7 sit in HBM
+// as 32 blocks of 128 B, each holding a level-5 node with its children and
+// grandchildren (slots 1..7 of eight 16-B entries), and levels 8-9 as 256
+// blocks of 64 B, each a level-8 node with its two children (slots 0..2).
+// A sift-down below LDS reads two adjacent blocks per step: levels 5-7,
+// then levels 8-9.
+constexpr int kHeapLds = 32;                          // logical 1..31 in LDS
+constexpr int kMidBlocks = 32;                        // levels 5-7, 8 slots each
+constexpr int kDeepBase = kMidBlocks * 8;             // levels 8-9: 4 slots per block after them
+constexpr int kHeapSlotsMid = kDeepBase;              // storage below 256 items (max index 255)
+constexpr int kHeapSlotsFull = kDeepBase + 256 * 4;   // storage up to the reference's 1023 items
 
-__device__ __forceinline__ int bslot(int i)           // i >= 16
+__device__ __forceinline__ int bslot(int i)           // i >= 32
 {
-    const int d = 31 - __clz(i);                      // 4..9
-    const int b = d >= 7 ? 1 : 0;
-    const int r = d - 4 - 3 * b;
-    const int root = i >> r;
-    const int block = b ? 16 + (root - 128) : (root - 16);
-    return block * 8 + ((1 << r) | (i & ((1 << r) - 1)));
+    const int d = 31 - __clz(i);                      // 5..9
+    if (d <= 7) {
+        const int r = d - 5;
+        return ((i >> r) - 32) * 8 + ((1 << r) | (i & ((1 << r) - 1)));
+    }
+    const int r = d - 8;                              // 0: the level-8 node, 1: one of its children
+    return kDeepBase + ((i >> r) - 256) * 4 + (r ? 1 + (i & 1) : 0);
 }
 
 // Multiplicative hashing (Knuth): the slot comes from bits 11 and up of
@@ -370,7 +376,7 @@ struct BatchedPolicy {
     BHeapEnt* wh; BWinRec* wr; uint16_t* wfree;
     int wcap, pcap;                  // queue storage per lane (<= kHeapCap); wmax, pmax: peak lengths
     int wmax, pmax;
-    double* lk; int* li; int lstride;   // LDS part of the heap (logical 1..15), lane-major
+    double* lk; uint16_t* li; int lstride;   // LDS part of the heap (logical 1..31), lane-major
     int wtail, wftop, wfresh, wcache, wcache2;
     double rootKey; int rootIdx;
     BWinRec nrec; int nrecIdx;       // record of the root, loaded right after a pop
@@ -381,7 +387,7 @@ struct BatchedPolicy {
 
     __device__ __forceinline__ void hset(int i, const BHeapEnt& e)
     {
-        if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = e.idx; }
+        if (i < kHeapLds) { lk[(i - 1) * lstride] = e.key; li[(i - 1) * lstride] = (uint16_t)e.idx; }
         else wh[bslot(i)] = e;
     }
     // The global-level ancestors (index >= kHeapLds) of the two slots a
@@ -393,7 +399,7 @@ struct BatchedPolicy {
     // coincide, chA[s] serves both).  A push into A keeps chA current, so
     // the push into A + 1 after it still reads current entries; a window
     // pop, or the push into A + 1, ends the hint (hintA = -1).
-    static constexpr int kChain = 4;          // ancestors kept (a slot below 512 has no more global ones)
+    static constexpr int kChain = 4;          // ancestors kept: every global ancestor of a slot up to 1023
     int hintA = -1;
     // chA<s> / chB<s> as named members: constant indices keep them in registers
     BHeapEnt cA0, cA1, cA2, cA3, cB0, cB1, cB2, cB3;
@@ -486,7 +492,7 @@ struct BatchedPolicy {
     __device__ __forceinline__ BPsRec* prr() const { return reinterpret_cast<BPsRec*>(wr + prOff); }
     __device__ __forceinline__ uint16_t* pfr() const { return wfree + pfOff; }
     __device__ __forceinline__ double* plk() const { return lk + plOff; }
-    __device__ __forceinline__ int* pli() const { return li + plOff; }
+    __device__ __forceinline__ uint16_t* pli() const { return li + plOff; }
     int ptail, pftop, pfresh, pcache, pcache2;
     double pRootKey; int pRootIdx;
     BPsRec pnrec;                    // record of the pseudo root
@@ -862,7 +868,7 @@ struct BatchedPolicy {
             chain_step<2>(slot, useA, key, cur, done);
             chain_step<3>(slot, useA, key, cur, done);
             p = cur / 2;
-            // a slot of 512 or more has one global ancestor above the kept ones
+            // ancestors above the kept ones (none below slot 1024)
             while (!done && p >= kHeapLds) {
                 const BHeapEnt pe = gget(p);
                 if (!(pe.key > key)) { done = true; break; }
@@ -913,7 +919,7 @@ struct BatchedPolicy {
     }
     __device__ __forceinline__ void hset_at(int i, const BHeapEnt& e, int goff, int loff)
     {
-        if (i < kHeapLds) { lk[loff + (i - 1) * lstride] = e.key; li[loff + (i - 1) * lstride] = e.idx; }
+        if (i < kHeapLds) { lk[loff + (i - 1) * lstride] = e.key; li[loff + (i - 1) * lstride] = (uint16_t)e.idx; }
         else wh[goff + bslot(i)] = e;
     }
     __device__ __forceinline__ bool sift_blocks_at(int& cur, const BHeapEnt& top, int blkOfLeft, int goff, int loff,
@@ -954,6 +960,40 @@ struct BatchedPolicy {
         { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
         cur = right ? r : l;
         return true;
+    }
+    // Sift-down through levels 8-9: the children of `cur` (level 7) are the
+    // level-8 roots of two adjacent 64-B blocks, each with its two children.
+    __device__ __forceinline__ void sift_deep_at(int& cur, const BHeapEnt& top, int dblkOfLeft, int goff, int loff,
+                                                 int tail)
+    {
+        double ka[3], kb[3];
+        int ia[3], ib[3];
+        const uint4* qa = reinterpret_cast<const uint4*>(wh + goff + kDeepBase + dblkOfLeft * 4);
+        const uint4* qb = qa + 4;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const uint4 va = qa[s], vb = qb[s];
+            ka[s] = __hiloint2double((int)va.y, (int)va.x); ia[s] = (int)va.z;
+            kb[s] = __hiloint2double((int)vb.y, (int)vb.x); ib[s] = (int)vb.z;
+        }
+        int l = 2 * cur, r = l + 1;
+        if (l >= tail) return;
+        bool right = r < tail && !(ka[0] < kb[0]);
+        double ck = right ? kb[0] : ka[0];
+        int ci = right ? ib[0] : ia[0];
+        if (!(ck < top.key)) return;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
+        cur = right ? r : l;
+        const double y0 = right ? kb[1] : ka[1], y1 = right ? kb[2] : ka[2];
+        const int j0 = right ? ib[1] : ia[1], j1 = right ? ib[2] : ia[2];
+        l = 2 * cur; r = l + 1;
+        if (l >= tail) return;
+        right = r < tail && !(y0 < y1);
+        ck = right ? y1 : y0;
+        ci = right ? j1 : j0;
+        if (!(ck < top.key)) return;
+        { BHeapEnt m; m.key = ck; m.idx = ci; m.pad = 0; hset_at(cur, m, goff, loff); }
+        cur = right ? r : l;
     }
     // PriorityQueue.cuh:75-105 on the window queue (isP false) or the
     // pseudo queue (isP true)
@@ -1015,9 +1055,9 @@ struct BatchedPolicy {
         bool deeper = false;
         while (mc != -1 && mk < top.key) {
             lk[loff + (cur - 1) * lstride] = mk;
-            li[loff + (cur - 1) * lstride] = mi;
+            li[loff + (cur - 1) * lstride] = (uint16_t)mi;
             cur = mc;
-            if (cur >= 8) { deeper = true; break; }
+            if (cur >= 16) { deeper = true; break; }
             const int l = 2 * cur, r = l + 1;
             if (r < tail) {
                 const double kl = lk[loff + (l - 1) * lstride], kr = lk[loff + (r - 1) * lstride];
@@ -1030,8 +1070,8 @@ struct BatchedPolicy {
                 mc = -1;
             }
         }
-        if (deeper && sift_blocks_at(cur, top, 2 * cur - 16, goff, loff, tail) && 2 * cur < tail)
-            sift_blocks_at(cur, top, 16 + (2 * cur - 128), goff, loff, tail);
+        if (deeper && sift_blocks_at(cur, top, 2 * cur - 32, goff, loff, tail) && 2 * cur < tail)
+            sift_deep_at(cur, top, 2 * cur - 256, goff, loff, tail);
         hset_at(cur, top, goff, loff);
         const double nk2 = cur == nl ? top.key : nlk;
         const int ni2 = cur == nl ? top.idx : nli;
@@ -1061,7 +1101,7 @@ struct BatchedPolicy {
     __device__ __forceinline__ BHeapEnt pgget(int i) const { return pgh()[bslot(i)]; }
     __device__ __forceinline__ void phset(int i, const BHeapEnt& e)
     {
-        if (i < kHeapLds) { plk()[(i - 1) * lstride] = e.key; pli()[(i - 1) * lstride] = e.idx; }
+        if (i < kHeapLds) { plk()[(i - 1) * lstride] = e.key; pli()[(i - 1) * lstride] = (uint16_t)e.idx; }
         else pgh()[bslot(i)] = e;
     }
     __device__ __forceinline__ bool pempty() const { return ptail == 1; }
@@ -1160,9 +1200,10 @@ constexpr int kNumBatchedCounters = 15;
 constexpr int kLdsCounters = 10;            // event counters 0..9 of a propagation lane, in LDS
 constexpr int kLdsCounterStride = 11;       // words per lane (odd: lanes hit distinct banks)
 // Launch shapes the autotuner tries, the default one first: threads per
-// block, source lanes per wave.
+// block, source lanes per wave.  Each keeps one block's LDS (664 B per
+// lane) within 64 KiB, so none is cut down to another.
 constexpr int kTuneShapes = 5;
-constexpr int kTuneShape[kTuneShapes][2] = {{128, 28}, {256, 28}, {64, 32}, {128, 32}, {256, 32}};
+constexpr int kTuneShape[kTuneShapes][2] = {{128, 28}, {64, 28}, {128, 30}, {128, 26}, {64, 32}};
 // A wave runs its pseudo-source events once at least 1/kPseudoShare of its
 // stepping lanes wait for one.
 constexpr int kPseudoShare = 4;
@@ -1218,9 +1259,11 @@ __global__ void __launch_bounds__(256) batched_prop_kernel(BArgs a)
     policy_mesh(p, a);
     p.lstride = lanesPerBlock;
     p.lk = ldsHeap + bl;
-    p.li = reinterpret_cast<int*>(ldsHeap + 2 * (kHeapLds - 1) * lanesPerBlock) + bl;
+    p.li = reinterpret_cast<uint16_t*>(ldsHeap + 2 * (kHeapLds - 1) * lanesPerBlock) + bl;
     p.plOff = (kHeapLds - 1) * lanesPerBlock;
-    p.cl = reinterpret_cast<int*>(ldsHeap + 3 * (kHeapLds - 1) * lanesPerBlock) + bl * kLdsCounterStride;
+    p.cl = reinterpret_cast<int*>(reinterpret_cast<char*>(ldsHeap) +
+                                  (size_t)2 * (kHeapLds - 1) * lanesPerBlock * (sizeof(double) + sizeof(uint16_t))) +
+           bl * kLdsCounterStride;
     p.wh = a.wheap + (size_t)lane * (a.hslots + a.phslots);
     p.wr = a.wrec + (size_t)lane * (a.wcap + 1 + a.precUnits);
     p.wfree = a.wfree + (size_t)lane * (a.wcap + a.pcap + 2);
@@ -1358,12 +1401,13 @@ static unsigned next_pow2(unsigned x)
     return p;
 }
 
-// LDS of one propagation block: heap levels 1..15 of both queues
-// (8-B key + 4-B index each) for every source lane of the block.
+// LDS of one propagation block: heap levels 1..31 of both queues (8-B key
+// and 16-bit record index each) and the event counters, for every source
+// lane of the block.
 static size_t lds_bytes(int block, int lanesPerWave)
 {
     const size_t lanes = (size_t)(block / 64) * (size_t)lanesPerWave;
-    return (size_t)2 * (kHeapLds - 1) * lanes * (sizeof(double) + sizeof(int)) + lanes * kLdsCounterStride * sizeof(int);
+    return (size_t)2 * (kHeapLds - 1) * lanes * (sizeof(double) + sizeof(uint16_t)) + lanes * kLdsCounterStride * sizeof(int);
 }
 
 // Source lanes the device holds at once for a launch shape: resident
@@ -1465,9 +1509,9 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         const hipError_t me = hipMemGetInfo(&freeB, &totB);
         if (me != hipSuccess) return std::string("batched build: hipMemGetInfo: ") + hipGetErrorString(me);
     }
-    // heap blocks: 16 (slots below 128) or all 144
-    const int hslots = (wcap < 128 ? 16 : kHeapBlocks) * 8;
-    const int phslots = (pcap < 128 ? 16 : kHeapBlocks) * 8;
+    // heap storage: the level 5-7 blocks (queues below 256 items) or all
+    const int hslots = wcap < 256 ? kHeapSlotsMid : kHeapSlotsFull;
+    const int phslots = pcap < 256 ? kHeapSlotsMid : kHeapSlotsFull;
     const int precUnits = (int)((sizeof(BPsRec) * (size_t)(pcap + 1) + sizeof(BWinRec) - 1) / sizeof(BWinRec));
     const size_t perLane = sizeof(BHeapEnt) * (size_t)(hslots + phslots) +
                            sizeof(BWinRec) * (size_t)(wcap + 1 + precUnits) + sizeof(uint16_t) * (size_t)(wcap + pcap + 2);
