@@ -1590,11 +1590,14 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
                            0, a);
         hipEventRecord(e1, 0);
         const int eblocks = std::min(n, 16384);
-        if (vcap <= 1024)
+        // an extracted source reached at most vrefcap vertices (beyond that
+        // it is flagged and recomputed on the plain path)
+        const unsigned sortCap = std::min(vcap, next_pow2((unsigned)std::max(vrefcap, 1)));
+        if (sortCap <= 1024)
             hipLaunchKernelGGL(batched_extract_kernel<1024>, dim3(eblocks), dim3(64), 0, 0, a);
-        else if (vcap <= 2048)
+        else if (sortCap <= 2048)
             hipLaunchKernelGGL(batched_extract_kernel<2048>, dim3(eblocks), dim3(64), 0, 0, a);
-        else if (vcap <= 4096)
+        else if (sortCap <= 4096)
             hipLaunchKernelGGL(batched_extract_kernel<4096>, dim3(eblocks), dim3(64), 0, 0, a);
         else
             hipLaunchKernelGGL(batched_extract_kernel<8192>, dim3(eblocks), dim3(64), 0, 0, a);
