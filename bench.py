@@ -367,7 +367,8 @@ def main():
         g = build(S.SVG_MODE_BATCHED, autotune=1)
         st0 = g.stats()
         if i == 0:
-            first_build = {"seconds": st0["seconds"], "tuned": st0["tuned"], "prop_block": st0["prop_block"]}
+            first_build = {"seconds": st0["seconds"], "tuned": st0["tuned"], "prop_block": st0["prop_block"],
+                           "source_order": st0["source_order"]}
         log(f"warmup {i}: build {st0['seconds']:.4f} s (tuned {st0['tuned']})")
         g.FreeSVGStructure()
     times, stats, walls = [], [], []
@@ -407,7 +408,7 @@ def main():
         "batched": {k: st[k] for k in ("prop_seconds", "extract_seconds", "redo_seconds", "redone",
                                        "flagged_drop", "flagged_overflow", "flagged_error", "lanes",
                                        "prop_block", "lanes_per_wave", "queue_cap", "pseudo_queue_cap",
-                                       "chunk", "vert_cap", "split_cap", "tuned", "arcs")},
+                                       "chunk", "vert_cap", "split_cap", "tuned", "source_order", "arcs")},
         "windows_per_s": c["win_pop"] / st["prop_seconds"],
         "counters": c,
         "roofline": {"bound": "hbm", "kernel": "batched_prop_kernel",

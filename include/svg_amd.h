@@ -80,6 +80,12 @@ typedef struct {
                                recorded by earlier builds).  A source that outgrows a storage
                                below 1023 is recomputed exactly on the plain path, so the
                                rows never depend on it. */
+    int source_order;       /* batched: order in which lanes take sources (rows never depend
+                               on it): 0 = auto (with autotune, the costliest first by the
+                               event counts recorded by an earlier build; otherwise, when
+                               there are more sources than lanes, the order predicted from
+                               the share of saddle vertices around each source), 1 = vertex
+                               id order, 2 = the predicted order */
 } svg_build_params;
 
 typedef struct {
@@ -104,6 +110,9 @@ typedef struct {
     int pseudo_queue_cap;   /* pseudo-window queue storage per lane used */
     int fallback_plain;     /* 1: a batched build ran every source on the plain path (the
                                mesh has 2^24-1 or more half-edges) */
+    int source_order;       /* order the sources ran in: 1 vertex id, 2 predicted from the
+                               mesh's saddle vertices, 3 costliest first by recorded event
+                               counts (0: plain build) */
 } svg_build_stats;
 
 void svg_build_params_default(svg_build_params *p, int K);

@@ -28,7 +28,7 @@ class BuildParams(C.Structure):
                 ("mode", C.c_int), ("plain_blocks", C.c_int), ("plain_threads", C.c_int),
                 ("lanes", C.c_int), ("prop_block", C.c_int), ("chunk", C.c_int),
                 ("vert_cap", C.c_int), ("split_cap", C.c_int), ("autotune", C.c_int),
-                ("lanes_per_wave", C.c_int), ("queue_cap", C.c_int)]
+                ("lanes_per_wave", C.c_int), ("queue_cap", C.c_int), ("source_order", C.c_int)]
 
 
 class BuildStats(C.Structure):
@@ -39,7 +39,8 @@ class BuildStats(C.Structure):
                 ("flagged_error", C.c_int), ("lanes", C.c_int), ("prop_block", C.c_int),
                 ("chunk", C.c_int), ("vert_cap", C.c_int), ("split_cap", C.c_int),
                 ("tuned", C.c_int), ("counters", C.c_longlong * 16), ("lanes_per_wave", C.c_int),
-                ("queue_cap", C.c_int), ("pseudo_queue_cap", C.c_int), ("fallback_plain", C.c_int)]
+                ("queue_cap", C.c_int), ("pseudo_queue_cap", C.c_int), ("fallback_plain", C.c_int),
+                ("source_order", C.c_int)]
 
 
 class ServerCfg(C.Structure):

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "svg_acos.h"
@@ -1215,6 +1216,12 @@ constexpr int kPseudoShare = 4;
 // the fastest (24: +16%, 26: +3%, 30: +2%, 32: +3.5%, 36: +5.5%).
 constexpr int kLanesPerWave = 28;
 constexpr int kPropBlock = 128;
+// Without recorded event counts, the predicted order is used while the
+// sources are at most this many per lane: the order decides how long the
+// launch's tail is when a lane runs one or two sources, while with many
+// per lane the shared work counter already evens the lanes out (no
+// difference measured at 13 per lane).
+constexpr int kPredictPerLane = 4;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
@@ -1401,6 +1408,54 @@ static unsigned next_pow2(unsigned x)
     return p;
 }
 
+// Source order for a build with no recorded event counts.  A source's
+// event count grows with the saddle vertices among those it reaches: each
+// reached saddle becomes a pseudo-source that emits windows over its whole
+// fan.  The costliest-first order is therefore estimated from the share of
+// saddles around each vertex: the saddle indicator averaged with the
+// neighbours ceil(sqrt(K/3)) times (the number of rings that hold K
+// vertices on a valence-6 mesh).  Sources run from the highest share down,
+// ordered by a stable counting sort over 2^16 levels, so vertex id breaks ties.
+// A mesh without saddles keeps the id order (returns false).
+static bool predicted_order(const HostMesh& h, int K, std::vector<int32_t>& order)
+{
+    const int V = h.nv;
+    std::vector<float> x((size_t)V), acc((size_t)V);
+    std::vector<float> inv((size_t)V);
+    std::vector<int32_t> deg((size_t)V, 0);
+    bool any = false;
+    for (int e = 0; e < 3 * h.nf; ++e) deg[h.tri[e]]++;
+    for (int v = 0; v < V; ++v) {
+        x[v] = (h.flags[v] & VF_SADDLE) ? 1.0f : 0.0f;
+        any |= x[v] != 0.0f;
+        inv[v] = 1.0f / (float)(1 + deg[v]);
+    }
+    if (!any) return false;
+    const int rings = std::min(32, std::max(1, (int)std::ceil(std::sqrt(K / 3.0))));
+    for (int it = 0; it < rings; ++it) {
+        acc = x;
+        for (int f = 0; f < h.nf; ++f) {
+            const int32_t* t = &h.tri[3 * (size_t)f];
+            acc[t[0]] += x[t[1]];
+            acc[t[1]] += x[t[2]];
+            acc[t[2]] += x[t[0]];
+        }
+        for (int v = 0; v < V; ++v) x[v] = acc[v] * inv[v];
+    }
+    constexpr int kLevels = 1 << 16;
+    std::vector<int32_t> start(kLevels + 1, 0);
+    std::vector<uint16_t> lvl((size_t)V);
+    for (int v = 0; v < V; ++v) {
+        const float s = std::min(1.0f, std::max(0.0f, x[v]));
+        lvl[v] = (uint16_t)(kLevels - 1 - (int)(s * (float)(kLevels - 1) + 0.5f));   // highest share first
+        start[lvl[v] + 1]++;
+    }
+    for (int i = 0; i < kLevels; ++i) start[i + 1] += start[i];
+    order.resize((size_t)V);
+    for (int v = 0; v < V; ++v) order[start[lvl[v]]++] = v;
+    return true;
+}
+
 // LDS of one propagation block: heap levels 1..31 of both queues (8-B key
 // and 16-bit record index each) and the event counters, for every source
 // lane of the block.
@@ -1461,6 +1516,13 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     // few candidates, each tried once on successive builds.
     TuneRecord* tune = c.autotune ? c.tune : nullptr;
     std::vector<int32_t> order;
+    // the predicted order is computed on a host thread while the device
+    // memory is allocated; joined before the order is uploaded (or on return)
+    struct Joiner {
+        std::thread t;
+        ~Joiner() { if (t.joinable()) t.join(); }
+    } predictor;
+    bool predicted = false;
     int blockSlot = -1;
     if (tune) {
         if (c.propBlock <= 0 && c.lanesPerWave <= 0) {
@@ -1486,7 +1548,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     // one block's LDS stays within 64 KiB
     while (block > 64 && lds_bytes(block, lanesPerWave) > (64u << 10)) block /= 2;
     const int lanes = c.lanes > 0 ? std::min(c.lanes, V) : std::min(V, resident_lanes(block, lanesPerWave));
-    if (tune && (int)tune->cost.size() == V && lanes < V) {
+    st.sourceOrder = 1;
+    if (tune && (int)tune->cost.size() == V && lanes < V && c.sourceOrder == 0) {
         // Sources are taken from the costliest down (longest processing
         // time first): a lane runs about 1.25 sources on the torus, so a
         // costly source left to start late would set the end of the launch.
@@ -1495,6 +1558,9 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         auto dearer = [&](int x, int y) { return tune->cost[x] > tune->cost[y] || (tune->cost[x] == tune->cost[y] && x < y); };
         std::sort(byCost.begin(), byCost.end(), dearer);
         order = byCost;
+        st.sourceOrder = 3;
+    } else if ((c.sourceOrder == 0 && lanes < V && V <= kPredictPerLane * lanes) || c.sourceOrder == 2) {
+        predictor.t = std::thread([&h, K, &order, &predicted] { predicted = predicted_order(h, K, order); });
     }
     if (tune && (int)tune->cost.size() == V) st.tuned = 1;
     if (vcap < 64) vcap = 64;
@@ -1555,6 +1621,10 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         err = "batched build: out of device memory";
     }
     int32_t* dOrder = nullptr;
+    if (predictor.t.joinable()) {
+        predictor.t.join();
+        if (predicted) st.sourceOrder = 2;
+    }
     if (err.empty() && !order.empty()) {
         if (hipMalloc(&dOrder, sizeof(int32_t) * (size_t)V) != hipSuccess ||
             hipMemcpy(dOrder, order.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice) != hipSuccess)

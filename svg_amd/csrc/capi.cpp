@@ -160,6 +160,7 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     if (p->K <= 0) return svg_set_error(SVG_E_ARG, "K must be positive");
     if (p->mode != SVG_MODE_PLAIN && p->mode != SVG_MODE_BATCHED) return svg_set_error(SVG_E_ARG, "unknown mode");
     if (p->split_coef == 0 || p->vert_coef == 0) return svg_set_error(SVG_E_ARG, "table coefficients must be positive");
+    if (p->source_order < 0 || p->source_order > 2) return svg_set_error(SVG_E_ARG, "source_order must be 0, 1 or 2");
     int rc = svg_mesh_upload_impl(m, true);
     if (rc) return rc;
     BuildConfig c;
@@ -169,6 +170,7 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
     c.vertCap = p->vert_cap; c.splitCap = p->split_cap; c.autotune = p->autotune;
     c.lanesPerWave = p->lanes_per_wave;
     c.queueCap = p->queue_cap;
+    c.sourceOrder = p->source_order;
     if (c.plainThreads > 1024 || (c.plainThreads % 64) != 0 && c.plainThreads > 0)
         return svg_set_error(SVG_E_ARG, "plain_threads must be a multiple of 64 up to 1024");
     svg_graph_s* g = new (std::nothrow) svg_graph_s();
@@ -224,6 +226,7 @@ extern "C" int svg_build_stats_get(svg_graph_t g, svg_build_stats* st)
     for (size_t i = 0; i < s.counters.size() && i < 16; ++i) st->counters[i] = s.counters[i];
     st->lanes_per_wave = s.lanesPerWave; st->fallback_plain = s.fallbackPlain;
     st->queue_cap = s.queueCap; st->pseudo_queue_cap = s.pseudoCap;
+    st->source_order = s.sourceOrder;
     return SVG_OK;
 }
 

@@ -38,6 +38,7 @@ struct BuildConfig {
     int splitCap = 0;            // per-source split table slots (power of two)
     int lanesPerWave = 0;        // source lanes per 64-wide wave of the propagation kernel
     int queueCap = 0;            // window / pseudo queue storage per lane (0: the reference's 1023)
+    int sourceOrder = 0;         // 0 auto, 1 vertex id order, 2 predicted from the mesh's saddles
     int autotune = 0;
     TuneRecord* tune = nullptr;  // autotune: what earlier builds of this mesh recorded
 };
@@ -56,6 +57,7 @@ struct BuildStats {
     int lanesPerWave = 0;
     int queueCap = 0, pseudoCap = 0;
     int fallbackPlain = 0;       // 1: the mesh exceeds the batched form's limits; every source ran plain
+    int sourceOrder = 0;         // order the sources ran in: 1 id, 2 predicted, 3 recorded event counts
     std::vector<long long> counters;   // summed event counters (CNT_NUM)
     std::vector<uint32_t> status;      // per-vertex status bits (SF_*)
 };

@@ -191,6 +191,22 @@ def test_lanes_per_wave_do_not_change_rows(lpw):
     _assert_rows(g, rows, tails, K)
 
 
+@pytest.mark.parametrize("order,lanes", [(1, 256), (2, 256), (0, 256), (0, 0)])
+def test_source_order_does_not_change_rows(order, lanes):
+    # 1: vertex id order; 2: predicted from the saddle share around each
+    # vertex; 0 with fewer lanes than sources picks the prediction, and with a
+    # lane per source keeps the id order.  Every order runs each source once
+    # and builds the oracle's rows.
+    xyz, tri = TORUS()
+    K = 80
+    print(f"case source_order {order} lanes {lanes}", flush=True)
+    rows, tails, _ = _oracle(xyz, tri, K)
+    _, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED, source_order=order, lanes=lanes)
+    st = g.stats()
+    assert st["source_order"] == {1: 1, 2: 2, 0: 2 if lanes else 1}[order]
+    _assert_rows(g, rows, tails, K)
+
+
 def test_server_memory_admission():
     # The slot count follows the memory budget; a budget below one slot is
     # refused at start (SVG_E_MEM), a budget of three slots gives three.
