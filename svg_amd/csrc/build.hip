@@ -1408,6 +1408,27 @@ static unsigned next_pow2(unsigned x)
     return p;
 }
 
+// Sources by recorded event count, costliest first, vertex id breaking
+// ties: two stable counting passes over 16-bit digits of the inverted
+// count (O(V), where a comparison sort took milliseconds on the host).
+static void costliest_first(const std::vector<int32_t>& cost, std::vector<int32_t>& order)
+{
+    const size_t V = cost.size();
+    std::vector<int32_t> tmp(V);
+    order.resize(V);
+    for (size_t i = 0; i < V; ++i) tmp[i] = (int32_t)i;
+    std::vector<int32_t> start((1 << 16) + 1);
+    for (int shift = 0; shift < 32; shift += 16) {
+        std::fill(start.begin(), start.end(), 0);
+        auto digit = [&](int32_t v) { return (~(uint32_t)std::max(cost[v], 0) >> shift) & 0xFFFFu; };
+        for (size_t i = 0; i < V; ++i) start[digit(tmp[i]) + 1]++;
+        for (int d = 0; d < (1 << 16); ++d) start[d + 1] += start[d];
+        for (size_t i = 0; i < V; ++i) order[start[digit(tmp[i])]++] = tmp[i];
+        tmp.swap(order);
+    }
+    order.swap(tmp);
+}
+
 // Source order for a build with no recorded event counts.  A source's
 // event count grows with the saddle vertices among those it reaches: each
 // reached saddle becomes a pseudo-source that emits windows over its whole
@@ -1553,11 +1574,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         // Sources are taken from the costliest down (longest processing
         // time first): a lane runs about 1.25 sources on the torus, so a
         // costly source left to start late would set the end of the launch.
-        std::vector<int32_t> byCost((size_t)V);
-        for (int i = 0; i < V; ++i) byCost[i] = i;
-        auto dearer = [&](int x, int y) { return tune->cost[x] > tune->cost[y] || (tune->cost[x] == tune->cost[y] && x < y); };
-        std::sort(byCost.begin(), byCost.end(), dearer);
-        order = byCost;
+        costliest_first(tune->cost, order);
         st.sourceOrder = 3;
     } else if ((c.sourceOrder == 0 && lanes < V && V <= kPredictPerLane * lanes) || c.sourceOrder == 2) {
         predictor.t = std::thread([&h, K, &order, &predicted] { predicted = predicted_order(h, K, order); });
