@@ -191,19 +191,23 @@ def test_lanes_per_wave_do_not_change_rows(lpw):
     _assert_rows(g, rows, tails, K)
 
 
-@pytest.mark.parametrize("order,lanes", [(1, 256), (2, 256), (0, 256), (0, 0)])
-def test_source_order_does_not_change_rows(order, lanes):
+@pytest.mark.parametrize("mesh,order,lanes,used", [
+    ("torus", 1, 256, 1), ("torus", 2, 256, 2), ("torus", 0, 256, 2), ("torus", 0, 0, 1),
+    ("heightfield", 0, 1024, 2), ("convex", 2, 128, 1)])
+def test_source_order_does_not_change_rows(mesh, order, lanes, used):
     # 1: vertex id order; 2: predicted from the saddle share around each
-    # vertex; 0 with fewer lanes than sources picks the prediction, and with a
-    # lane per source keeps the id order.  Every order runs each source once
-    # and builds the oracle's rows.
-    xyz, tri = TORUS()
-    K = 80
-    print(f"case source_order {order} lanes {lanes}", flush=True)
+    # vertex; 0 with at most four sources per lane picks the prediction, and
+    # with a lane per source keeps the id order.  The open heightfield (4,096
+    # sources on 1,024 lanes) takes the prediction; an icosphere without
+    # noise has no saddle vertex, so its prediction keeps the id order.
+    # Every order runs each source once and builds the oracle's rows.
+    xyz, tri = {"torus": TORUS, "heightfield": lambda: mg.heightfield64(),
+                "convex": lambda: mg.noisy_sphere(splits4=3, centroid_split=False, sigma=0.0)}[mesh]()
+    K = 50 if mesh == "heightfield" else 80
+    print(f"case source_order {mesh} order {order} lanes {lanes}", flush=True)
     rows, tails, _ = _oracle(xyz, tri, K)
     _, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED, source_order=order, lanes=lanes)
-    st = g.stats()
-    assert st["source_order"] == {1: 1, 2: 2, 0: 2 if lanes else 1}[order]
+    assert g.stats()["source_order"] == used
     _assert_rows(g, rows, tails, K)
 
 
