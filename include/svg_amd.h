@@ -25,6 +25,7 @@ extern "C" {
 #define SVG_E_IO -4         /* file error */
 #define SVG_E_STATE -5      /* handle in the wrong state */
 #define SVG_E_BUSY -6       /* serving: admission refused (memory / queue full) */
+#define SVG_E_INTERNAL -7   /* host-side failure inside the library (message in svg_last_error) */
 
 #define SVG_MODE_PLAIN 0    /* one source per thread, reference layout (SVG.cu:9-96) */
 #define SVG_MODE_BATCHED 1  /* persistent work-stealing lanes + wave extraction */

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -1577,7 +1578,18 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         costliest_first(tune->cost, order);
         st.sourceOrder = 3;
     } else if ((c.sourceOrder == 0 && lanes < V && V <= kPredictPerLane * lanes) || c.sourceOrder == 2) {
-        predictor.t = std::thread([&h, K, &order, &predicted] { predicted = predicted_order(h, K, order); });
+        try {
+            predictor.t = std::thread([&h, K, &order, &predicted] {
+                try {
+                    predicted = predicted_order(h, K, order);
+                } catch (...) {   // no memory for the prediction: id order
+                    predicted = false;
+                    order.clear();
+                }
+            });
+        } catch (const std::system_error&) {
+            // no thread to be had: the sources keep the id order
+        }
     }
     if (tune && (int)tune->cost.size() == V) st.tuned = 1;
     if (vcap < 64) vcap = 64;
@@ -1638,10 +1650,8 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         err = "batched build: out of device memory";
     }
     int32_t* dOrder = nullptr;
-    if (predictor.t.joinable()) {
-        predictor.t.join();
-        if (predicted) st.sourceOrder = 2;
-    }
+    if (predictor.t.joinable()) predictor.t.join();
+    if (predicted) st.sourceOrder = 2;
     if (err.empty() && !order.empty()) {
         if (hipMalloc(&dOrder, sizeof(int32_t) * (size_t)V) != hipSuccess ||
             hipMemcpy(dOrder, order.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice) != hipSuccess)

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -180,7 +181,7 @@ bool least_loaded(const svg_server_s* S, const Group* G, int mine)
 // the slot records of batch k-1 (complete by then) while batch k runs.  A
 // request admitted between two batches joins the next one; its record is
 // trusted from the first batch queued after its admission.
-static void worker_loop(svg_server_s* S, Group* Gp)
+static void worker_body(svg_server_s* S, Group* Gp)
 {
     Group& G = *Gp;
     svg_ensure_device();
@@ -295,6 +296,19 @@ static void worker_loop(svg_server_s* S, Group* Gp)
     }
 }
 
+// A worker never lets an exception end the process: it fails the server's
+// requests (their waiters and callbacks see the error) and stops.
+static void worker_loop(svg_server_s* S, Group* Gp)
+{
+    try {
+        worker_body(S, Gp);
+    } catch (const std::exception& e) {
+        try { fail_all(S, std::string("server worker: ") + e.what()); } catch (...) {}
+    } catch (...) {
+        try { fail_all(S, "server worker failed"); } catch (...) {}
+    }
+}
+
 static void group_free(Group& G)
 {
     solver_destroy(G.st);
@@ -317,62 +331,77 @@ static void server_free(svg_server_s* S)
 
 extern "C" int svg_server_start(svg_graph_t g, const svg_server_cfg* cfg, svg_server_t* out)
 {
-    if (!g || !out) return svg_set_error(SVG_E_ARG, "null argument");
-    int rc = svg_ensure_device();
-    if (rc) return rc;
-    svg_server_s* S = new (std::nothrow) svg_server_s();
-    if (!S) return svg_set_error(SVG_E_MEM, "host allocation failed");
-    S->g = g;
-    int slots = cfg && cfg->max_inflight > 0 ? cfg->max_inflight : 64;
-    const size_t perSlot = solver_bytes_per_slot(g->d.V);
-    size_t freeB = 0, totB = 0;
-    if (hipMemGetInfo(&freeB, &totB) != hipSuccess) { delete S; return svg_set_error(SVG_E_HIP, "hipMemGetInfo failed"); }
-    size_t budget = cfg && cfg->mem_budget > 0 ? (size_t)cfg->mem_budget : freeB / 4;
-    const int memSlots = (int)std::min<size_t>(4096, budget / perSlot);
-    if (memSlots < 1) { delete S; return svg_set_error(SVG_E_MEM, "memory budget below one request slot"); }
-    slots = std::min(slots, memSlots);
-    const int nGroups = std::max(1, std::min(cfg && cfg->streams > 0 ? cfg->streams : 4, std::min(slots, 8)));
-    S->slots = slots;
-    S->slotBytes = perSlot;
-    S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
-    S->queueCap = cfg && cfg->queue_cap > 0 ? (size_t)cfg->queue_cap : (size_t)std::max(1024, 64 * slots);
-    const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
-    const double delta = graph_mean_arc(g->d) * scale;
-    for (int gi = 0; gi < nGroups; ++gi) {
-        S->groups.emplace_back(new Group());
-        Group& G = *S->groups.back();
-        G.slots = slots / nGroups + (gi < slots % nGroups ? 1 : 0);
-        std::string e;
-        if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) e = "stream creation failed";
-        G.st.stream = G.stream;
-        if (e.empty()) e = solver_create(g->d, g->mesh->d, G.slots, delta, G.st);
-        if (!e.empty()) { server_free(S); return svg_set_error(SVG_E_MEM, e); }
-        if (cfg && cfg->rounds_per_sync > 0) G.st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
-        // the groups' expand passes share the device: together about two
-        // machine-fills of waves
-        G.st.expandBlocks = std::max(192, std::min(768, 1536 / nGroups));
-        if (cfg && cfg->chain_rows != 0) G.st.chainBudget = std::max(0, std::min(cfg->chain_rows, 64));
-        if (cfg && cfg->msad_delta_scale > 0.0) G.st.deltaMsad = delta / scale * cfg->msad_delta_scale;
-        bool ok = true;
-        for (int i = 0; i < 2 && ok; ++i) {
-            ok = hipHostMalloc(&G.hs[i], sizeof(SlotParam) * (size_t)(G.slots + 1), hipHostMallocDefault) == hipSuccess &&
-                 hipEventCreateWithFlags(&G.batchEv[i], hipEventDisableTiming) == hipSuccess;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !out) return svg_set_error(SVG_E_ARG, "null argument");
+        int rc = svg_ensure_device();
+        if (rc) return rc;
+        svg_server_s* S = new (std::nothrow) svg_server_s();
+        if (!S) return svg_set_error(SVG_E_MEM, "host allocation failed");
+        S->g = g;
+        int slots = cfg && cfg->max_inflight > 0 ? cfg->max_inflight : 64;
+        const size_t perSlot = solver_bytes_per_slot(g->d.V);
+        size_t freeB = 0, totB = 0;
+        if (hipMemGetInfo(&freeB, &totB) != hipSuccess) { delete S; return svg_set_error(SVG_E_HIP, "hipMemGetInfo failed"); }
+        size_t budget = cfg && cfg->mem_budget > 0 ? (size_t)cfg->mem_budget : freeB / 4;
+        const int memSlots = (int)std::min<size_t>(4096, budget / perSlot);
+        if (memSlots < 1) { delete S; return svg_set_error(SVG_E_MEM, "memory budget below one request slot"); }
+        slots = std::min(slots, memSlots);
+        const int nGroups = std::max(1, std::min(cfg && cfg->streams > 0 ? cfg->streams : 4, std::min(slots, 8)));
+        S->slots = slots;
+        S->slotBytes = perSlot;
+        S->maxBatch = cfg && cfg->max_batch > 0 ? cfg->max_batch : slots;
+        S->queueCap = cfg && cfg->queue_cap > 0 ? (size_t)cfg->queue_cap : (size_t)std::max(1024, 64 * slots);
+        const double scale = cfg && cfg->delta_scale > 0.0 ? cfg->delta_scale : 1.0;
+        const double delta = graph_mean_arc(g->d) * scale;
+        for (int gi = 0; gi < nGroups; ++gi) {
+            S->groups.emplace_back(new Group());
+            Group& G = *S->groups.back();
+            G.slots = slots / nGroups + (gi < slots % nGroups ? 1 : 0);
+            std::string e;
+            if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) e = "stream creation failed";
+            G.st.stream = G.stream;
+            if (e.empty()) e = solver_create(g->d, g->mesh->d, G.slots, delta, G.st);
+            if (!e.empty()) { server_free(S); return svg_set_error(SVG_E_MEM, e); }
+            if (cfg && cfg->rounds_per_sync > 0) G.st.roundsPerSync = std::min(cfg->rounds_per_sync, 64);
+            // the groups' expand passes share the device: together about two
+            // machine-fills of waves
+            G.st.expandBlocks = std::max(192, std::min(768, 1536 / nGroups));
+            if (cfg && cfg->chain_rows != 0) G.st.chainBudget = std::max(0, std::min(cfg->chain_rows, 64));
+            if (cfg && cfg->msad_delta_scale > 0.0) G.st.deltaMsad = delta / scale * cfg->msad_delta_scale;
+            bool ok = true;
+            for (int i = 0; i < 2 && ok; ++i) {
+                ok = hipHostMalloc(&G.hs[i], sizeof(SlotParam) * (size_t)(G.slots + 1), hipHostMallocDefault) == hipSuccess &&
+                     hipEventCreateWithFlags(&G.batchEv[i], hipEventDisableTiming) == hipSuccess;
+            }
+            ok = ok && hipStreamCreateWithFlags(&G.fetchStream, hipStreamNonBlocking) == hipSuccess;
+            G.fetchEv.assign((size_t)G.slots, nullptr);
+            for (int q = 0; q < G.slots && ok; ++q)
+                ok = hipEventCreateWithFlags(&G.fetchEv[q], hipEventDisableTiming) == hipSuccess;
+            ok = ok && hipHostMalloc(&G.fDist, sizeof(double) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc(&G.fSrc, sizeof(int32_t) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess;
+            if (!ok) {
+                server_free(S);
+                return svg_set_error(SVG_E_MEM, "server staging allocation failed");
+            }
+            G.slotTicket.assign((size_t)G.slots, nullptr);
         }
-        ok = ok && hipStreamCreateWithFlags(&G.fetchStream, hipStreamNonBlocking) == hipSuccess;
-        G.fetchEv.assign((size_t)G.slots, nullptr);
-        for (int q = 0; q < G.slots && ok; ++q)
-            ok = hipEventCreateWithFlags(&G.fetchEv[q], hipEventDisableTiming) == hipSuccess;
-        ok = ok && hipHostMalloc(&G.fDist, sizeof(double) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc(&G.fSrc, sizeof(int32_t) * (size_t)G.slots * g->d.V, hipHostMallocDefault) == hipSuccess;
-        if (!ok) {
+        try {
+            for (auto& G : S->groups) G->worker = std::thread(worker_loop, S, G.get());
+        } catch (const std::system_error& e) {
+            // the workers already started end before the server is freed
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->stopping = true;
+            }
+            S->cvWork.notify_all();
+            for (auto& G : S->groups)
+                if (G->worker.joinable()) G->worker.join();
             server_free(S);
-            return svg_set_error(SVG_E_MEM, "server staging allocation failed");
+            return svg_set_error(SVG_E_INTERNAL, std::string("server worker start: ") + e.what());
         }
-        G.slotTicket.assign((size_t)G.slots, nullptr);
-    }
-    for (auto& G : S->groups) G->worker = std::thread(worker_loop, S, G.get());
-    *out = S;
-    return SVG_OK;
+        *out = S;
+        return SVG_OK;
+    });
 }
 
 static long long submit_impl(svg_server_t S, const svg_request* r, svg_done_fn cb, void* user)
@@ -410,13 +439,17 @@ static long long submit_impl(svg_server_t S, const svg_request* r, svg_done_fn c
 
 extern "C" long long svg_server_submit(svg_server_t S, const svg_request* r)
 {
-    return submit_impl(S, r, nullptr, nullptr);
+    return svg_guarded<long long>([&]() -> long long {
+        return submit_impl(S, r, nullptr, nullptr);
+    });
 }
 
 extern "C" long long svg_server_submit_cb(svg_server_t S, const svg_request* r, svg_done_fn cb, void* user)
 {
-    if (!cb) return svg_set_error(SVG_E_ARG, "null callback");
-    return submit_impl(S, r, cb, user);
+    return svg_guarded<long long>([&]() -> long long {
+        if (!cb) return svg_set_error(SVG_E_ARG, "null callback");
+        return submit_impl(S, r, cb, user);
+    });
 }
 
 // Waits for a ticket without a callback and frees it; *done receives its
@@ -439,12 +472,14 @@ static int wait_ticket(svg_server_t S, long long ticket, Clock::time_point* subm
 
 extern "C" int svg_server_wait(svg_server_t S, long long ticket, double* latency_ms)
 {
-    if (!S) return svg_set_error(SVG_E_ARG, "null server");
-    Clock::time_point a, b;
-    const int rc = wait_ticket(S, ticket, &a, &b);
-    if (rc) return rc;
-    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(b - a).count();
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!S) return svg_set_error(SVG_E_ARG, "null server");
+        Clock::time_point a, b;
+        const int rc = wait_ticket(S, ticket, &a, &b);
+        if (rc) return rc;
+        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(b - a).count();
+        return SVG_OK;
+    });
 }
 
 static double pct(std::vector<double> v, double p)
@@ -459,67 +494,73 @@ static double pct(std::vector<double> v, double p)
 
 extern "C" int svg_server_stats_get(svg_server_t S, svg_server_stats* st)
 {
-    if (!S || !st) return svg_set_error(SVG_E_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(S->mu);
-    std::memset(st, 0, sizeof(*st));
-    st->completed = (long long)S->lat.size();
-    st->refused = S->refused;
-    st->p50_ms = pct(S->lat, 0.50);
-    st->p99_ms = pct(S->lat, 0.99);
-    double sum = 0.0, mx = 0.0;
-    for (double x : S->lat) { sum += x; mx = std::max(mx, x); }
-    st->mean_ms = S->lat.empty() ? 0.0 : sum / (double)S->lat.size();
-    st->max_ms = mx;
-    const double span = S->anySubmit ? std::chrono::duration<double>(S->lastDone - S->firstSubmit).count() : 0.0;
-    st->qps = span > 0 ? (double)S->lat.size() / span : 0.0;
-    st->max_batch_seen = S->maxBatchSeen;
-    st->slots = S->slots;
-    st->slot_bytes = (long long)S->slotBytes;
-    st->launches = 0;
-    for (const auto& G : S->groups) st->launches += G->launches.load(std::memory_order_relaxed);
-    for (int k = 0; k < 2; ++k) {
-        st->n_kind[k] = S->nKind[k];
-        st->rounds_kind[k] = S->roundsKind[k];
-        st->settled_kind[k] = S->settledKind[k];
-        st->relaxed_kind[k] = S->relaxedKind[k];
-    }
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!S || !st) return svg_set_error(SVG_E_ARG, "null argument");
+        std::lock_guard<std::mutex> lk(S->mu);
+        std::memset(st, 0, sizeof(*st));
+        st->completed = (long long)S->lat.size();
+        st->refused = S->refused;
+        st->p50_ms = pct(S->lat, 0.50);
+        st->p99_ms = pct(S->lat, 0.99);
+        double sum = 0.0, mx = 0.0;
+        for (double x : S->lat) { sum += x; mx = std::max(mx, x); }
+        st->mean_ms = S->lat.empty() ? 0.0 : sum / (double)S->lat.size();
+        st->max_ms = mx;
+        const double span = S->anySubmit ? std::chrono::duration<double>(S->lastDone - S->firstSubmit).count() : 0.0;
+        st->qps = span > 0 ? (double)S->lat.size() / span : 0.0;
+        st->max_batch_seen = S->maxBatchSeen;
+        st->slots = S->slots;
+        st->slot_bytes = (long long)S->slotBytes;
+        st->launches = 0;
+        for (const auto& G : S->groups) st->launches += G->launches.load(std::memory_order_relaxed);
+        for (int k = 0; k < 2; ++k) {
+            st->n_kind[k] = S->nKind[k];
+            st->rounds_kind[k] = S->roundsKind[k];
+            st->settled_kind[k] = S->settledKind[k];
+            st->relaxed_kind[k] = S->relaxedKind[k];
+        }
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_server_stop(svg_server_t S)
 {
-    if (!S) return SVG_OK;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->stopping = true;
-    }
-    S->cvWork.notify_all();
-    for (auto& G : S->groups)
-        if (G->worker.joinable()) G->worker.join();
-    svg_ensure_device();
-    server_free(S);
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!S) return SVG_OK;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stopping = true;
+        }
+        S->cvWork.notify_all();
+        for (auto& G : S->groups)
+            if (G->worker.joinable()) G->worker.join();
+        svg_ensure_device();
+        server_free(S);
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_server_replay(svg_server_t S, const svg_request* reqs, const double* arrival_s, int n,
                                  double* latency_ms)
 {
-    if (!S || (!reqs && n > 0) || (!arrival_s && n > 0)) return svg_set_error(SVG_E_ARG, "null argument");
-    std::vector<long long> ids((size_t)n, -1);
-    std::vector<Clock::time_point> sched((size_t)n);
-    const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(5);
-    for (int i = 0; i < n; ++i) {
-        sched[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
-        std::this_thread::sleep_until(sched[i]);
-        const long long id = svg_server_submit(S, &reqs[i]);
-        ids[i] = id;
-    }
-    for (int i = 0; i < n; ++i) {
-        if (ids[i] < 0) { if (latency_ms) latency_ms[i] = -1.0; continue; }
-        Clock::time_point done;
-        int rc = wait_ticket(S, ids[i], nullptr, &done);
-        if (rc) return rc;
-        if (latency_ms) latency_ms[i] = std::chrono::duration<double, std::milli>(done - sched[i]).count();
-    }
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!S || (!reqs && n > 0) || (!arrival_s && n > 0)) return svg_set_error(SVG_E_ARG, "null argument");
+        std::vector<long long> ids((size_t)n, -1);
+        std::vector<Clock::time_point> sched((size_t)n);
+        const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(5);
+        for (int i = 0; i < n; ++i) {
+            sched[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+            std::this_thread::sleep_until(sched[i]);
+            const long long id = svg_server_submit(S, &reqs[i]);
+            ids[i] = id;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (ids[i] < 0) { if (latency_ms) latency_ms[i] = -1.0; continue; }
+            Clock::time_point done;
+            int rc = wait_ticket(S, ids[i], nullptr, &done);
+            if (rc) return rc;
+            if (latency_ms) latency_ms[i] = std::chrono::duration<double, std::milli>(done - sched[i]).count();
+        }
+        return SVG_OK;
+    });
 }

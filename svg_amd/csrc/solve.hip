@@ -1255,55 +1255,59 @@ static int run_to_completion(svg_graph_s* g, SolverState* s)
 
 extern "C" int svg_sssd(svg_graph_t g, int s, int t, double* dist_t, int32_t* path, int path_cap, int* path_len)
 {
-    if (!g || !dist_t) return svg_set_error(SVG_E_ARG, "null argument");
-    if (s < 0 || t < 0 || s >= g->d.V || t >= g->d.V) return svg_set_error(SVG_E_ARG, "vertex out of range");
-    std::lock_guard<std::mutex> lk(g->mu);
-    SolverState* st = nullptr;
-    int rc = get_solver(g, 1, &st);
-    if (rc) return rc;
-    const double w0 = 0.0;
-    const bool wantPath = path && path_cap > 0;
-    std::string e = solver_admit(*st, g->mesh->d, 0, Q_SSSD, &s, &w0, 1, t, wantPath);
-    if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
-    rc = run_to_completion(g, st);
-    if (rc) return rc;
-    std::vector<int32_t> par;
-    if (wantPath) par.resize((size_t)g->d.V);
-    e = solver_fetch(*st, 0, nullptr, dist_t, nullptr, wantPath ? par.data() : nullptr);
-    solver_release_slot(*st, 0);
-    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-    if (wantPath) {
-        int n = 0, u = t;
-        if (*dist_t < DBL_MAX) {
-            while (u != -1 && n < path_cap) {
-                path[n++] = u;
-                if (u == s) break;
-                u = par[u];
-                if (u == INT_MAX) { u = -1; }
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !dist_t) return svg_set_error(SVG_E_ARG, "null argument");
+        if (s < 0 || t < 0 || s >= g->d.V || t >= g->d.V) return svg_set_error(SVG_E_ARG, "vertex out of range");
+        std::lock_guard<std::mutex> lk(g->mu);
+        SolverState* st = nullptr;
+        int rc = get_solver(g, 1, &st);
+        if (rc) return rc;
+        const double w0 = 0.0;
+        const bool wantPath = path && path_cap > 0;
+        std::string e = solver_admit(*st, g->mesh->d, 0, Q_SSSD, &s, &w0, 1, t, wantPath);
+        if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
+        rc = run_to_completion(g, st);
+        if (rc) return rc;
+        std::vector<int32_t> par;
+        if (wantPath) par.resize((size_t)g->d.V);
+        e = solver_fetch(*st, 0, nullptr, dist_t, nullptr, wantPath ? par.data() : nullptr);
+        solver_release_slot(*st, 0);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        if (wantPath) {
+            int n = 0, u = t;
+            if (*dist_t < DBL_MAX) {
+                while (u != -1 && n < path_cap) {
+                    path[n++] = u;
+                    if (u == s) break;
+                    u = par[u];
+                    if (u == INT_MAX) { u = -1; }
+                }
             }
+            if (path_len) *path_len = n;
         }
-        if (path_len) *path_len = n;
-    }
-    return SVG_OK;
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_msad(svg_graph_t g, const int32_t* src, const double* w, int ns, double* dist, int32_t* src_id,
                         int32_t* parent)
 {
-    if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
-    std::lock_guard<std::mutex> lk(g->mu);
-    SolverState* st = nullptr;
-    int rc = get_solver(g, 1, &st);
-    if (rc) return rc;
-    const bool wantParent = src_id || parent;
-    std::string e = solver_admit(*st, g->mesh->d, 0, Q_MSAD, src, w, ns, -1, wantParent);
-    if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
-    rc = run_to_completion(g, st);
-    if (rc) return rc;
-    e = solver_fetch(*st, 0, dist, nullptr, src_id, parent);
-    solver_release_slot(*st, 0);
-    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        std::lock_guard<std::mutex> lk(g->mu);
+        SolverState* st = nullptr;
+        int rc = get_solver(g, 1, &st);
+        if (rc) return rc;
+        const bool wantParent = src_id || parent;
+        std::string e = solver_admit(*st, g->mesh->d, 0, Q_MSAD, src, w, ns, -1, wantParent);
+        if (!e.empty()) return svg_set_error(SVG_E_ARG, e);
+        rc = run_to_completion(g, st);
+        if (rc) return rc;
+        e = solver_fetch(*st, 0, dist, nullptr, src_id, parent);
+        solver_release_slot(*st, 0);
+        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+        return SVG_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,110 +1400,112 @@ __global__ void k_noalloc_reset(SolveCounts* cnt, SlotAcc* acc0, SlotAcc* acc1, 
 extern "C" int svg_msad_noalloc(svg_graph_t g, const int32_t* src, const double* w, int ns, double* dist,
                                 int32_t* src_id, int32_t* parent)
 {
-    if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
-    const int V = g->d.V;
-    for (int i = 0; i < ns; ++i)
-        if (src[i] < 0 || src[i] >= V) return svg_set_error(SVG_E_ARG, "source out of range");
-    if (g->d.K > 0xFFFF) return svg_set_error(SVG_E_ARG, "K above 65535");
-    const NoAllocLayout L = noalloc_layout(V, ns);
-    if (L.total > kNoAllocBytes)
-        return svg_set_error(SVG_E_ARG, "graph too large for svg_msad_noalloc: it needs " + std::to_string(L.total) +
-                                            " bytes of its " + std::to_string(kNoAllocBytes) +
-                                            "-byte device array (about 87,000 vertices at most)");
-    int rc = svg_ensure_device();
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk(g_scratchMu);
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices)
-        return svg_set_error(SVG_E_HIP, "no current device");
-    if (!g_scratchDev[dev]) {
-        void* p = nullptr;
-        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-        if (e != hipSuccess || !p)
-            return svg_set_error(SVG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
-        g_scratchDev[dev] = p;
-    }
-    char* base = static_cast<char*>(g_scratchDev[dev]);
-    SolverState s;
-    s.V = V; s.K = g->d.K; s.S = 1; s.stream = 0;
-    s.dist = (unsigned long long*)(base + L.off[0]);
-    s.parent = (int32_t*)(base + L.off[1]);
-    s.stamp = (uint32_t*)(base + L.off[2]);
-    s.inq = (uint32_t*)(base + L.off[3]);
-    s.srcid = (int32_t*)(base + L.off[4]);
-    s.hdist = (float*)(base + L.off[5]);
-    s.dl[0] = (uint64_t*)(base + L.off[6]); s.dl[1] = (uint64_t*)(base + L.off[7]);
-    s.pl[0] = (uint64_t*)(base + L.off[8]); s.pl[1] = (uint64_t*)(base + L.off[9]);
-    s.exp[0] = (ExpEnt*)(base + L.off[10]); s.exp[1] = (ExpEnt*)(base + L.off[11]);
-    s.cnt = (SolveCounts*)(base + L.off[12]);
-    s.won[0] = (WonEnt*)(base + L.off[13]); s.won[1] = (WonEnt*)(base + L.off[14]);
-    s.acc[0] = (SlotAcc*)(base + L.off[15]); s.acc[1] = (SlotAcc*)(base + L.off[16]);
-    s.thr = (double*)(base + L.off[17]);
-    s.doneP = (int*)(base + L.off[18]);
-    s.sp = (SlotParam*)(base + L.off[19]);
-    s.ovDev = &s.sp[1].kind;
-    s.frontCap = 2 * (size_t)V + 65536;
-    s.pCap = (size_t)V + 65536;
-    s.wonCap = std::min<size_t>((size_t)V / 4 + 65536, 0x7ffffff0ull);
-    int* stage = (int*)(base + L.off[20]);
-    const size_t intBytes = (((size_t)7 + (size_t)ns + 2) * sizeof(int) + 15) / 16 * 16;
-    double* stageW = (double*)((char*)stage + intBytes);
-    double* msum = (double*)(base + L.off[21]);
-    unsigned long long* mcnt = (unsigned long long*)(msum + 1);
-    // host copy of the slot record and overflow flag (no pinned memory)
-    SlotParam spHost[2];
-    std::memset(spHost, 0, sizeof(spHost));
-    spHost[0].kind = -1; spHost[0].done = 1; spHost[1].kind = 0;
-    s.spHost = spHost;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        const int V = g->d.V;
+        for (int i = 0; i < ns; ++i)
+            if (src[i] < 0 || src[i] >= V) return svg_set_error(SVG_E_ARG, "source out of range");
+        if (g->d.K > 0xFFFF) return svg_set_error(SVG_E_ARG, "K above 65535");
+        const NoAllocLayout L = noalloc_layout(V, ns);
+        if (L.total > kNoAllocBytes)
+            return svg_set_error(SVG_E_ARG, "graph too large for svg_msad_noalloc: it needs " + std::to_string(L.total) +
+                                                " bytes of its " + std::to_string(kNoAllocBytes) +
+                                                "-byte device array (about 87,000 vertices at most)");
+        int rc = svg_ensure_device();
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices)
+            return svg_set_error(SVG_E_HIP, "no current device");
+        if (!g_scratchDev[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return svg_set_error(SVG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            g_scratchDev[dev] = p;
+        }
+        char* base = static_cast<char*>(g_scratchDev[dev]);
+        SolverState s;
+        s.V = V; s.K = g->d.K; s.S = 1; s.stream = 0;
+        s.dist = (unsigned long long*)(base + L.off[0]);
+        s.parent = (int32_t*)(base + L.off[1]);
+        s.stamp = (uint32_t*)(base + L.off[2]);
+        s.inq = (uint32_t*)(base + L.off[3]);
+        s.srcid = (int32_t*)(base + L.off[4]);
+        s.hdist = (float*)(base + L.off[5]);
+        s.dl[0] = (uint64_t*)(base + L.off[6]); s.dl[1] = (uint64_t*)(base + L.off[7]);
+        s.pl[0] = (uint64_t*)(base + L.off[8]); s.pl[1] = (uint64_t*)(base + L.off[9]);
+        s.exp[0] = (ExpEnt*)(base + L.off[10]); s.exp[1] = (ExpEnt*)(base + L.off[11]);
+        s.cnt = (SolveCounts*)(base + L.off[12]);
+        s.won[0] = (WonEnt*)(base + L.off[13]); s.won[1] = (WonEnt*)(base + L.off[14]);
+        s.acc[0] = (SlotAcc*)(base + L.off[15]); s.acc[1] = (SlotAcc*)(base + L.off[16]);
+        s.thr = (double*)(base + L.off[17]);
+        s.doneP = (int*)(base + L.off[18]);
+        s.sp = (SlotParam*)(base + L.off[19]);
+        s.ovDev = &s.sp[1].kind;
+        s.frontCap = 2 * (size_t)V + 65536;
+        s.pCap = (size_t)V + 65536;
+        s.wonCap = std::min<size_t>((size_t)V / 4 + 65536, 0x7ffffff0ull);
+        int* stage = (int*)(base + L.off[20]);
+        const size_t intBytes = (((size_t)7 + (size_t)ns + 2) * sizeof(int) + 15) / 16 * 16;
+        double* stageW = (double*)((char*)stage + intBytes);
+        double* msum = (double*)(base + L.off[21]);
+        unsigned long long* mcnt = (unsigned long long*)(msum + 1);
+        // host copy of the slot record and overflow flag (no pinned memory)
+        SlotParam spHost[2];
+        std::memset(spHost, 0, sizeof(spHost));
+        spHost[0].kind = -1; spHost[0].done = 1; spHost[1].kind = 0;
+        s.spHost = spHost;
 
-    hipLaunchKernelGGL(k_noalloc_reset, dim3(1), dim3(256), 0, 0, s.cnt, s.acc[0], s.acc[1], s.thr, s.doneP, s.sp,
-                       msum, mcnt);
-    // bucket width: the graph's mean arc length, as graph_mean_arc computes it
-    hipLaunchKernelGGL(k_mean_arc, dim3(256), dim3(256), 0, 0, g->d.tail, g->d.w, V, g->d.K, msum, mcnt);
-    double msumH = 0.0;
-    unsigned long long mcntH = 0;
-    hipError_t he = hipMemcpy(&msumH, msum, sizeof(double), hipMemcpyDeviceToHost);
-    if (he == hipSuccess) he = hipMemcpy(&mcntH, mcnt, sizeof(mcntH), hipMemcpyDeviceToHost);
-    if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc set-up: ") + hipGetErrorString(he));
-    s.delta = s.deltaMsad = mcntH ? msumH / (double)mcntH : 1.0;
-    // admission staging, laid out as solver_admit_many lays it out for one
-    // slot: slot, kind, target, wantParent, gen, off[2], seeds, xoff[2]; weights
-    const int wantParent = (src_id || parent) ? 1 : 0;
-    const int head[7] = {0, Q_MSAD, -1, wantParent, 1, 0, ns};
-    const int tailOff[2] = {0, 0};
-    he = hipMemcpy(stage, head, sizeof(head), hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(stage + 7, src, sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(stage + 7 + ns, tailOff, sizeof(tailOff), hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(stageW, w, sizeof(double) * (size_t)ns, hipMemcpyHostToDevice);
-    if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc seeds: ") + hipGetErrorString(he));
-    const DevMesh& m = g->mesh->d;
-    const int* dSlots = stage;
-    const int* dKinds = stage + 1;
-    const int* dTargets = stage + 2;
-    const int* dWant = stage + 3;
-    const int* dGens = stage + 4;
-    const int* dOff = stage + 5;
-    const int* dSeeds = stage + 7;
-    const int* dXoff = stage + 7 + ns;
-    hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, ((size_t)V + 255) / 256)), dim3(256), 0, 0,
-                       s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, 1, V);
-    {
-        DevGraph none;
-        SolveArgs args = make_args(s, none, m);
-        hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
-                           dSeeds, stageW, s.delta, s.deltaMsad, dXoff, dXoff, stageW, s.hdist);
-    }
-    spHost[0].kind = Q_MSAD; spHost[0].target = -1; spHost[0].done = 0; spHost[0].wantParent = wantParent;
-    spHost[0].gen = 1;
-    for (;;) {
-        int running = 0;
-        std::string e = solver_round(s, g->d, m, running);
+        hipLaunchKernelGGL(k_noalloc_reset, dim3(1), dim3(256), 0, 0, s.cnt, s.acc[0], s.acc[1], s.thr, s.doneP, s.sp,
+                           msum, mcnt);
+        // bucket width: the graph's mean arc length, as graph_mean_arc computes it
+        hipLaunchKernelGGL(k_mean_arc, dim3(256), dim3(256), 0, 0, g->d.tail, g->d.w, V, g->d.K, msum, mcnt);
+        double msumH = 0.0;
+        unsigned long long mcntH = 0;
+        hipError_t he = hipMemcpy(&msumH, msum, sizeof(double), hipMemcpyDeviceToHost);
+        if (he == hipSuccess) he = hipMemcpy(&mcntH, mcnt, sizeof(mcntH), hipMemcpyDeviceToHost);
+        if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc set-up: ") + hipGetErrorString(he));
+        s.delta = s.deltaMsad = mcntH ? msumH / (double)mcntH : 1.0;
+        // admission staging, laid out as solver_admit_many lays it out for one
+        // slot: slot, kind, target, wantParent, gen, off[2], seeds, xoff[2]; weights
+        const int wantParent = (src_id || parent) ? 1 : 0;
+        const int head[7] = {0, Q_MSAD, -1, wantParent, 1, 0, ns};
+        const int tailOff[2] = {0, 0};
+        he = hipMemcpy(stage, head, sizeof(head), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(stage + 7, src, sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(stage + 7 + ns, tailOff, sizeof(tailOff), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(stageW, w, sizeof(double) * (size_t)ns, hipMemcpyHostToDevice);
+        if (he != hipSuccess) return svg_set_error(SVG_E_HIP, std::string("svg_msad_noalloc seeds: ") + hipGetErrorString(he));
+        const DevMesh& m = g->mesh->d;
+        const int* dSlots = stage;
+        const int* dKinds = stage + 1;
+        const int* dTargets = stage + 2;
+        const int* dWant = stage + 3;
+        const int* dGens = stage + 4;
+        const int* dOff = stage + 5;
+        const int* dSeeds = stage + 7;
+        const int* dXoff = stage + 7 + ns;
+        hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::min<size_t>(2048, ((size_t)V + 255) / 256)), dim3(256), 0, 0,
+                           s.dist, s.parent, s.stamp, s.inq, s.srcid, s.hdist, m.pos, dSlots, dTargets, 1, V);
+        {
+            DevGraph none;
+            SolveArgs args = make_args(s, none, m);
+            hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, 0, args, dSlots, dKinds, dTargets, dWant, dGens, dOff,
+                               dSeeds, stageW, s.delta, s.deltaMsad, dXoff, dXoff, stageW, s.hdist);
+        }
+        spHost[0].kind = Q_MSAD; spHost[0].target = -1; spHost[0].done = 0; spHost[0].wantParent = wantParent;
+        spHost[0].gen = 1;
+        for (;;) {
+            int running = 0;
+            std::string e = solver_round(s, g->d, m, running);
+            if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+            if (running == 0) break;
+        }
+        std::string e = solver_fetch(s, 0, dist, nullptr, src_id, parent);
         if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-        if (running == 0) break;
-    }
-    std::string e = solver_fetch(s, 0, dist, nullptr, src_id, parent);
-    if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-    return SVG_OK;
+        return SVG_OK;
+    });
 }
 
 // Surface-point queries (SVG.cu:253-352) on the graph: the ICH runs and
@@ -1616,126 +1622,134 @@ extern "C" int svg_sssd_point(svg_graph_t g, int f0, const double* p0, int f1, c
                               int32_t* key_edge, double* key_x, int32_t* last_vert, int* route, int32_t* path,
                               int path_cap, int* path_len)
 {
-    if (!g || !p0 || !p1 || !dist) return svg_set_error(SVG_E_ARG, "null argument");
-    if (path_cap > 0 && !path) return svg_set_error(SVG_E_ARG, "null path");
-    std::lock_guard<std::mutex> lk(g->mu);
-    const int32_t faces[2] = {f0, f1};
-    const double pts[6] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2]};
-    PointRes r;
-    int rc = point_queries(g, 1, faces, pts, &r, path, path_cap, path_len);
-    if (rc) return rc;
-    if (r.route == -2)
-        return svg_set_error(SVG_E_ARG, "the propagation from a point stalled on unordered keys "
-                                        "(a point on a vertex of its face)");
-    *dist = r.dist;
-    if (key_edge) { key_edge[0] = r.srcEdge; key_edge[1] = r.dstEdge; }
-    if (key_x) { key_x[0] = r.srcX; key_x[1] = r.dstX; }
-    if (last_vert) *last_vert = r.lastVert;
-    if (route) *route = r.route;
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !p0 || !p1 || !dist) return svg_set_error(SVG_E_ARG, "null argument");
+        if (path_cap > 0 && !path) return svg_set_error(SVG_E_ARG, "null path");
+        std::lock_guard<std::mutex> lk(g->mu);
+        const int32_t faces[2] = {f0, f1};
+        const double pts[6] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2]};
+        PointRes r;
+        int rc = point_queries(g, 1, faces, pts, &r, path, path_cap, path_len);
+        if (rc) return rc;
+        if (r.route == -2)
+            return svg_set_error(SVG_E_ARG, "the propagation from a point stalled on unordered keys "
+                                            "(a point on a vertex of its face)");
+        *dist = r.dist;
+        if (key_edge) { key_edge[0] = r.srcEdge; key_edge[1] = r.dstEdge; }
+        if (key_x) { key_x[0] = r.srcX; key_x[1] = r.dstX; }
+        if (last_vert) *last_vert = r.lastVert;
+        if (route) *route = r.route;
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_sssd_point_batch(svg_graph_t g, int nq, const int32_t* faces, const double* points,
                                     double* dist, int32_t* key_edge, double* key_x, int32_t* last_vert,
                                     int32_t* route)
 {
-    if (!g || !faces || !points || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
-    if (nq == 0) return SVG_OK;
-    std::lock_guard<std::mutex> lk(g->mu);
-    std::vector<PointRes> r((size_t)nq);
-    int rc = point_queries(g, nq, faces, points, r.data(), nullptr, 0, nullptr);
-    if (rc) return rc;
-    for (int q = 0; q < nq; ++q) {
-        dist[q] = r[q].dist;
-        if (key_edge) { key_edge[2 * q] = r[q].srcEdge; key_edge[2 * q + 1] = r[q].dstEdge; }
-        if (key_x) { key_x[2 * q] = r[q].srcX; key_x[2 * q + 1] = r[q].dstX; }
-        if (last_vert) last_vert[q] = r[q].lastVert;
-        if (route) route[q] = r[q].route;
-    }
-    return SVG_OK;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !faces || !points || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        if (nq == 0) return SVG_OK;
+        std::lock_guard<std::mutex> lk(g->mu);
+        std::vector<PointRes> r((size_t)nq);
+        int rc = point_queries(g, nq, faces, points, r.data(), nullptr, 0, nullptr);
+        if (rc) return rc;
+        for (int q = 0; q < nq; ++q) {
+            dist[q] = r[q].dist;
+            if (key_edge) { key_edge[2 * q] = r[q].srcEdge; key_edge[2 * q + 1] = r[q].dstEdge; }
+            if (key_x) { key_x[2 * q] = r[q].srcX; key_x[2 * q + 1] = r[q].dstX; }
+            if (last_vert) last_vert[q] = r[q].lastVert;
+            if (route) route[q] = r[q].route;
+        }
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_msad_batch(svg_graph_t g, const int32_t* src, const double* w, const int64_t* offsets, int nq,
                               double* dist, int32_t* src_id)
 {
-    if (!g || !offsets || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
-    const size_t V = (size_t)g->d.V;
-    for (int i = 0; i < nq; ++i) {
-        if (offsets[i + 1] <= offsets[i]) return svg_set_error(SVG_E_ARG, "every query needs a source");
-        if (offsets[i] < 0) return svg_set_error(SVG_E_ARG, "negative offset");
-    }
-    if (nq > 0 && (!src || !w)) return svg_set_error(SVG_E_ARG, "null sources or weights");
-    std::lock_guard<std::mutex> lk(g->mu);
-    const int S = std::min(32, std::max(1, nq));
-    SolverState* st = nullptr;
-    int rc = get_solver(g, S, &st);
-    if (rc) return rc;
-    std::vector<int> slotQuery((size_t)st->S, -1);
-    int nextQ = 0, done = 0;
-    while (done < nq) {
-        std::vector<Admission> adm;
-        for (int q = 0; q < st->S && nextQ < nq; ++q) {
-            if (slotQuery[q] != -1) continue;
-            Admission a;
-            a.slot = q; a.kind = Q_MSAD; a.target = -1; a.wantParent = src_id != nullptr;
-            a.src.assign(src + offsets[nextQ], src + offsets[nextQ + 1]);
-            a.w.assign(w + offsets[nextQ], w + offsets[nextQ + 1]);
-            adm.push_back(std::move(a));
-            slotQuery[q] = nextQ++;
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !offsets || !dist || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        const size_t V = (size_t)g->d.V;
+        for (int i = 0; i < nq; ++i) {
+            if (offsets[i + 1] <= offsets[i]) return svg_set_error(SVG_E_ARG, "every query needs a source");
+            if (offsets[i] < 0) return svg_set_error(SVG_E_ARG, "negative offset");
         }
-        std::string ea = solver_admit_many(*st, g->mesh->d, adm);
-        if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
-        int running = 0;
-        std::string e = solver_round(*st, g->d, g->mesh->d, running);
-        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-        for (int q = 0; q < st->S; ++q) {
-            if (slotQuery[q] == -1 || !st->spHost[q].done) continue;
-            const size_t o = (size_t)slotQuery[q] * V;
-            e = solver_fetch(*st, q, dist + o, nullptr, src_id ? src_id + o : nullptr, nullptr);
+        if (nq > 0 && (!src || !w)) return svg_set_error(SVG_E_ARG, "null sources or weights");
+        std::lock_guard<std::mutex> lk(g->mu);
+        const int S = std::min(32, std::max(1, nq));
+        SolverState* st = nullptr;
+        int rc = get_solver(g, S, &st);
+        if (rc) return rc;
+        std::vector<int> slotQuery((size_t)st->S, -1);
+        int nextQ = 0, done = 0;
+        while (done < nq) {
+            std::vector<Admission> adm;
+            for (int q = 0; q < st->S && nextQ < nq; ++q) {
+                if (slotQuery[q] != -1) continue;
+                Admission a;
+                a.slot = q; a.kind = Q_MSAD; a.target = -1; a.wantParent = src_id != nullptr;
+                a.src.assign(src + offsets[nextQ], src + offsets[nextQ + 1]);
+                a.w.assign(w + offsets[nextQ], w + offsets[nextQ + 1]);
+                adm.push_back(std::move(a));
+                slotQuery[q] = nextQ++;
+            }
+            std::string ea = solver_admit_many(*st, g->mesh->d, adm);
+            if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
+            int running = 0;
+            std::string e = solver_round(*st, g->d, g->mesh->d, running);
             if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-            solver_release_slot(*st, q);
-            slotQuery[q] = -1;
-            ++done;
+            for (int q = 0; q < st->S; ++q) {
+                if (slotQuery[q] == -1 || !st->spHost[q].done) continue;
+                const size_t o = (size_t)slotQuery[q] * V;
+                e = solver_fetch(*st, q, dist + o, nullptr, src_id ? src_id + o : nullptr, nullptr);
+                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                solver_release_slot(*st, q);
+                slotQuery[q] = -1;
+                ++done;
+            }
         }
-    }
-    return SVG_OK;
+        return SVG_OK;
+    });
 }
 
 extern "C" int svg_sssd_batch(svg_graph_t g, const int32_t* stp, int nq, double* out)
 {
-    if (!g || !stp || !out || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
-    std::lock_guard<std::mutex> lk(g->mu);
-    const int S = std::min(64, std::max(1, nq));
-    SolverState* st = nullptr;
-    int rc = get_solver(g, S, &st);
-    if (rc) return rc;
-    std::vector<int> slotQuery((size_t)st->S, -1);
-    int nextQ = 0, done = 0;
-    const double w0 = 0.0;
-    while (done < nq) {
-        std::vector<Admission> adm;
-        for (int q = 0; q < st->S && nextQ < nq; ++q) {
-            if (slotQuery[q] != -1) continue;
-            Admission a;
-            a.slot = q; a.kind = Q_SSSD; a.target = stp[2 * nextQ + 1]; a.wantParent = false;
-            a.src.push_back(stp[2 * nextQ]);
-            a.w.push_back(w0);
-            adm.push_back(std::move(a));
-            slotQuery[q] = nextQ++;
-        }
-        std::string ea = solver_admit_many(*st, g->mesh->d, adm);
-        if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
-        int running = 0;
-        std::string e = solver_round(*st, g->d, g->mesh->d, running);
-        if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-        for (int q = 0; q < st->S; ++q) {
-            if (slotQuery[q] == -1 || !st->spHost[q].done) continue;
-            e = solver_fetch(*st, q, nullptr, &out[slotQuery[q]], nullptr, nullptr);
+    return svg_guarded<int>([&]() -> int {
+        if (!g || !stp || !out || nq < 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        std::lock_guard<std::mutex> lk(g->mu);
+        const int S = std::min(64, std::max(1, nq));
+        SolverState* st = nullptr;
+        int rc = get_solver(g, S, &st);
+        if (rc) return rc;
+        std::vector<int> slotQuery((size_t)st->S, -1);
+        int nextQ = 0, done = 0;
+        const double w0 = 0.0;
+        while (done < nq) {
+            std::vector<Admission> adm;
+            for (int q = 0; q < st->S && nextQ < nq; ++q) {
+                if (slotQuery[q] != -1) continue;
+                Admission a;
+                a.slot = q; a.kind = Q_SSSD; a.target = stp[2 * nextQ + 1]; a.wantParent = false;
+                a.src.push_back(stp[2 * nextQ]);
+                a.w.push_back(w0);
+                adm.push_back(std::move(a));
+                slotQuery[q] = nextQ++;
+            }
+            std::string ea = solver_admit_many(*st, g->mesh->d, adm);
+            if (!ea.empty()) return svg_set_error(SVG_E_ARG, ea);
+            int running = 0;
+            std::string e = solver_round(*st, g->d, g->mesh->d, running);
             if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
-            solver_release_slot(*st, q);
-            slotQuery[q] = -1;
-            ++done;
+            for (int q = 0; q < st->S; ++q) {
+                if (slotQuery[q] == -1 || !st->spHost[q].done) continue;
+                e = solver_fetch(*st, q, nullptr, &out[slotQuery[q]], nullptr, nullptr);
+                if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
+                solver_release_slot(*st, q);
+                slotQuery[q] = -1;
+                ++done;
+            }
         }
-    }
-    return SVG_OK;
+        return SVG_OK;
+    });
 }

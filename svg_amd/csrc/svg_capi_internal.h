@@ -1,7 +1,9 @@
 // Handle layouts shared by the C ABI translation units.
 #pragma once
 
+#include <exception>
 #include <mutex>
+#include <new>
 #include <string>
 
 #include "svg_amd.h"
@@ -29,6 +31,23 @@ struct svg_graph_s {
 };
 
 int svg_set_error(int code, const std::string& msg);
+
+// Every entry point of the C ABI runs its body through this guard, so no
+// C++ exception reaches a C caller: a failed host allocation is SVG_E_MEM,
+// anything else thrown is SVG_E_INTERNAL with its message.
+template <class R, class F>
+R svg_guarded(F&& body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return (R)svg_set_error(SVG_E_MEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return (R)svg_set_error(SVG_E_INTERNAL, std::string("host failure: ") + e.what());
+    } catch (...) {
+        return (R)svg_set_error(SVG_E_INTERNAL, "host failure");
+    }
+}
 int svg_ensure_device();
 void svg_solver_release(svg_graph_s* g);
 void svg_point_release(svg_graph_s* g);
