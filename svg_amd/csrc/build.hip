@@ -1223,6 +1223,10 @@ constexpr int kPropBlock = 128;
 // per lane the shared work counter already evens the lanes out (no
 // difference measured at 13 per lane).
 constexpr int kPredictPerLane = 4;
+// Weight of a pseudo-source pop against a window pop in the recorded cost
+// that orders sources: the wave iterations that run fan sweeps take about
+// three times as long as those that run window events only.
+constexpr int kPseudoCost = 3;
 
 __device__ __forceinline__ void policy_mesh(BatchedPolicy& p, const BArgs& a)
 {
@@ -1727,7 +1731,7 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
             st.counters[CNT_FAN_HALFEDGE] += cc[8]; st.counters[CNT_PSEUDO_NEIGH] += cc[9];
             st.counters[CNT_BT_HOPS] += cc[10]; st.counters[CNT_REACHED] += cc[11];
             if (tune) {
-                newCost[src] = cc[1] + cc[3];
+                newCost[src] = cc[1] + kPseudoCost * cc[3];
                 // a source that hit the capacity stopped early: its peak is unknown
                 const int grow = (hs[i] & SF_CAPACITY) ? 2 : 1;
                 newMaxV = std::max(newMaxV, cc[11] * grow);

@@ -16,7 +16,7 @@ namespace svg {
 struct TuneRecord {
     int K = 0;
     unsigned splitCoef = 0, vertCoef = 0;
-    std::vector<int32_t> cost;     // per source: window + pseudo pops
+    std::vector<int32_t> cost;     // per source: window pops + 3 x pseudo pops (time-weighted)
     int maxVert = 0, maxSplit = 0; // largest table occupancy seen
     int maxWinQ = 1023, maxPseudoQ = 1023;  // longest window / pseudo queue seen
     double blockMs[8] = {0.0};     // propagation time of each launch shape tried
