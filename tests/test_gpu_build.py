@@ -43,6 +43,30 @@ def test_build_matches_oracle_small(name, mode):
     assert not np.any(st & 16), "error flags"
 
 
+@pytest.mark.parametrize("name,K", [("tetrahedron", 1), ("tetrahedron", 2), ("tetrahedron", 3),
+                                    ("tetrahedron", 10), ("sphere642", 1), ("sphere642", 2),
+                                    ("sphere642", 700)])
+@pytest.mark.parametrize("mode", [S.SVG_MODE_PLAIN, S.SVG_MODE_BATCHED])
+def test_build_extreme_K_matches_oracle(name, K, mode):
+    # K = 1, 2: the reference tables (2K vertex, 10K split entries) overflow
+    # on most sphere sources and alias their overflow slot; K above V: every
+    # source runs until its queues are empty.  Rows, and for the batched
+    # build the overflow flags, equal the oracle's.
+    xyz, tri = mg.tetrahedron() if name == "tetrahedron" else \
+        mg.noisy_sphere(seed=4, splits4=3, centroid_split=False)
+    print(f"case extreme K {name} K={K} mode {mode}", flush=True)
+    _, srcs, orows, otails, cnt = oracle_build(xyz, tri, K)
+    m, g = _build(xyz, tri, K, mode)
+    bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+    st = g.status()
+    assert not np.any(st & 16), "error flags"
+    if mode == S.SVG_MODE_BATCHED:
+        ci = {n: i for i, n in enumerate(O.COUNTERS)}
+        over = (cnt[:, ci["vert_overflow"]] + cnt[:, ci["split_overflow"]]) > 0
+        assert np.array_equal((st[srcs] & 4) > 0, over)
+
+
 def test_build_heightfield64_full_both_modes():
     """The small plain-method workload: 64x64 heightfield, K=50, every source."""
     xyz, tri = mg.heightfield64()
