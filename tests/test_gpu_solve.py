@@ -37,6 +37,26 @@ def test_sssd_matches_oracle_astar(small_graph):
         assert abs(d_gpu - d_ref) <= 1e-9 * d_ref, (s, t, d_gpu, d_ref)
 
 
+def test_sssd_to_an_unreachable_target(small_graph):
+    # rows keep the first K qualifying ids (SVG.cu:44-75), so some vertices
+    # have no in-arcs: A* to them ends with the oracle's distance (the
+    # initial DBL_MAX) and a path that is only the target, s == t gives 0
+    m, g, og, h = small_graph
+    indeg = np.zeros(og.V, np.int64)
+    for u in range(og.V):
+        np.add.at(indeg, h["adj"][u, : h["tails"][u]], 1)
+    targets = np.nonzero(indeg == 0)[0][:4]
+    assert targets.size > 0, "the graph must have a vertex without in-arcs"
+    for t in targets:
+        s = int((t + 101) % og.V)
+        print(f"sssd unreachable {s}->{t}", flush=True)
+        d_ref, *_ = og.sssd(s, int(t))
+        d_gpu, path = g.SolveSSSD(s, int(t), path=True)
+        assert d_gpu == d_ref == np.finfo(np.float64).max
+        assert path.tolist() in ([], [int(t)])
+    assert g.SolveSSSD(7, 7) == 0.0 == og.sssd(7, 7)[0]
+
+
 def test_sssd_path_reproduces_distance(small_graph):
     m, g, og, h = small_graph
     d, path = g.SolveSSSD(5, 400, path=True)
