@@ -114,6 +114,26 @@ def test_msad_matches_oracle_dijkstra(small_graph, ns):
     assert np.mean(sid == rsid) > 0.9
 
 
+@pytest.mark.parametrize("src,w", [([11, 300, 11], [0.05, 0.0, 0.01]),
+                                   ([11, 11], [0.0, 0.2]),
+                                   ([5, 400, 5, 400], [0.3, 0.1, 0.0, 0.02])])
+def test_msad_repeated_sources_keep_their_last_weight(small_graph, src, w):
+    # SVG.cu:359-365 writes each source's distance in list order, so a
+    # repeated source keeps its last weight, larger or smaller
+    m, g, og, h = small_graph
+    src = np.array(src, np.int32)
+    w = np.array(w)
+    print(f"msad repeated {src.tolist()} {w.tolist()}", flush=True)
+    dist, sid, par = g.SolveMSAD(src, w)
+    rdist, _, rsid, _ = og.msad(src, w)
+    assert np.array_equal(dist, rdist)
+    last = {int(v): float(x) for v, x in zip(src, w)}
+    for v, x in last.items():
+        assert dist[v] <= x and sid[v] == rsid[v]
+    arcs = graph_arcs(h["tails"], h["adj"], h["dist"], h["adj"].shape[1])
+    assert_valid_source_ids(arcs, np.array(list(last)), np.array(list(last.values())), dist, sid)
+
+
 def test_sssd_batch_equals_single(small_graph):
     m, g, og, _ = small_graph
     rng = np.random.default_rng(7)
