@@ -621,6 +621,9 @@ static void ich_initialize(OIch *s)
     const OMesh *m = s->mesh;
     unsigned src = s->sourceVert;
     int startEdge = m->v[src].firstEdge;
+    /* decision I: a vertex that no face uses keeps firstEdge -1 (Mesh.cuh:22)
+     * and starts no window (the reference would read edges[-1]) */
+    if (startEdge == -1) return;
     int cur = startEdge;
     do {
         unsigned opEdge = (unsigned)m->e[cur].next;

@@ -151,6 +151,9 @@ struct Ich {
     {
         const int src = source;
         const int start = p.first_edge(src);
+        // a vertex that no face uses (firstEdge -1, Mesh.cuh:22) starts no
+        // window: its row stays empty and no other source reaches it
+        if (start == -1) return;
         int cur = start;
         do {
             const int opEdge = p.next(cur);

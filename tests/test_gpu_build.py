@@ -67,6 +67,22 @@ def test_build_extreme_K_matches_oracle(name, K, mode):
         assert np.array_equal((st[srcs] & 4) > 0, over)
 
 
+@pytest.mark.parametrize("mode", [S.SVG_MODE_PLAIN, S.SVG_MODE_BATCHED])
+def test_vertex_without_faces(mode):
+    # a vertex that no face uses starts no window: its row is empty and the
+    # other rows are the oracle's (which equal those of the mesh without it)
+    xyz, tri = mg.noisy_sphere(seed=4, splits4=3, centroid_split=False)
+    xyz = np.vstack([xyz, [[0.0, 0.0, 0.0]]])
+    K = 60
+    print(f"case vertex without faces mode {mode}", flush=True)
+    _, srcs, orows, otails, _ = oracle_build(xyz, tri, K)
+    assert otails[-1] == 0
+    m, g = _build(xyz, tri, K, mode)
+    bad = compare_rows(gpu_rows(g), orows, otails, srcs, K)
+    assert not bad, bad[:10]
+    assert not np.any(g.status() & 16), "error flags"
+
+
 def test_build_heightfield64_full_both_modes():
     """The small plain-method workload: 64x64 heightfield, K=50, every source."""
     xyz, tri = mg.heightfield64()

@@ -293,3 +293,15 @@ def test_angle_exactly_two_pi_counts_as_saddle_in_initialize_only():
     dist, _, c = O.ich_single(m, 5, K=-1)
     assert c["pseudo_push"] == 0
     assert abs(dist[0] - 5.0) < 1e-12
+
+
+def test_vertex_without_faces_starts_no_window():
+    # decision I: a vertex that no face uses (firstEdge -1) has an empty row
+    # and leaves every other row as it is without it
+    xyz, tri = mg.noisy_sphere(seed=4, splits4=3, centroid_split=False)
+    K = 60
+    r0, t0, _, _ = O.svg_build(O.Mesh(xyz, tri), K)
+    xyz2 = np.vstack([xyz, [[0.0, 0.0, 0.0]]])
+    r1, t1, _, _ = O.svg_build(O.Mesh(xyz2, tri), K)
+    assert t1[-1] == 0
+    assert np.array_equal(t0, t1[:-1]) and np.array_equal(r0, r1[:-1])
