@@ -29,6 +29,10 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
     if ((long long)nf * 3 > 0x7fffffffLL) return "too many faces";
     for (long long i = 0; i < 3LL * nf; ++i)
         if (tri[i] < 0 || tri[i] >= nv) return "triangle index out of range";
+    for (int f = 0; f < nf; ++f) {
+        const int32_t a = tri[3 * (size_t)f], b = tri[3 * (size_t)f + 1], c = tri[3 * (size_t)f + 2];
+        if (a == b || b == c || a == c) return "face " + std::to_string(f) + " repeats a vertex";
+    }
     m.nv = nv; m.nf = nf; m.ne = 3 * nf;
     m.xyz.assign(xyz, xyz + 3 * (size_t)nv);
     m.tri.assign(tri, tri + 3 * (size_t)nf);
@@ -43,8 +47,12 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
             m.len[3 * f + j] = std::sqrt(x * x + y * y + z * z);
         }
     }
-    // Twins: the first half-edge of each unordered vertex pair is kept and
-    // every later one is paired with it (Mesh.cu:79-89).
+    // Twins: the half-edges of one unordered vertex pair are paired
+    // (Mesh.cu:79-89).  A pair met a third time (an edge of more than two
+    // faces) or twice in the same direction (faces of opposite orientation)
+    // is refused: the reference would overwrite or mis-pair the twins, and
+    // every walk around a vertex relies on twin(twin(e)) = e running the
+    // other way.
     {
         std::unordered_map<uint64_t, int32_t> first;
         first.reserve((size_t)m.ne);
@@ -56,8 +64,14 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
             if (it == first.end()) {
                 first.emplace(key, e);
             } else {
-                m.twin[e] = it->second;
-                m.twin[it->second] = e;
+                const int32_t o = it->second;
+                const std::string edge = "(" + std::to_string(a) + ", " + std::to_string(b) + ")";
+                if (m.twin[o] != -1) return "edge " + edge + " is shared by more than two faces";
+                if (tri[o] == tri[e])
+                    return "faces " + std::to_string(o / 3) + " and " + std::to_string(e / 3) +
+                           " are oriented oppositely across edge " + edge;
+                m.twin[e] = o;
+                m.twin[o] = e;
             }
         }
     }

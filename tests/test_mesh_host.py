@@ -31,6 +31,34 @@ def test_mesh_rejects_bad_indices():
         S.Mesh.from_arrays(xyz, tri)
 
 
+def _fan_of_three_faces_on_one_edge():
+    # edge (0, 1) shared by three faces
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0], [0.5, -1, 0], [0.5, 0, 1]], float)
+    tri = np.array([[0, 1, 2], [1, 0, 3], [1, 0, 4]], np.int32)
+    return xyz, tri
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("repeat", "repeats a vertex"),
+    ("three", "shared by more than two faces"),
+    ("flip", "oriented oppositely"),
+])
+def test_mesh_refuses_broken_topology(case, msg):
+    # meshes on which walks around a vertex would not close (the reference
+    # would mis-pair the twins, Mesh.cu:79-89) are refused with SVG_E_ARG
+    xyz, tri = mg.tetrahedron()
+    tri = tri.copy()
+    if case == "repeat":
+        tri[1] = [tri[1, 0], tri[1, 0], tri[1, 2]]
+    elif case == "three":
+        xyz, tri = _fan_of_three_faces_on_one_edge()
+    else:
+        tri[2] = tri[2, ::-1]
+    with pytest.raises(S.SvgError) as e:
+        S.Mesh.from_arrays(xyz, tri)
+    assert msg in str(e.value) and "error -1" in str(e.value)
+
+
 def test_obj_roundtrip(tmp_path):
     xyz, tri = mg.noisy_sphere(seed=4, splits4=2, centroid_split=False)
     p = tmp_path / "m.obj"
