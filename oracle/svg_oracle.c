@@ -957,6 +957,11 @@ static void ich_execute(OIch *s, int K)
             s->cnt[C_PSEUDO_POP]++;
             if (pw.level >= (unsigned)m->nf) { s->cnt[C_LEVEL_SKIP]++; continue; }
             gen_sub_wins(s, &pw);
+        } else {
+            /* decision S: the tops compare unordered (a NaN key); the
+             * reference would spin on them (ICHDevice.cu:88, 110) */
+            s->cnt[C_ERRORS]++;
+            break;
         }
         if (K != -1 && s->totalCalcVertNum >= K) break;
     }

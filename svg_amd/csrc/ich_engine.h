@@ -669,6 +669,11 @@ struct Ich {
                 p.count(CNT_PSEUDO_POP, 1);
                 if (ptop.level >= nf) continue;
                 gen_sub_wins(ptop);
+            } else {
+                // the tops compare unordered (a NaN key): the reference would
+                // spin on them (ICHDevice.cu:88, 110); the source ends flagged
+                p.count(CNT_ERRORS, 1);
+                break;
             }
             if (K != -1 && total >= K) break;
             if (p.stop()) break;

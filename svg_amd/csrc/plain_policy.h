@@ -75,7 +75,9 @@ struct PlainPolicy {
         t.l0 = E[e0].len; t.l1 = E[t.e1].len; t.l2 = E[t.e2].len;
         t.tw1 = E[t.e1].twin; t.tw2 = E[t.e2].twin;
     }
-    __device__ void count(int, long long) {}
+    // event counts are not kept on the reference-layout path; an error
+    // (a source ended on unordered queue keys) still sets its status bit
+    __device__ void count(int c, long long) { if (c == CNT_ERRORS) flags |= SF_ERROR; }
 
     // MyHashTable (MyHashTable.cuh:98-117)
     __device__ static unsigned hfun(unsigned index, unsigned size)

@@ -33,6 +33,10 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
         const int32_t a = tri[3 * (size_t)f], b = tri[3 * (size_t)f + 1], c = tri[3 * (size_t)f + 2];
         if (a == b || b == c || a == c) return "face " + std::to_string(f) + " repeats a vertex";
     }
+    for (int v = 0; v < nv; ++v)
+        if (!std::isfinite(xyz[3 * (size_t)v]) || !std::isfinite(xyz[3 * (size_t)v + 1]) ||
+            !std::isfinite(xyz[3 * (size_t)v + 2]))
+            return "vertex " + std::to_string(v) + " has a non-finite coordinate";
     m.nv = nv; m.nf = nf; m.ne = 3 * nf;
     m.xyz.assign(xyz, xyz + 3 * (size_t)nv);
     m.tri.assign(tri, tri + 3 * (size_t)nf);
@@ -45,6 +49,8 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
             double y = xyz[3 * a + 1] - xyz[3 * b + 1];
             double z = xyz[3 * a + 2] - xyz[3 * b + 2];
             m.len[3 * f + j] = std::sqrt(x * x + y * y + z * z);
+            // coincident vertices would put 0/0 into the window geometry
+            if (!(m.len[3 * f + j] > 0.0)) return "face " + std::to_string(f) + " has an edge of length 0";
         }
     }
     // Twins: the half-edges of one unordered vertex pair are paired

@@ -42,18 +42,27 @@ def _fan_of_three_faces_on_one_edge():
     ("repeat", "repeats a vertex"),
     ("three", "shared by more than two faces"),
     ("flip", "oriented oppositely"),
+    ("nan", "non-finite coordinate"),
+    ("coincident", "edge of length 0"),
 ])
 def test_mesh_refuses_broken_topology(case, msg):
     # meshes on which walks around a vertex would not close (the reference
-    # would mis-pair the twins, Mesh.cu:79-89) are refused with SVG_E_ARG
+    # would mis-pair the twins, Mesh.cu:79-89), and meshes whose geometry
+    # would put NaN into the window keys, are refused with SVG_E_ARG
     xyz, tri = mg.tetrahedron()
     tri = tri.copy()
     if case == "repeat":
         tri[1] = [tri[1, 0], tri[1, 0], tri[1, 2]]
     elif case == "three":
         xyz, tri = _fan_of_three_faces_on_one_edge()
-    else:
+    elif case == "flip":
         tri[2] = tri[2, ::-1]
+    elif case == "nan":
+        xyz = xyz.copy()
+        xyz[2, 1] = np.nan
+    else:                                  # two vertices at one position
+        xyz = xyz.copy()
+        xyz[1] = xyz[0]
     with pytest.raises(S.SvgError) as e:
         S.Mesh.from_arrays(xyz, tri)
     assert msg in str(e.value) and "error -1" in str(e.value)
