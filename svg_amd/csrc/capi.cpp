@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <algorithm>
@@ -478,6 +479,16 @@ extern "C" int svg_graph_load(svg_mesh_t m, int K, const char* path, svg_graph_t
             for (int i = 0; i < t; ++i) {
                 const size_t k = (size_t)v * K + i;
                 if (row[i].adjNode < 0 || row[i].adjNode >= V) return svg_set_error(SVG_E_IO, "arc target out of range");
+                // the solvers bucket distances by the arc weights and the
+                // polyline tracer walks from the key-point edges: a file
+                // that is not the output of a build is checked here
+                if (!(row[i].geodDist >= 0.0) || !std::isfinite(row[i].geodDist))
+                    return svg_set_error(SVG_E_IO, "arc length is not a finite non-negative number");
+                const int32_t es = (int32_t)row[i].nextToSrcEdge, ed = (int32_t)row[i].nextToDstEdge;
+                if (es < -1 || es >= m->h.ne || ed < -1 || ed >= m->h.ne)
+                    return svg_set_error(SVG_E_IO, "key-point edge out of range");
+                if (!std::isfinite(row[i].nextToSrcX) || !std::isfinite(row[i].nextToDstX))
+                    return svg_set_error(SVG_E_IO, "key-point position is not finite");
                 adj[k] = row[i].adjNode; w[k] = row[i].geodDist;
                 se[k] = (int32_t)row[i].nextToSrcEdge; de[k] = (int32_t)row[i].nextToDstEdge;
                 sx[k] = row[i].nextToSrcX; dx[k] = row[i].nextToDstX;

@@ -116,3 +116,33 @@ def test_obj_bad_corner_is_an_error(tmp_path):
     short = tmp_path / "short.obj"
     _write(short, "v 0 0 0\nv 1 0 0\nf 1 2\n")
     assert not S.Mesh().LoadFromFile(str(short))
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("adj", 7, "arc target out of range"),
+    ("w", -1.0, "arc length is not a finite non-negative number"),
+    ("w", float("nan"), "arc length is not a finite non-negative number"),
+    ("se", 99, "key-point edge out of range"),
+    ("de", -2, "key-point edge out of range"),
+    ("sx", float("inf"), "key-point position is not finite"),
+])
+def test_svg_file_with_bad_records_is_refused(tmp_path, field, value, msg):
+    # a .svg file (SVG.cu:208-239 layout: int32 row length, then 40-B records
+    # int32 adj, 4 B padding, f64 dist, u32 src edge, u32 dst edge, f64 src x,
+    # f64 dst x) whose record is out of range is refused before any upload
+    import ctypes as C
+    import struct
+    xyz, tri = mg.tetrahedron()
+    mesh = S.Mesh.from_arrays(xyz, tri)
+    rec = dict(adj=1, w=0.5, se=0, de=3, sx=0.1, dx=0.2)
+    rec[field] = value
+    p = tmp_path / "bad.svg"
+    with open(p, "wb") as f:
+        f.write(struct.pack("<i", 1))
+        f.write(struct.pack("<iidIIdd", rec["adj"], 0, rec["w"], rec["se"] & 0xFFFFFFFF, rec["de"] & 0xFFFFFFFF,
+                            rec["sx"], rec["dx"]))
+        for _ in range(3):
+            f.write(struct.pack("<i", 0))
+    g = C.c_void_p()
+    rc = S.lib().svg_graph_load(mesh.h, 3, str(p).encode(), C.byref(g))
+    assert rc == -4 and msg in S.lib().svg_last_error().decode()
