@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -415,8 +416,10 @@ static long long submit_impl(svg_server_t S, const svg_request* r, svg_done_fn c
         T->s = r->s; T->t = r->t;
     } else {
         if (!r->src || !r->w || r->ns <= 0) return svg_set_error(SVG_E_ARG, "MSAD needs sources and weights");
-        for (int i = 0; i < r->ns; ++i)
+        for (int i = 0; i < r->ns; ++i) {
             if (r->src[i] < 0 || r->src[i] >= V) return svg_set_error(SVG_E_ARG, "source out of range");
+            if (!std::isfinite(r->w[i]) || std::signbit(r->w[i])) return svg_set_error(SVG_E_ARG, kBadWeight);
+        }
         T->src.assign(r->src, r->src + r->ns);
         T->w.assign(r->w, r->w + r->ns);
     }

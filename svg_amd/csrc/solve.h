@@ -43,6 +43,11 @@ struct SlotParam {              // device-resident, one per slot
                                 // JOIN: best exit distance + offset so far (an upper bound)
 };
 
+// Distances and keys are compared and lowered as the bits of non-negative
+// doubles (atomicMin on their 64-bit patterns), so MSAD source weights must
+// be finite with the sign bit clear; the reference takes any weight
+// (SVG.cu:361), and the callers refuse the others with this message.
+constexpr const char* kBadWeight = "source weight must be finite and non-negative (not -0.0)";
 constexpr int kCBlocks = 512;     // classify blocks = deferred-list regions
 constexpr int kPLists = 8;        // push lists (and won lists)
 constexpr int kCntPad = 32;       // counters one 128-B line apart

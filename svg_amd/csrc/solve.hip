@@ -1294,6 +1294,8 @@ extern "C" int svg_msad(svg_graph_t g, const int32_t* src, const double* w, int 
 {
     return svg_guarded<int>([&]() -> int {
         if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
+        for (int i = 0; i < ns; ++i)
+            if (!std::isfinite(w[i]) || std::signbit(w[i])) return svg_set_error(SVG_E_ARG, kBadWeight);
         std::lock_guard<std::mutex> lk(g->mu);
         SolverState* st = nullptr;
         int rc = get_solver(g, 1, &st);
@@ -1403,8 +1405,10 @@ extern "C" int svg_msad_noalloc(svg_graph_t g, const int32_t* src, const double*
     return svg_guarded<int>([&]() -> int {
         if (!g || !src || !w || !dist || ns <= 0) return svg_set_error(SVG_E_ARG, "bad argument");
         const int V = g->d.V;
-        for (int i = 0; i < ns; ++i)
+        for (int i = 0; i < ns; ++i) {
             if (src[i] < 0 || src[i] >= V) return svg_set_error(SVG_E_ARG, "source out of range");
+            if (!std::isfinite(w[i]) || std::signbit(w[i])) return svg_set_error(SVG_E_ARG, kBadWeight);
+        }
         if (g->d.K > 0xFFFF) return svg_set_error(SVG_E_ARG, "K above 65535");
         const NoAllocLayout L = noalloc_layout(V, ns);
         if (L.total > kNoAllocBytes)
@@ -1676,6 +1680,8 @@ extern "C" int svg_msad_batch(svg_graph_t g, const int32_t* src, const double* w
             if (offsets[i] < 0) return svg_set_error(SVG_E_ARG, "negative offset");
         }
         if (nq > 0 && (!src || !w)) return svg_set_error(SVG_E_ARG, "null sources or weights");
+        for (int64_t i = nq > 0 ? offsets[0] : 0; nq > 0 && i < offsets[nq]; ++i)
+            if (!std::isfinite(w[i]) || std::signbit(w[i])) return svg_set_error(SVG_E_ARG, kBadWeight);
         std::lock_guard<std::mutex> lk(g->mu);
         const int S = std::min(32, std::max(1, nq));
         SolverState* st = nullptr;

@@ -134,6 +134,22 @@ def test_msad_repeated_sources_keep_their_last_weight(small_graph, src, w):
     assert_valid_source_ids(arcs, np.array(list(last)), np.array(list(last.values())), dist, sid)
 
 
+def test_msad_refuses_weights_outside_the_distance_order(small_graph):
+    # distances are ordered by the bits of non-negative doubles: a NaN,
+    # infinite or negative weight (and -0.0, whose sign bit is set) is
+    # refused before any device work, through every MSAD entry point
+    m, g, og, h = small_graph
+    src = np.array([3, 200, 500], np.int32)
+    for bad in (np.nan, np.inf, -np.inf, -0.5, -0.0):
+        w = np.array([0.0, bad, 0.1])
+        with pytest.raises(S.SvgError):
+            g.SolveMSAD(src, w)
+        with pytest.raises(S.SvgError):
+            g.SolveMSADBatch([(src, w)])
+    dist, _, _ = g.SolveMSAD(src, np.array([0.0, 0.0, 0.1]))
+    assert np.array_equal(dist, og.msad(src, [0.0, 0.0, 0.1])[0])
+
+
 def test_sssd_batch_equals_single(small_graph):
     m, g, og, _ = small_graph
     rng = np.random.default_rng(7)
