@@ -323,3 +323,35 @@ def test_non_manifold_vertex_rows_are_the_oracles(mode):
     rows, tails, _ = _oracle(xyz, tri, K)
     _, g = _build(xyz, tri, K, mode)
     _assert_rows(g, rows, tails, K)
+
+
+def test_server_refuses_weights_outside_the_distance_order():
+    # the server's MSAD requests follow the rule of svg_msad: a NaN,
+    # infinite or negative weight (or -0.0) is refused at submission
+    xyz, tri = TORUS()
+    _, g = _build(xyz, tri, 60, S.SVG_MODE_BATCHED)
+    lib = L.lib()
+    print("case server weight refusal", flush=True)
+    srv = C.c_void_p()
+    L.check(lib.svg_server_start(g.g, C.byref(L.ServerCfg(8, 0, 8)), C.byref(srv)))
+    V = len(xyz)
+    d = np.empty(V)
+    src = np.array([1, 2], np.int32)
+    try:
+        for bad in (np.nan, np.inf, -1.0, -0.0):
+            w = np.array([0.0, bad])
+            r = L.Request(1, 0, 0, src.ctypes.data_as(C.POINTER(C.c_int32)), w.ctypes.data_as(C.POINTER(C.c_double)),
+                          2, d.ctypes.data_as(C.POINTER(C.c_double)), None)
+            assert lib.svg_server_submit(srv, C.byref(r)) == -1
+            assert b"non-negative" in lib.svg_last_error()
+        w = np.array([0.0, 0.25])
+        r = L.Request(1, 0, 0, src.ctypes.data_as(C.POINTER(C.c_int32)), w.ctypes.data_as(C.POINTER(C.c_double)),
+                      2, d.ctypes.data_as(C.POINTER(C.c_double)), None)
+        t = lib.svg_server_submit(srv, C.byref(r))
+        assert t >= 0
+        lat = C.c_double()
+        L.check(lib.svg_server_wait(srv, t, C.byref(lat)))
+        ref, _, _ = g.SolveMSAD(src, w)
+        assert np.array_equal(d, ref)
+    finally:
+        L.check(lib.svg_server_stop(srv))
