@@ -5,17 +5,43 @@
 // correctly rounded values on every build.
 #include "svg_mesh.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
 #include <fstream>
 #include <sstream>
-#include <unordered_map>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 namespace svg {
 
 static const double kPi = 3.141592653589793238;   // Geometry.cuh:18
+
+// Runs body(begin, end) over [0, n) in chunks on up to 16 host threads (on
+// the calling thread alone for small n or when no thread can be started);
+// every element is computed the same way whatever the split.
+template <class F>
+static void parallel_for(int n, F&& body)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int T = (int)std::max(1u, std::min(16u, hw));
+    if (n < (1 << 16) || T == 1) { body(0, n); return; }
+    const int chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const int a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        try {
+            th.emplace_back([&body, a, b] { body(a, b); });
+        } catch (const std::system_error&) {
+            body(a, b);
+        }
+    }
+    for (std::thread& x : th) x.join();
+}
 
 static double clamp_acos(double c)
 {
@@ -42,54 +68,90 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
     m.tri.assign(tri, tri + 3 * (size_t)nf);
     m.twin.assign(m.ne, -1);                                   // boundary: -1
     m.len.resize(m.ne);
-    for (int f = 0; f < nf; ++f) {
-        for (int j = 0; j < 3; ++j) {
-            int a = tri[3 * f + j], b = tri[3 * f + (j + 1) % 3];
-            double x = xyz[3 * a] - xyz[3 * b];
-            double y = xyz[3 * a + 1] - xyz[3 * b + 1];
-            double z = xyz[3 * a + 2] - xyz[3 * b + 2];
-            m.len[3 * f + j] = std::sqrt(x * x + y * y + z * z);
-            // coincident vertices would put 0/0 into the window geometry
-            if (!(m.len[3 * f + j] > 0.0)) return "face " + std::to_string(f) + " has an edge of length 0";
-        }
-    }
-    // Twins: the half-edges of one unordered vertex pair are paired
-    // (Mesh.cu:79-89).  A pair met a third time (an edge of more than two
-    // faces) or twice in the same direction (faces of opposite orientation)
-    // is refused: the reference would overwrite or mis-pair the twins, and
-    // every walk around a vertex relies on twin(twin(e)) = e running the
-    // other way.
-    {
-        std::unordered_map<uint64_t, int32_t> first;
-        first.reserve((size_t)m.ne);
-        for (int e = 0; e < m.ne; ++e) {
-            uint64_t a = (uint32_t)tri[e], b = (uint32_t)tri[3 * (e / 3) + (e % 3 + 1) % 3];
-            if (a > b) std::swap(a, b);
-            uint64_t key = (a << 32) | b;
-            auto it = first.find(key);
-            if (it == first.end()) {
-                first.emplace(key, e);
-            } else {
-                const int32_t o = it->second;
-                const std::string edge = "(" + std::to_string(a) + ", " + std::to_string(b) + ")";
-                if (m.twin[o] != -1) return "edge " + edge + " is shared by more than two faces";
-                if (tri[o] == tri[e])
-                    return "faces " + std::to_string(o / 3) + " and " + std::to_string(e / 3) +
-                           " are oriented oppositely across edge " + edge;
-                m.twin[e] = o;
-                m.twin[o] = e;
-            }
-        }
-    }
     auto next = [](int e) { return 3 * (e / 3) + (e % 3 + 1) % 3; };
     auto prev = [](int e) { return 3 * (e / 3) + (e % 3 + 2) % 3; };
     auto v0 = [&](int e) { return tri[e]; };
+    parallel_for(nf, [&](int f0, int f1) {
+        for (int f = f0; f < f1; ++f) {
+            for (int j = 0; j < 3; ++j) {
+                int a = tri[3 * f + j], b = tri[3 * f + (j + 1) % 3];
+                double x = xyz[3 * a] - xyz[3 * b];
+                double y = xyz[3 * a + 1] - xyz[3 * b + 1];
+                double z = xyz[3 * a + 2] - xyz[3 * b + 2];
+                m.len[3 * f + j] = std::sqrt(x * x + y * y + z * z);
+            }
+        }
+    });
+    for (int e = 0; e < m.ne; ++e)                   // coincident vertices would put 0/0 into the window geometry
+        if (!(m.len[e] > 0.0)) return "face " + std::to_string(e / 3) + " has an edge of length 0";
+    // Twins: the half-edge a -> b pairs with b -> a (Mesh.cu:79-89 pairs the
+    // half-edges of one unordered vertex pair).  A direction met twice (an
+    // edge of more than two faces, or faces of opposite orientation) is
+    // refused: the reference would overwrite or mis-pair the twins, and
+    // every walk around a vertex relies on twin(twin(e)) = e running the
+    // other way.  The pairs are found in each vertex's list of outgoing
+    // half-edges (destinations stored alongside).
+    {
+        std::vector<int32_t> ofs((size_t)nv + 1, 0), outE((size_t)m.ne), outD((size_t)m.ne);
+        std::vector<uint8_t> bad((size_t)m.ne, 0);
+        for (int e = 0; e < m.ne; ++e) ofs[tri[e] + 1]++;
+        for (int v = 0; v < nv; ++v) ofs[v + 1] += ofs[v];
+        {
+            std::vector<int32_t> fill(ofs.begin(), ofs.end() - 1);
+            for (int e = 0; e < m.ne; ++e) {
+                const int k = fill[tri[e]]++;
+                outE[k] = e;
+                outD[k] = tri[next(e)];
+            }
+        }
+        parallel_for(m.ne, [&](int e0, int e1) {
+            for (int e = e0; e < e1; ++e) {
+                const int32_t a = tri[e], b = tri[next(e)];
+                int same = 0, nopp = 0, opp = -1;
+                for (int k = ofs[a]; k < ofs[a + 1]; ++k) same += outD[k] == b;        // e itself included
+                for (int k = ofs[b]; k < ofs[b + 1]; ++k)
+                    if (outD[k] == a) { opp = outE[k]; ++nopp; }
+                if (same > 1 || nopp > 1) bad[e] = 1;
+                else m.twin[e] = opp;
+            }
+        });
+        for (int e = 0; e < m.ne; ++e) {
+            if (!bad[e]) continue;
+            const int32_t a = tri[e], b = tri[next(e)];
+            const std::string edge = "(" + std::to_string(std::min(a, b)) + ", " + std::to_string(std::max(a, b)) + ")";
+            int same = 0, nopp = 0, other = -1;
+            for (int k = ofs[a]; k < ofs[a + 1]; ++k)
+                if (outD[k] == b) { ++same; if (outE[k] != e) other = outE[k]; }
+            for (int k = ofs[b]; k < ofs[b + 1]; ++k) nopp += outD[k] == a;
+            if (same + nopp > 2) return "edge " + edge + " is shared by more than two faces";
+            return "faces " + std::to_string(std::min(e, other) / 3) + " and " + std::to_string(std::max(e, other) / 3) +
+                   " are oriented oppositely across edge " + edge;
+        }
+    }
     auto v1 = [&](int e) { return tri[next(e)]; };
+    // Corner angles used by the pseudo-source fan sweeps, with the operand
+    // order of ICHDevice.cu:830 / 847 (and 876 / 893):
+    //   A(e): L0 = len(e), L1 = len(prev e), L2 = len(next e)   (left sweep)
+    //   B(e): L0 = len(e), L1 = len(next e), L2 = len(prev e)   (right sweep)
+    m.cornerA.resize(m.ne);
+    m.cornerB.resize(m.ne);
+    parallel_for(m.ne, [&](int e0, int e1) {
+        for (int e = e0; e < e1; ++e) {
+            double L0 = m.len[e];
+            double La1 = m.len[prev(e)], La2 = m.len[next(e)];
+            m.cornerA[e] = clamp_acos((L0 * L0 + La1 * La1 - La2 * La2) / (2.0 * L0 * La1));
+            double Lb1 = m.len[next(e)], Lb2 = m.len[prev(e)];
+            m.cornerB[e] = clamp_acos((L0 * L0 + Lb1 * Lb1 - Lb2 * Lb2) / (2.0 * L0 * Lb1));
+        }
+    });
+    // Vertex angle sums (Mesh.cu:107-121) add, in half-edge order, the corner
+    // at verts[0]: (l0^2 + l2^2 - l1^2) / (2 l0 l2) with l1 = len(next e),
+    // l2 = len(prev e) is A(e)'s expression operand for operand, so the
+    // stored A(e) is that corner to the bit.
     m.angle.assign(nv, 0.0);
     m.firstEdge.assign(nv, -1);
     for (int e = 0; e < m.ne; ++e) {
-        double l0 = m.len[e], l1 = m.len[next(e)], l2 = m.len[prev(e)];
-        m.angle[v0(e)] += clamp_acos((l0 * l0 + l2 * l2 - l1 * l1) / (2.0 * l0 * l2));
+        m.angle[v0(e)] += m.cornerA[e];
         m.firstEdge[v0(e)] = e;
     }
     std::vector<uint8_t> bnd(nv, 0);
@@ -104,19 +166,6 @@ std::string build_host_mesh(const double* xyz, int nv, const int32_t* tri, int n
         if (!bnd[v] && m.angle[v] > 2.0 * kPi) f |= VF_SADDLE;
         if (bnd[v]) f |= VF_BOUNDARY;
         m.flags[v] = f;
-    }
-    // Corner angles used by the pseudo-source fan sweeps, with the operand
-    // order of ICHDevice.cu:830 / 847 (and 876 / 893):
-    //   A(e): L0 = len(e), L1 = len(prev e), L2 = len(next e)   (left sweep)
-    //   B(e): L0 = len(e), L1 = len(next e), L2 = len(prev e)   (right sweep)
-    m.cornerA.resize(m.ne);
-    m.cornerB.resize(m.ne);
-    for (int e = 0; e < m.ne; ++e) {
-        double L0 = m.len[e];
-        double La1 = m.len[prev(e)], La2 = m.len[next(e)];
-        m.cornerA[e] = clamp_acos((L0 * L0 + La1 * La1 - La2 * La2) / (2.0 * L0 * La1));
-        double Lb1 = m.len[next(e)], Lb2 = m.len[prev(e)];
-        m.cornerB[e] = clamp_acos((L0 * L0 + Lb1 * Lb1 - Lb2 * Lb2) / (2.0 * L0 * Lb1));
     }
     return std::string();
 }
