@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +31,7 @@
 #include "svg_build.h"
 #include "plain_policy.h"
 #include "source_order.h"
+#include "host_parallel.h"
 
 namespace svg {
 
@@ -52,20 +54,24 @@ std::string upload_mesh(const HostMesh& h, DevMesh& d, bool aos)
     d.nv = h.nv; d.nf = h.nf; d.ne = h.ne;
     if (!d.face) {
         std::vector<FaceRec> fr((size_t)h.nf);
-        for (int f = 0; f < h.nf; ++f) {
-            FaceRec& r = fr[f];
-            for (int j = 0; j < 3; ++j) {
-                r.l[j] = h.len[3 * f + j];
-                r.v[j] = h.tri[3 * f + j];
-                r.tw[j] = h.twin[3 * f + j];
+        parallel_for(h.nf, [&](int f0, int f1) {
+            for (int f = f0; f < f1; ++f) {
+                FaceRec& r = fr[f];
+                for (int j = 0; j < 3; ++j) {
+                    r.l[j] = h.len[3 * f + j];
+                    r.v[j] = h.tri[3 * f + j];
+                    r.tw[j] = h.twin[3 * f + j];
+                }
+                r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
             }
-            r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
-        }
+        });
         std::vector<double> pos(4 * (size_t)h.nv);
-        for (int v = 0; v < h.nv; ++v) {
-            pos[4 * v] = h.xyz[3 * v]; pos[4 * v + 1] = h.xyz[3 * v + 1];
-            pos[4 * v + 2] = h.xyz[3 * v + 2]; pos[4 * v + 3] = 0.0;
-        }
+        parallel_for(h.nv, [&](int v0, int v1) {
+            for (int v = v0; v < v1; ++v) {
+                pos[4 * v] = h.xyz[3 * v]; pos[4 * v + 1] = h.xyz[3 * v + 1];
+                pos[4 * v + 2] = h.xyz[3 * v + 2]; pos[4 * v + 3] = 0.0;
+            }
+        });
         HIP_TRY(hipMalloc(&d.face, sizeof(FaceRec) * (size_t)h.nf));
         HIP_TRY(hipMalloc(&d.cornerA, sizeof(double) * (size_t)h.ne));
         HIP_TRY(hipMalloc(&d.cornerB, sizeof(double) * (size_t)h.ne));
@@ -79,34 +85,42 @@ std::string upload_mesh(const HostMesh& h, DevMesh& d, bool aos)
         HIP_TRY(hipMemcpy(d.flags, h.flags.data(), (size_t)h.nv, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.pos, pos.data(), sizeof(double) * 4 * (size_t)h.nv, hipMemcpyHostToDevice));
         // fans (only when every half-edge lies on the fan its origin's
-        // firstEdge starts: a manifold mesh)
+        // firstEdge starts: a manifold mesh).  A vertex's fan then holds all
+        // of its outgoing half-edges, so the fans start at the prefix sums of
+        // the out-degrees and are filled vertex by vertex in parallel; a walk
+        // that leaves some outgoing half-edge out marks the mesh non-manifold.
         std::vector<FanRec> fan((size_t)h.ne);
         std::vector<int32_t> fidx((size_t)h.ne, -1), fbase((size_t)h.nv + 1, 0);
-        bool manifold = true;
-        int fp = 0;
+        for (int e = 0; e < h.ne; ++e) fbase[h.tri[e] + 1]++;
+        for (int v = 0; v < h.nv; ++v) fbase[v + 1] += fbase[v];
+        std::atomic<bool> manifold{true};
         auto nxt = [](int e) { return (e % 3 == 2) ? e - 2 : e + 1; };
         auto prv = [](int e) { return (e % 3 == 0) ? e + 2 : e - 1; };
-        for (int v = 0; v < h.nv && manifold; ++v) {
-            fbase[v] = fp;
-            const int start = h.firstEdge[v];
-            int e = start;
-            while (e != -1) {
-                if (fp >= h.ne || fidx[e] != -1 || h.tri[e] != v) { manifold = false; break; }
-                fidx[e] = fp;
-                FanRec& r = fan[fp++];
-                const int tw = h.twin[e];
-                r.cA = h.cornerA[e];
-                r.cB = tw != -1 ? h.cornerB[tw] : 0.0;
-                r.lo = h.len[e]; r.lnx = h.len[nxt(e)]; r.lpv = h.len[prv(e)];
-                r.o = e; r.twn = h.twin[nxt(e)];
-                r.nb = h.tri[nxt(e)];
-                r.tw = tw; r.nbFlag = h.flags[r.nb]; r.pad = 0;
-                e = tw != -1 ? nxt(tw) : -1;
-                if (e == start) break;
+        parallel_for(h.nv, [&](int va, int vb) {
+            for (int v = va; v < vb && manifold.load(std::memory_order_relaxed); ++v) {
+                int fp = fbase[v];
+                const int end = fbase[v + 1];
+                const int start = h.firstEdge[v];
+                int e = start;
+                while (e != -1) {
+                    // origin first: fidx of another vertex's half-edge belongs to another thread
+                    if (h.tri[e] != v || fp >= end || fidx[e] != -1) { manifold = false; break; }
+                    fidx[e] = fp;
+                    FanRec& r = fan[fp++];
+                    const int tw = h.twin[e];
+                    r.cA = h.cornerA[e];
+                    r.cB = tw != -1 ? h.cornerB[tw] : 0.0;
+                    r.lo = h.len[e]; r.lnx = h.len[nxt(e)]; r.lpv = h.len[prv(e)];
+                    r.o = e; r.twn = h.twin[nxt(e)];
+                    r.nb = h.tri[nxt(e)];
+                    r.tw = tw; r.nbFlag = h.flags[r.nb]; r.pad = 0;
+                    e = tw != -1 ? nxt(tw) : -1;
+                    if (e == start) break;
+                }
+                if (fp != end) manifold = false;
             }
-        }
-        fbase[h.nv] = fp;
-        if (manifold && fp == h.ne) {
+        });
+        if (manifold) {
             HIP_TRY(hipMalloc(&d.fan, sizeof(FanRec) * (size_t)h.ne));
             HIP_TRY(hipMalloc(&d.fanIdx, sizeof(int32_t) * (size_t)h.ne));
             HIP_TRY(hipMalloc(&d.fanBase, sizeof(int32_t) * ((size_t)h.nv + 1)));
@@ -1446,6 +1460,10 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
     if (h.ne >= (int)kEnterNone) {
         // enter edges are packed into 24 bits of a vertex-table slot: larger
         // meshes run every source on the plain path (same rows)
+        if (c.ensurePlainMesh) {
+            const std::string ue = c.ensurePlainMesh();
+            if (!ue.empty()) return ue;
+        }
         std::string e = build_plain(h, d, c, g, st, nullptr, 0);
         st.fallbackPlain = 1;
         return e;
@@ -1694,6 +1712,10 @@ std::string build_batched(const HostMesh& h, const DevMesh& d, const BuildConfig
         pc.plainBlocks = std::max(1, std::min(512, (int)((redo.size() + 127) / 128)));
         pc.plainThreads = 128;
         BuildStats ps;
+        if (c.ensurePlainMesh) {
+            const std::string ue = c.ensurePlainMesh();
+            if (!ue.empty()) return "redo of flagged sources: " + ue;
+        }
         err = build_plain(h, d, pc, g, ps, redo.data(), (int)redo.size());
         if (!err.empty()) return "redo of flagged sources: " + err;
         for (int v : redo) st.status[v] = ps.status[v] | SF_REDONE | (st.status[v] & (SF_CAPACITY | SF_REF_OVERFLOW));

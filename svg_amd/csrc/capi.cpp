@@ -104,7 +104,7 @@ int svg_mesh_upload_impl(svg_mesh_s* m, bool aos)
 {
     int rc = svg_ensure_device();
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(m->mu);
+    std::lock_guard<std::mutex> lk(m->upMu);
     std::string e = upload_mesh(m->h, m->d, aos);
     if (!e.empty()) return svg_set_error(SVG_E_HIP, e);
     return SVG_OK;
@@ -179,9 +179,15 @@ extern "C" int svg_build(svg_mesh_t m, const svg_build_params* p, svg_graph_t* o
         if (p->mode != SVG_MODE_PLAIN && p->mode != SVG_MODE_BATCHED) return svg_set_error(SVG_E_ARG, "unknown mode");
         if (p->split_coef == 0 || p->vert_coef == 0) return svg_set_error(SVG_E_ARG, "table coefficients must be positive");
         if (p->source_order < 0 || p->source_order > 2) return svg_set_error(SVG_E_ARG, "source_order must be 0, 1 or 2");
-        int rc = svg_mesh_upload_impl(m, true);
+        // the reference-layout arrays only for the plain path; a batched
+        // build uploads them if it has sources to recompute there
+        int rc = svg_mesh_upload_impl(m, p->mode == SVG_MODE_PLAIN);
         if (rc) return rc;
         BuildConfig c;
+        c.ensurePlainMesh = [m]() -> std::string {
+            const int urc = svg_mesh_upload_impl(m, true);
+            return urc ? std::string("reference-layout mesh upload: ") + svg_last_error() : std::string();
+        };
         c.K = p->K; c.splitCoef = p->split_coef; c.vertCoef = p->vert_coef; c.mode = p->mode;
         c.plainBlocks = p->plain_blocks; c.plainThreads = p->plain_threads;
         c.lanes = p->lanes; c.propBlock = p->prop_block; c.chunk = p->chunk;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -41,6 +42,9 @@ struct BuildConfig {
     int sourceOrder = 0;         // 0 auto, 1 vertex id order, 2 predicted from the mesh's saddles
     int autotune = 0;
     TuneRecord* tune = nullptr;  // autotune: what earlier builds of this mesh recorded
+    // uploads the mesh's reference-layout arrays (the plain path's) when a
+    // batched build first needs them; empty string on success
+    std::function<std::string()> ensurePlainMesh;
 };
 
 struct BuildStats {

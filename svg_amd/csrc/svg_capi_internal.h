@@ -16,7 +16,8 @@ namespace svg { struct SolverState; struct PointWork; }   // solve.h, point.h
 struct svg_mesh_s {
     svg::HostMesh h;
     svg::DevMesh d;
-    std::mutex mu;
+    std::mutex mu;                        // autotuner records
+    std::mutex upMu;                      // device upload of the mesh arrays
     std::vector<svg::TuneRecord> tunes;   // batched-build autotuner state, per (K, coefficients)
 };
 

@@ -4,6 +4,7 @@
 // Compiled with -ffp-contract=off so lengths and angle sums are the same
 // correctly rounded values on every build.
 #include "svg_mesh.h"
+#include "host_parallel.h"
 
 #include <algorithm>
 #include <cmath>
@@ -20,28 +21,6 @@ namespace svg {
 
 static const double kPi = 3.141592653589793238;   // Geometry.cuh:18
 
-// Runs body(begin, end) over [0, n) in chunks on up to 16 host threads (on
-// the calling thread alone for small n or when no thread can be started);
-// every element is computed the same way whatever the split.
-template <class F>
-static void parallel_for(int n, F&& body)
-{
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int T = (int)std::max(1u, std::min(16u, hw));
-    if (n < (1 << 16) || T == 1) { body(0, n); return; }
-    const int chunk = (n + T - 1) / T;
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) {
-        const int a = t * chunk, b = std::min(n, a + chunk);
-        if (a >= b) break;
-        try {
-            th.emplace_back([&body, a, b] { body(a, b); });
-        } catch (const std::system_error&) {
-            body(a, b);
-        }
-    }
-    for (std::thread& x : th) x.join();
-}
 
 static double clamp_acos(double c)
 {
