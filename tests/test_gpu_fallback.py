@@ -118,6 +118,21 @@ def test_chunked_build_with_redo():
     _assert_rows(g, rows, tails, K)
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_chunked_build_in_source_order(order):
+    # chunks take consecutive slices of the source order (here on fewer
+    # lanes than sources, so the predicted order is a real permutation) and,
+    # with a capacity that forces redos, still give the oracle's rows
+    xyz, tri = TORUS()
+    K = 80
+    print(f"case chunk + order {order}", flush=True)
+    rows, tails, _ = _oracle(xyz, tri, K)
+    _, g = _build(xyz, tri, K, S.SVG_MODE_BATCHED, chunk=250, lanes=128, source_order=order, vert_cap=64)
+    st = g.stats()
+    assert st["chunk"] == 250 and st["source_order"] == order and st["redone"] > 0
+    _assert_rows(g, rows, tails, K)
+
+
 def test_large_k_caps_the_vertex_table():
     # K = 2,100 asks for 4K+2 > 8,192 vertex slots, more than the
     # extraction's LDS sort holds: the table is capped at 8,192 and the
